@@ -1,0 +1,167 @@
+"""int2e_sph throughput on one MI355X (quartets/s), mixed vs plain FP64.
+
+Workload: the 256-atom synthetic molecule (8x8x4 jittered lattice, cfg-1 shell
+recipe, 2,819 spherical functions), unique 8-fold shell quartets that pass a
+Schwarz screen of 1e-12, bra pairs restricted to every `--bra-stride`-th
+significant pair so one step (one pass over that quartet set) stays within
+seconds.  Inputs (pair data, quartet lists) are resident in HBM before timing;
+outputs are written to HBM.
+
+Prints one JSON line (see the module docstring of the repository's bench
+contract): metric/value/unit, ms_per_step, roofline of the dominant launch
+set, the FP64 path's rate beside the mixed path's, the max abs error of the
+mixed results against the FP64 results, and a CPU baseline (the oracle on the
+host's cores, bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--bra-stride", type=int, default=64)
+    ap.add_argument("--threshold", type=float, default=1e-12)
+    ap.add_argument("--mixed-threshold", type=float, default=1e-4)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the integral work shards without exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from cintx_amd import capi, molecule
+    mol = molecule.config(3)
+    log(f"molecule: {len(mol.coords)} atoms, {mol.nbas} shells, {mol.nao()} functions")
+
+    t0 = time.time()
+    p64 = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env,
+                    screened=(args.threshold, args.bra_stride), precision=capi.PREC_FP64)
+    log(f"fp64 plan: {p64.ntuples} quartets, {p64.nelem} doubles ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    pmx = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env,
+                    screened=(args.threshold, args.bra_stride), precision=capi.PREC_MIXED,
+                    mixed_threshold=args.mixed_threshold)
+    log(f"mixed plan: {pmx.ntuples} quartets ({time.time() - t0:.1f}s)")
+
+    def timed(plan, label):
+        for _ in range(args.warmup):
+            plan.execute(graph=True)
+            plan.sync()
+        ms = []
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            plan.execute(graph=True)
+            plan.sync()
+            ms.append(plan.last_ms())
+        wall = (time.perf_counter() - t_start) * 1e3 / args.steps
+        log(f"{label}: events {np.mean(ms):.2f} ms/step, wall {wall:.2f} ms/step")
+        return float(np.mean(ms)), wall
+
+    ms64, wall64 = timed(p64, "fp64")
+    msmx, wallmx = timed(pmx, "mixed")
+
+    # accuracy of the mixed path against the FP64 path, tuple by tuple
+    out64 = p64.output()
+    outmx = pmx.output()
+    t64, o64 = p64.tuples()
+    tmx, omx = pmx.tuples()
+    key64 = {tuple(t): o for t, o in zip(map(tuple, t64), o64)}
+    sizes = np.diff(np.append(omx, len(outmx)))
+    err = 0.0
+    for t, o, s in zip(map(tuple, tmx), omx, sizes):
+        a = outmx[o:o + s]
+        b = out64[key64[t]:key64[t] + s]
+        err = max(err, float(np.abs(a - b).max()) if s else 0.0)
+    log(f"mixed vs fp64 max abs err {err:.3e}")
+
+    prim64, roots64, flops64 = p64.work()
+    step_s = ms64 / 1e3
+    achieved = flops64 / step_s / 1e12
+    value = pmx.ntuples / (msmx / 1e3)
+    fp64_rate = p64.ntuples / step_s
+    out_bytes = p64.nelem * 8
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(mol, t64, args.cpu_seconds)
+
+    res = {
+        "metric": "int2e_sph quartets/s: mixed vs FP64 GPU at <=1e-10 abs err, 1 MI355X",
+        "value": value,
+        "unit": "quartets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": msmx,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64+f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "int2e_sph, 256-atom 8x8x4 lattice (cfg-1 shells, 2819 sph functions), "
+                        f"unique quartets, Schwarz 1e-12, bra stride {args.bra_stride}",
+            "quartets_per_step": pmx.ntuples,
+            "mixed_threshold": args.mixed_threshold,
+        },
+        "fp64": {"quartets_per_s": fp64_rate, "ms_per_step": ms64,
+                 "wall_ms_per_step": wall64, "gflops": achieved * 1e3,
+                 "prim_quartets": prim64, "root_evals": roots64},
+        "mixed_over_fp64": value / fp64_rate,
+        "mixed_max_abs_err_vs_fp64": err,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "note": "FP64 VALU-bound class kernels (FP64 vector and matrix peaks are "
+                             "both 78.6 TFLOP/s); achieved = algorithmic FLOPs of the FP64 "
+                             "step / event time of the step",
+                     "output_GBps": out_bytes / step_s / 1e9},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+def cpu_baseline(mol, tuples, seconds):
+    """The oracle (C restatement, OpenMP over tuples) on a bounded sample."""
+    from oracle import oracle as O
+    ncores = os.cpu_count() or 1
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(tuples))
+    n = 2000
+    t0 = time.perf_counter()
+    done = 0
+    while True:
+        sub = [tuple(int(v) for v in tuples[q]) for q in order[done:done + n]]
+        if not sub:
+            break
+        O.eval_list("int2e_sph", sub, mol.atm, mol.bas, mol.env, nthreads=ncores)
+        done += len(sub)
+        el = time.perf_counter() - t0
+        if el > seconds:
+            break
+        n = min(n * 2, 200000)
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "quartets/s", "cores": ncores, "kind": "port",
+            "sample": f"{done} random quartets of the same screened set, {el:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

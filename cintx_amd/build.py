@@ -1,0 +1,147 @@
+"""Build libcintx_hip.so (gfx950) in-tree.
+
+Generates the class-kernel instantiation units, compiles them with hipcc in
+parallel and links the shared library next to this file.  Run as
+``python -m cintx_amd.build`` or through ``__graft_entry__.build()``.
+"""
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+GEN = os.path.join(BUILD, "gen")
+LIB = os.path.join(HERE, "libcintx_hip.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-I", CSRC,
+]
+
+
+def class_sets():
+    """(family_deriv, precision, (li, lj, lk, ll)) instantiated as templates."""
+    out = []
+    pairs = [(a, b) for a in range(3) for b in range(3) if a >= b]
+    # int2e_sph: canonical li>=lj, lk>=ll, (li,lj)>=(lk,ll), l<=2; FP64 and FP32
+    for bra in pairs:
+        for ket in pairs:
+            if bra >= ket:
+                for prec in (0, 1):
+                    out.append((0, prec, bra + ket))
+    # int2e_ip1_sph: derivative on i, only k<->l canonicalised
+    for li in range(3):
+        for lj in range(3):
+            for ket in pairs:
+                out.append((1, 0, (li, lj) + ket))
+    # int3c2e_sph: li>=lj <= 2, aux l <= 4 in the k slot, FP64 and FP32
+    for bra in pairs:
+        for lk in range(5):
+            for prec in (0, 1):
+                out.append((0, prec, bra + (lk, 0)))
+    # int3c2e_ip1_sph
+    for li in range(3):
+        for lj in range(3):
+            for lk in range(5):
+                out.append((1, 0, (li, lj, lk, 0)))
+    # 3c2e (ij|k) with LL == 0 coincides with int2e classes (ij|k s); dedupe
+    seen = set()
+    uniq = []
+    for c in out:
+        if c not in seen:
+            seen.add(c)
+            uniq.append(c)
+    return uniq
+
+
+def generate(nparts=24):
+    os.makedirs(GEN, exist_ok=True)
+    classes = class_sets()
+    parts = [classes[i::nparts] for i in range(nparts)]
+    files = []
+    for p, cls in enumerate(parts):
+        lines = ['#include "eri_device.h"', "namespace cx {",
+                 f"void register_part_{p}(Registry& reg) {{"]
+        for deriv, prec, (a, b, c, d) in cls:
+            real = "double" if prec == 0 else "float"
+            lines.append(f"  reg.k[{deriv}][{prec}][{a}][{b}][{c}][{d}] = "
+                         f"class_info<{a}, {b}, {c}, {d}, {deriv}, {real}>();")
+        lines += ["}", "}  // namespace cx", ""]
+        path = os.path.join(GEN, f"inst_{p:02d}.hip")
+        text = "\n".join(lines)
+        if not os.path.exists(path) or open(path).read() != text:
+            with open(path, "w") as f:
+                f.write(text)
+        files.append(path)
+    reg = ['#include "launch.h"', "namespace cx {"]
+    reg += [f"void register_part_{p}(Registry& reg);" for p in range(nparts)]
+    reg += ["void register_class_kernels(Registry& reg) {"]
+    reg += [f"  register_part_{p}(reg);" for p in range(nparts)]
+    reg += ["}", "}  // namespace cx", ""]
+    path = os.path.join(GEN, "inst_registry.cpp")
+    text = "\n".join(reg)
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as f:
+            f.write(text)
+    files.append(path)
+    return files
+
+
+def _compile(src, obj, extra):
+    deps = [src, os.path.join(CSRC, "eri_device.h"), os.path.join(CSRC, "common.h"),
+            os.path.join(CSRC, "launch.h"), os.path.join(CSRC, "rys_table.h"),
+            os.path.join(HERE, "..", "include", "cintx_hip.h")]
+    if os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if all(os.path.getmtime(d) <= t for d in deps if os.path.exists(d)):
+            return obj, None
+    cmd = [HIPCC] + COMMON_FLAGS + extra + ["-c", src, "-o", obj]
+    if src.endswith(".cpp"):
+        cmd = [HIPCC] + COMMON_FLAGS + extra + ["-x", "hip", "-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        return obj, (" ".join(cmd)) + "\n" + r.stdout + r.stderr
+    return obj, None
+
+
+def build(jobs=None, verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    gen = generate()
+    srcs = [os.path.join(CSRC, "runtime.cpp"), os.path.join(CSRC, "rys_table.cpp"),
+            os.path.join(CSRC, "rys_kernels.hip")] + gen
+    extra = {os.path.join(CSRC, "rys_table.cpp"): ["-fopenmp"]}
+    jobs = jobs or min(16, os.cpu_count() or 4)
+    objs = []
+    errors = []
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        futs = []
+        for s in srcs:
+            obj = os.path.join(BUILD, os.path.basename(s) + ".o")
+            futs.append(ex.submit(_compile, s, obj, extra.get(s, [])))
+        for f in futs:
+            obj, err = f.result()
+            objs.append(obj)
+            if err:
+                errors.append(err)
+    if errors:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-fopenmp", "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB} ({len(class_sets())} class kernels)")
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

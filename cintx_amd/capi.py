@@ -1,0 +1,291 @@
+"""Python mirror of the reference's raw operator interface over libcintx_hip.so.
+
+Three layers, matching include/cintx_hip.h:
+
+* ``int2e_sph(shls, atm, bas, env, dims=None)`` & co. -- libcint raw
+  signature (libcint-master/src/cint2e.c:1186), returning (out, not0);
+* ``eval_raw(api, out, dims, shls, atm, bas, env)`` -- cintx-compat's
+  eval_raw contract (crates/cintx-compat/src/raw.rs:874) through the C shim
+  ``cintrs_eval`` (crates/cintx-capi/src/shim.rs:371), raising CintxError
+  with the shim's status codes;
+* ``Plan`` -- the batched executor boundary (device-resident basis, class
+  bins, device output).
+
+The library is loaded from this directory.  There is no fallback: if the HIP
+library is missing, importing the symbols raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcintx_hip.so")
+
+# crates/cintx-capi/src/shim.rs:8-49 (+ this library's int3c2e_sph id)
+INT2E_SPH = 10
+INT3C2E_IP1_SPH = 19
+INT2E_IP1_SPH = 36
+INT3C2E_SPH = 40
+
+FAMILY = {"int2e_sph": 0, "int2e_ip1_sph": 1, "int3c2e_sph": 2, "int3c2e_ip1_sph": 3}
+API_OF_FAMILY = {0: INT2E_SPH, 1: INT2E_IP1_SPH, 2: INT3C2E_SPH, 3: INT3C2E_IP1_SPH}
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+STATUS = {
+    0: "Success", 1: "InvalidInput", 2: "UnsupportedApi", 3: "UnsupportedRepresentation",
+    4: "BufferTooSmall", 5: "MemoryLimitExceeded", 6: "AllocationFailed",
+    7: "ExecutionFailed", 8: "NullPointer", 9: "Panic", 10: "BackendNotCompiled",
+}
+
+EXPORTS = [
+    "int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph",
+    "int2e_optimizer", "int2e_ip1_optimizer", "int3c2e_optimizer", "int3c2e_ip1_optimizer",
+    "CINTdel_optimizer",
+    "cintrs_query_workspace", "cintrs_eval", "cintrs_last_error_code",
+    "cintrs_clear_last_error", "cintrs_copy_last_error_message",
+    "cintx_plan_create", "cintx_plan_create_screened", "cintx_plan_destroy",
+    "cintx_plan_sizes", "cintx_plan_tuples", "cintx_plan_execute", "cintx_plan_sync",
+    "cintx_plan_copy_output", "cintx_plan_execute_graph", "cintx_plan_last_time_ms",
+    "cintx_plan_work", "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
+]
+
+
+class CintxError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+class WorkspaceQuery(ctypes.Structure):
+    _fields_ = [("bytes", ctypes.c_size_t), ("alignment", ctypes.c_size_t),
+                ("required_bytes", ctypes.c_size_t), ("chunk_count", ctypes.c_size_t),
+                ("work_units", ctypes.c_size_t)]
+
+
+class EvalSummary(ctypes.Structure):
+    _fields_ = [("not0", ctypes.c_int32), ("bytes_written", ctypes.c_size_t),
+                ("workspace_bytes", ctypes.c_size_t)]
+
+
+_lib = None
+DP = ctypes.POINTER(ctypes.c_double)
+IP = ctypes.POINTER(ctypes.c_int)
+I64P = ctypes.POINTER(ctypes.c_int64)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run python -m cintx_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        raw = [DP, IP, IP, IP, ctypes.c_int, IP, ctypes.c_int, DP, ctypes.c_void_p, DP]
+        for n in ("int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph"):
+            getattr(L, n).argtypes = raw
+            getattr(L, n).restype = ctypes.c_int
+        sz = ctypes.c_size_t
+        L.cintrs_eval.argtypes = [ctypes.c_int, DP, sz, IP, sz, IP, sz, IP, sz, IP, sz, DP, sz,
+                                  ctypes.c_void_p, DP, sz, ctypes.POINTER(EvalSummary)]
+        L.cintrs_query_workspace.argtypes = [ctypes.c_int, IP, sz, IP, sz, IP, sz, IP, sz, DP,
+                                             sz, ctypes.c_void_p,
+                                             ctypes.POINTER(WorkspaceQuery)]
+        L.cintrs_last_error_code.restype = ctypes.c_int
+        L.cintrs_copy_last_error_message.argtypes = [ctypes.c_char_p, sz]
+        L.cintrs_copy_last_error_message.restype = sz
+        vp = ctypes.c_void_p
+        L.cintx_plan_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, IP, ctypes.c_int, IP,
+                                        ctypes.c_int, DP, IP, ctypes.c_int64, ctypes.c_int,
+                                        ctypes.c_double]
+        L.cintx_plan_create_screened.argtypes = [ctypes.POINTER(vp), IP, ctypes.c_int, IP,
+                                                 ctypes.c_int, DP, ctypes.c_double, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_double]
+        L.cintx_plan_destroy.argtypes = [vp]
+        L.cintx_plan_sizes.argtypes = [vp, I64P, I64P]
+        L.cintx_plan_tuples.argtypes = [vp, IP, I64P]
+        L.cintx_plan_execute.argtypes = [vp, DP]
+        L.cintx_plan_execute_graph.argtypes = [vp, DP]
+        L.cintx_plan_sync.argtypes = [vp]
+        L.cintx_plan_copy_output.argtypes = [vp, DP]
+        L.cintx_plan_last_time_ms.argtypes = [vp, DP]
+        L.cintx_plan_work.argtypes = [vp, DP, DP, DP]
+        L.cintx_device_count.argtypes = [IP]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(DP)
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(IP)
+
+
+def _arrays(atm, bas, env):
+    return (np.ascontiguousarray(atm, dtype=np.int32), np.ascontiguousarray(bas, dtype=np.int32),
+            np.ascontiguousarray(env, dtype=np.float64))
+
+
+def last_error():
+    L = lib()
+    buf = ctypes.create_string_buffer(4096)
+    L.cintrs_copy_last_error_message(buf, 4096)
+    return L.cintrs_last_error_code(), buf.value.decode()
+
+
+def _check(st):
+    if st != 0:
+        code, msg = last_error()
+        raise CintxError(st, msg)
+
+
+def natural_dims(family, shls, bas):
+    arity = 3 if family.startswith("int3c2e") else 4
+    return [(2 * int(bas[s][1]) + 1) * int(bas[s][3]) for s in shls[:arity]]
+
+
+def ncomp(family):
+    return 3 if "ip1" in family else 1
+
+
+def raw_call(family, shls, atm, bas, env, dims=None):
+    """libcint raw signature: returns (out, not0)."""
+    atm, bas, env = _arrays(atm, bas, env)
+    d = natural_dims(family, shls, bas) if dims is None else list(dims)
+    out = np.zeros(int(np.prod(d)) * ncomp(family))
+    sh = np.ascontiguousarray(list(shls), dtype=np.int32)
+    dd = None if dims is None else np.ascontiguousarray(dims, dtype=np.int32)
+    fn = getattr(lib(), family)
+    not0 = fn(_dp(out), _ip(dd), _ip(sh), _ip(atm), len(atm), _ip(bas), len(bas), _dp(env),
+              None, None)
+    return out, not0
+
+
+def cache_size(family, shls, atm, bas, env):
+    atm, bas, env = _arrays(atm, bas, env)
+    sh = np.ascontiguousarray(list(shls), dtype=np.int32)
+    return getattr(lib(), family)(None, None, _ip(sh), _ip(atm), len(atm), _ip(bas), len(bas),
+                                  _dp(env), None, None)
+
+
+def int2e_sph(shls, atm, bas, env, dims=None):
+    return raw_call("int2e_sph", shls, atm, bas, env, dims)
+
+
+def int2e_ip1_sph(shls, atm, bas, env, dims=None):
+    return raw_call("int2e_ip1_sph", shls, atm, bas, env, dims)
+
+
+def int3c2e_sph(shls, atm, bas, env, dims=None):
+    return raw_call("int3c2e_sph", shls, atm, bas, env, dims)
+
+
+def int3c2e_ip1_sph(shls, atm, bas, env, dims=None):
+    return raw_call("int3c2e_ip1_sph", shls, atm, bas, env, dims)
+
+
+def eval_raw(api, out, dims, shls, atm, bas, env, cache=None):
+    """cintx-compat eval_raw through cintrs_eval; returns the summary dict."""
+    atm, bas, env = _arrays(atm, bas, env)
+    sh = np.ascontiguousarray(shls, dtype=np.int32)
+    dd = None if dims is None else np.ascontiguousarray(dims, dtype=np.int32)
+    s = EvalSummary()
+    st = lib().cintrs_eval(api, _dp(out), 0 if out is None else out.size, _ip(dd),
+                           0 if dd is None else dd.size, _ip(sh), sh.size, _ip(atm), atm.size,
+                           _ip(bas), bas.size, _dp(env), env.size, None, _dp(cache),
+                           0 if cache is None else cache.size, ctypes.byref(s))
+    _check(st)
+    return {"not0": s.not0, "bytes_written": s.bytes_written,
+            "workspace_bytes": s.workspace_bytes}
+
+
+def query_workspace_raw(api, dims, shls, atm, bas, env):
+    atm, bas, env = _arrays(atm, bas, env)
+    sh = np.ascontiguousarray(shls, dtype=np.int32)
+    dd = None if dims is None else np.ascontiguousarray(dims, dtype=np.int32)
+    q = WorkspaceQuery()
+    st = lib().cintrs_query_workspace(api, _ip(dd), 0 if dd is None else dd.size, _ip(sh),
+                                      sh.size, _ip(atm), atm.size, _ip(bas), bas.size, _dp(env),
+                                      env.size, None, ctypes.byref(q))
+    _check(st)
+    return {"bytes": q.bytes, "required_bytes": q.required_bytes}
+
+
+class Plan:
+    """Batched executor: device-resident basis/pairs, class-binned tuples."""
+
+    def __init__(self, family, atm, bas, env, tuples=None, precision=PREC_FP64,
+                 mixed_threshold=0.0, screened=None):
+        self._atm, self._bas, self._env = _arrays(atm, bas, env)
+        self.family = family
+        h = ctypes.c_void_p()
+        L = lib()
+        if screened is not None:
+            thr, stride = screened
+            st = L.cintx_plan_create_screened(ctypes.byref(h), _ip(self._atm), len(self._atm),
+                                              _ip(self._bas), len(self._bas), _dp(self._env),
+                                              float(thr), int(stride), precision,
+                                              float(mixed_threshold))
+        else:
+            t = np.zeros((len(tuples), 4), dtype=np.int32)
+            for i, s in enumerate(tuples):
+                t[i, :len(s)] = s
+            st = L.cintx_plan_create(ctypes.byref(h), FAMILY[family], _ip(self._atm),
+                                     len(self._atm), _ip(self._bas), len(self._bas),
+                                     _dp(self._env), _ip(t), len(t), precision,
+                                     float(mixed_threshold))
+        _check(st)
+        self.h = h
+        n = ctypes.c_int64()
+        nt = ctypes.c_int64()
+        _check(L.cintx_plan_sizes(h, ctypes.byref(n), ctypes.byref(nt)))
+        self.nelem = n.value
+        self.ntuples = nt.value
+
+    def tuples(self):
+        t = np.zeros((self.ntuples, 4), dtype=np.int32)
+        off = np.zeros(self.ntuples, dtype=np.int64)
+        _check(lib().cintx_plan_tuples(self.h, _ip(t), off.ctypes.data_as(I64P)))
+        return t, off
+
+    def execute(self, graph=False):
+        L = lib()
+        _check((L.cintx_plan_execute_graph if graph else L.cintx_plan_execute)(self.h, None))
+
+    def sync(self):
+        _check(lib().cintx_plan_sync(self.h))
+
+    def last_ms(self):
+        v = ctypes.c_double()
+        _check(lib().cintx_plan_last_time_ms(self.h, ctypes.byref(v)))
+        return v.value
+
+    def output(self):
+        out = np.zeros(self.nelem)
+        _check(lib().cintx_plan_copy_output(self.h, _dp(out)))
+        return out
+
+    def work(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(lib().cintx_plan_work(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().cintx_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_count():
+    n = ctypes.c_int()
+    _check(lib().cintx_device_count(ctypes.byref(n)))
+    return n.value

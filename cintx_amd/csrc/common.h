@@ -1,0 +1,73 @@
+// Shared host/device definitions for libcintx_hip.
+//
+// Data layout in HBM (all device-resident, built once per basis):
+//   shells      : virtual (segmented, nctr == 1) shells; general contractions of
+//                 the caller's basis are split into one virtual shell per
+//                 contraction column, which is exact by linearity.
+//   pairs       : one record per shell pair (i, j) that a plan needs.
+//   prim pairs  : structure-of-arrays over all primitive pairs of all pairs,
+//                 contiguous per pair, ordered jp-major/ip-minor like
+//                 CINTset_pairdata (libcint-master/src/optimizer.c:317-341).
+//   quartets    : (bra pair, ket pair) int32 tuples, binned by class.
+#pragma once
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+namespace cx {
+
+constexpr int kMaxL = 4;          // largest shell l on the hot path (g)
+constexpr int kMaxRoots = 14;     // table covers n = 1 .. kMaxRoots
+constexpr int kRysDeg = 11;       // Chebyshev degree per unit x-interval
+constexpr int kRysNc = kRysDeg + 1;
+constexpr int kC2sL = 6;          // c2s table covers l = 0 .. kC2sL
+constexpr int kC2sM = 2 * kC2sL + 1;
+constexpr int kC2sC = (kC2sL + 1) * (kC2sL + 2) / 2;
+
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+__host__ __device__ constexpr int nsph(int l) { return 2 * l + 1; }
+__host__ __device__ constexpr double rys_largex(int n) { return 35.0 + 5.0 * n; }
+
+// Rys table: for n in 1..kMaxRoots, x in [0, largex(n)) split into unit
+// intervals; per interval, per root r: kRysNc Chebyshev coefficients of u_r
+// followed (after all n roots) by those of w_r.  For x >= largex(n) the
+// asymptotic rule u = R/(x-R), w = W/sqrt(x) is used.
+struct RysTable {
+  const double* coef;   // device
+  const double* large;  // device: per n at n(n-1): R[0..n-1], W[0..n-1]
+  int off[kMaxRoots + 1];  // start of n's block in coef
+};
+
+// Per-launch arguments of a class kernel.
+struct EriArgs {
+  // primitive-pair SoA
+  const double* aij;
+  const double* px;
+  const double* py;
+  const double* pz;
+  const double* kfac;   // exp(-a_i a_j/a_ij |R_ij|^2) * c_i * c_j
+  const double* ai;     // exponent of the first shell (nabla_i)
+  const double* cce;    // screening key without the l-dependent term
+  // per pair
+  const int* pair_off;  // first prim pair of the pair
+  const double* pair_geo;  // 8 per pair: ri[3], rj[3], log(|Rij|+1), unused
+  // work list
+  const int2* quartets;    // (bra pair, ket pair)
+  const int64_t* out_off;  // output offset (elements) of each quartet block
+  int64_t nq;
+  int npij, npkl;          // primitive pairs per bra / ket pair (uniform)
+  double common;           // pi^3 2/sqrt(pi) prod common_fac_sp
+  double expcutoff;
+  int lij_ceil, lkl_ceil;  // li_ceil+lj_ceil, lk_ceil+ll_ceil for the screen
+  int three_center;        // ket pairs are pseudo pairs (no kl screen)
+  // output strides for the four canonical axes and the component
+  int64_t st[4];
+  int64_t st_comp;
+  double* out;
+  RysTable rys;
+  const double* c2s;       // [kC2sL+1][kC2sM][kC2sC]
+  double* scratch;         // generic kernel only
+  unsigned long long* counters;  // optional: [0] surviving primitive tuples
+};
+
+}  // namespace cx

@@ -1,0 +1,462 @@
+// Device code of the Rys two-electron class kernels (gfx950 / CDNA4).
+//
+// One template covers int2e, int2e_ip1, int3c2e and int3c2e_ip1 for a
+// compile-time angular-momentum class (LI, LJ, LK, LL):
+//   * a workgroup holds Q shell tuples, T lanes per tuple (T | 64, so a tuple
+//     never straddles a wavefront);
+//   * per primitive tuple: the T lanes evaluate the NR Rys roots from the
+//     Chebyshev table (lane r -> root r), then the 3*NR (axis, root) recurrence
+//     chains (2D VRR + bra/ket HRR, registers only) and store the 4D g tensor
+//     in LDS as g[axis][i][j][k][l][root], then every lane accumulates its
+//     Cartesian elements sum_r gx*gy*gz in registers;
+//   * after the primitive loops the Cartesian block is moved to LDS, turned
+//     into real spherical harmonics axis by axis, and written with caller
+//     strides (so a canonicalised tuple lands in the requested orientation).
+// Arithmetic follows libcint: per-root coefficients g2e.c:4523-4543, VRR
+// g2e.c:272-411, HRR g2e.c:428-690 (ibase/kbase choice g2e.c:102-150), gout
+// cint2e.c:961-1180, nabla_i g2e.c:4574-4620, screening cint2e.c:199-236.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "launch.h"
+
+namespace cx {
+
+__host__ __device__ constexpr int cmax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
+
+// libcint Cartesian order (cint_bas.c:166-180): lx from l down, then ly from
+// l-lx down; within an lx group the position equals lz.
+__host__ __device__ inline void cart_decode(int l, int c, int& lx, int& ly, int& lz) {
+  int start = 0;
+  for (int x = l; x >= 0; --x) {
+    int cnt = l - x + 1;
+    if (c < start + cnt) {
+      lx = x;
+      lz = c - start;
+      ly = l - x - lz;
+      return;
+    }
+    start += cnt;
+  }
+  lx = ly = lz = 0;
+}
+
+// Rys roots/weights for n roots, root r, argument x (rys_roots.c:57-123
+// regimes: Chebyshev table below 35+5n, asymptotic Hermite rule above).
+__device__ __forceinline__ void rys_eval(const RysTable& tab, int n, int r, double x, double& u,
+                                         double& w) {
+  if (!(x < rys_largex(n))) {
+    const double* L = tab.large + n * (n - 1);
+    double R = L[r];
+    u = R / (x - R);
+    w = L[n + r] / sqrt(x);
+    return;
+  }
+  int iv = (int)x;
+  double t = 2.0 * (x - (double)iv) - 1.0;
+  double t2 = t + t;
+  const double* cu = tab.coef + tab.off[n] + ((size_t)iv * 2 * n + r) * kRysNc;
+  const double* cw = cu + (size_t)n * kRysNc;
+  double bu1 = 0, bu2 = 0, bw1 = 0, bw2 = 0;
+#pragma unroll
+  for (int k = kRysDeg; k >= 1; --k) {
+    double bu0 = fma(t2, bu1, cu[k] - bu2);
+    double bw0 = fma(t2, bw1, cw[k] - bw2);
+    bu2 = bu1;
+    bu1 = bu0;
+    bw2 = bw1;
+    bw1 = bw0;
+  }
+  u = fma(t, bu1, cu[0] - bu2);
+  w = fma(t, bw1, cw[0] - bw2);
+}
+
+// Per-tuple scalars of one primitive tuple.
+struct PrimTuple {
+  double aij, akl, a0, a1;
+  double dx[3];   // P - Q
+  double prb[3];  // P - R(bra base)
+  double qrk[3];  // Q - R(ket base)
+  double ai;      // exponent on shell i (nabla_i)
+};
+
+// One (axis, root) recurrence chain: 2D VRR then bra and ket HRR; writes
+// g[((axis*DI + i)*DJ + j)*DK + k)*DL + l)*NR + r].
+template <int LIc, int LJ, int LK, int LL, typename R>
+__device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, double u,
+                                            double wz, const PrimTuple& p, double rij_ax,
+                                            double rkl_ax) {
+  constexpr int NR = (LIc + LJ + LK + LL) / 2 + 1;
+  constexpr int NMAX = LIc + LJ, MMAX = LK + LL;
+  constexpr bool IBASE = LIc > LJ, KBASE = LK > LL;
+  constexpr int DI = LIc + 1, DJ = LJ + 1, DK = LK + 1, DL = LL + 1;
+  constexpr int A = IBASE ? LIc : LJ, B = IBASE ? LJ : LIc;
+  constexpr int C = KBASE ? LK : LL, D = KBASE ? LL : LK;
+
+  const double u2 = p.a0 * u;
+  const double tmp4 = 0.5 / (u2 * (p.aij + p.akl) + p.a1);
+  const double tmp5 = u2 * tmp4;
+  const R b00 = (R)tmp5;
+  const R b10 = (R)(tmp5 + tmp4 * p.akl);
+  const R b01 = (R)(tmp5 + tmp4 * p.aij);
+  const R c00 = (R)(p.prb[axis] - 2.0 * tmp5 * p.akl * p.dx[axis]);
+  const R c0p = (R)(p.qrk[axis] + 2.0 * tmp5 * p.aij * p.dx[axis]);
+
+  R g2[NMAX + 1][MMAX + 1];
+  g2[0][0] = (axis == 2) ? (R)wz : (R)1;
+#pragma unroll
+  for (int n = 0; n < NMAX; ++n) {
+    R v = c00 * g2[n][0];
+    if (n > 0) v += (R)n * b10 * g2[n - 1][0];
+    g2[n + 1][0] = v;
+  }
+#pragma unroll
+  for (int m = 0; m < MMAX; ++m) {
+#pragma unroll
+    for (int n = 0; n <= NMAX; ++n) {
+      R v = c0p * g2[n][m];
+      if (m > 0) v += (R)m * b01 * g2[n][m - 1];
+      if (n > 0) v += (R)n * b00 * g2[n - 1][m];
+      g2[n][m + 1] = v;
+    }
+  }
+  // bra HRR: H(a, b+1) = H(a+1, b) + dr H(a, b) on the base/other powers.
+  const R drb = (R)rij_ax;
+  R h1[DI][DJ][MMAX + 1];
+#pragma unroll
+  for (int m = 0; m <= MMAX; ++m) {
+    R t[A + B + 1][B + 1];
+#pragma unroll
+    for (int a = 0; a <= A + B; ++a) t[a][0] = g2[a][m];
+#pragma unroll
+    for (int b = 0; b < B; ++b)
+#pragma unroll
+      for (int a = 0; a < A + B - b; ++a) t[a][b + 1] = t[a + 1][b] + drb * t[a][b];
+#pragma unroll
+    for (int a = 0; a <= A; ++a)
+#pragma unroll
+      for (int b = 0; b <= B; ++b) {
+        if (IBASE)
+          h1[a][b][m] = t[a][b];
+        else
+          h1[b][a][m] = t[a][b];
+      }
+  }
+  const R drk = (R)rkl_ax;
+  R* __restrict__ ga = g + (size_t)axis * DI * DJ * DK * DL * NR + r;
+#pragma unroll
+  for (int i = 0; i < DI; ++i)
+#pragma unroll
+    for (int j = 0; j < DJ; ++j) {
+      R t[C + D + 1][D + 1];
+#pragma unroll
+      for (int c = 0; c <= C + D; ++c) t[c][0] = h1[i][j][c];
+#pragma unroll
+      for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int c = 0; c < C + D - d; ++c) t[c][d + 1] = t[c + 1][d] + drk * t[c][d];
+#pragma unroll
+      for (int c = 0; c <= C; ++c)
+#pragma unroll
+        for (int d = 0; d <= D; ++d) {
+          const int k = KBASE ? c : d;
+          const int l = KBASE ? d : c;
+          ga[(((i * DJ + j) * DK + k) * DL + l) * NR] = t[c][d];
+        }
+    }
+}
+
+// Apply the real-spherical transform along one axis of a [l][k][j][i] block
+// (i fastest) held in LDS: src dims (n0,n1,n2,n3), axis `ax` goes from
+// ncart(lax) to nsph(lax) entries.
+__device__ __forceinline__ void c2s_pass(double* __restrict__ dst, const double* __restrict__ src,
+                                         const int* dims, int ax, int lax,
+                                         const double* __restrict__ c2s, int t, int T) {
+  int inner = 1, outer = 1;
+  for (int b = 0; b < ax; ++b) inner *= dims[b];
+  for (int b = ax + 1; b < 4; ++b) outer *= dims[b];
+  const int nc = dims[ax], ns = nsph(lax);
+  const int total = outer * ns * inner;
+  const double* tab = c2s + (size_t)lax * kC2sM * kC2sC;
+  for (int o = t; o < total; o += T) {
+    int in = o % inner;
+    int rest = o / inner;
+    int s = rest % ns;
+    int ou = rest / ns;
+    const double* row = tab + s * kC2sC;
+    const double* sp = src + (size_t)ou * nc * inner + in;
+    double acc = 0;
+    for (int c = 0; c < nc; ++c) {
+      double cf = row[c];
+      if (cf != 0.0) acc += cf * sp[(size_t)c * inner];
+    }
+    dst[o] = acc;
+  }
+}
+
+// Lanes per tuple and tuples per workgroup for a class.
+template <int LI, int LJ, int LK, int LL, int DERIV>
+struct ClassShape {
+  static constexpr int LIc = LI + DERIV;
+  static constexpr int NR = (LIc + LJ + LK + LL) / 2 + 1;
+  static constexpr int DI = LIc + 1, DJ = LJ + 1, DK = LK + 1, DL = LL + 1;
+  static constexpr int G4 = DI * DJ * DK * DL;
+  static constexpr int NF = ncart(LI) * ncart(LJ) * ncart(LK) * ncart(LL);
+  static constexpr int NCOMP = DERIV ? 3 : 1;
+  static constexpr int NE = NF * NCOMP;
+  static constexpr int T = NE <= 8 ? 1 : (NE <= 48 ? 4 : (NE <= 192 ? 16 : 64));
+  static constexpr int NFT = (NF + T - 1) / T;
+  template <typename R>
+  static constexpr int lds_per_tuple() {
+    return (int)(3 * G4 * NR * sizeof(R) + 2 * NR * sizeof(double) + 2 * NF * sizeof(double));
+  }
+  template <typename R>
+  static constexpr int Q() {
+    int q = 256 / T;
+    while (q > 1 && q * lds_per_tuple<R>() > 60 * 1024) q /= 2;
+    return q;
+  }
+};
+
+template <int LI, int LJ, int LK, int LL, int DERIV, typename R>
+__global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
+                                   ClassShape<LI, LJ, LK, LL, DERIV>::template Q<R>()))
+    eri_class_kernel(EriArgs a) {
+  using S = ClassShape<LI, LJ, LK, LL, DERIV>;
+  constexpr int T = S::T;
+  constexpr int Q = S::template Q<R>();
+  constexpr int LIc = S::LIc;
+  constexpr int NR = S::NR;
+  constexpr int DI = S::DI, DJ = S::DJ, DK = S::DK, DL = S::DL;
+  constexpr int G4 = S::G4;
+  constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
+  constexpr int NF = S::NF;
+  constexpr int NCOMP = S::NCOMP;
+  constexpr int NE = S::NE;
+  constexpr int NFT = S::NFT;
+  constexpr bool IBASE = LIc > LJ, KBASE = LK > LL;
+  constexpr int GS = 3 * G4 * NR;
+
+  __shared__ R sg[Q][GS];
+  __shared__ double suw[Q][2 * NR];
+  __shared__ double sbuf[Q][2][NF];
+
+  const int slot = threadIdx.x / T;
+  const int t = threadIdx.x - slot * T;
+  const int64_t q = (int64_t)blockIdx.x * Q + slot;
+  const bool active = q < a.nq;
+  const int2 pq = a.quartets[active ? q : 0];
+  const int boff = a.pair_off[pq.x];
+  const int koff = a.pair_off[pq.y];
+  const double* gb = a.pair_geo + 8 * (size_t)pq.x;
+  const double* gk = a.pair_geo + 8 * (size_t)pq.y;
+  const double ri[3] = {gb[0], gb[1], gb[2]};
+  const double rj[3] = {gb[3], gb[4], gb[5]};
+  const double rk[3] = {gk[0], gk[1], gk[2]};
+  const double rl[3] = {gk[3], gk[4], gk[5]};
+  const double logd_ij = gb[6], logd_kl = gk[6];
+  double rij[3], rkl[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    rij[d] = IBASE ? ri[d] - rj[d] : rj[d] - ri[d];
+    rkl[d] = KBASE ? rk[d] - rl[d] : rl[d] - rk[d];
+  }
+
+  // packed LDS positions of this lane's elements: x | y << 10 | z << 20
+  int pos[NFT];
+#pragma unroll
+  for (int kk = 0; kk < NFT; ++kk) {
+    int e = t + kk * T;
+    if (e >= NF) e = 0;
+    int fi = e % NFI, e1 = e / NFI;
+    int fj = e1 % NFJ, e2 = e1 / NFJ;
+    int fk = e2 % NFK, fl = e2 / NFK;
+    int ix, iy, iz, jx, jy, jz, kx, ky, kz, lx, ly, lz;
+    cart_decode(LI, fi, ix, iy, iz);
+    cart_decode(LJ, fj, jx, jy, jz);
+    cart_decode(LK, fk, kx, ky, kz);
+    cart_decode(LL, fl, lx, ly, lz);
+    int px = ((ix * DJ + jx) * DK + kx) * DL + lx;
+    int py = ((iy * DJ + jy) * DK + ky) * DL + ly;
+    int pz = ((iz * DJ + jz) * DK + kz) * DL + lz;
+    pos[kk] = px | (py << 10) | (pz << 20);
+  }
+
+  double acc[NFT * NCOMP];
+#pragma unroll
+  for (int kk = 0; kk < NFT * NCOMP; ++kk) acc[kk] = 0.0;
+  unsigned int nsurv = 0;
+
+  R* __restrict__ g = sg[slot];
+  const double* __restrict__ uw = suw[slot];
+
+  for (int kp = 0; kp < a.npkl; ++kp) {
+    const int kidx = koff + kp;
+    const double akl = a.aij[kidx];
+    const double Qv[3] = {a.px[kidx], a.py[kidx], a.pz[kidx]};
+    const double kkl = a.kfac[kidx];
+    const double ccekl = a.three_center ? 0.0 : a.cce[kidx] - a.lkl_ceil * logd_kl;
+    const bool klok = active && ccekl <= a.expcutoff;
+    const double eijcut = a.expcutoff - ccekl;
+    for (int ip = 0; ip < a.npij; ++ip) {
+      const int iidx = boff + ip;
+      const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
+      const bool ok = klok && cceij <= eijcut;
+      if constexpr (T > 1) {
+        if (!__syncthreads_or(ok)) continue;
+      } else {
+        if (!ok) continue;
+      }
+      PrimTuple p;
+      p.aij = a.aij[iidx];
+      p.akl = akl;
+      const double Pv[3] = {a.px[iidx], a.py[iidx], a.pz[iidx]};
+      double rr = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        p.dx[d] = Pv[d] - Qv[d];
+        rr += p.dx[d] * p.dx[d];
+        p.prb[d] = Pv[d] - (IBASE ? ri[d] : rj[d]);
+        p.qrk[d] = Qv[d] - (KBASE ? rk[d] : rl[d]);
+      }
+      p.a1 = p.aij * akl;
+      p.a0 = p.a1 / (p.aij + akl);
+      const double fac = ok ? a.common * a.kfac[iidx] * kkl : 0.0;
+      const double fac1 = sqrt(p.a0 / (p.a1 * p.a1 * p.a1)) * fac;
+      const double x = p.a0 * rr;
+      if (ok && t == 0) ++nsurv;
+
+      // roots
+      double* uwm = suw[slot];
+      for (int r = t; r < NR; r += T) {
+        double u, w;
+        rys_eval(a.rys, NR, r, x, u, w);
+        uwm[r] = u;
+        uwm[NR + r] = w * fac1;
+      }
+      if constexpr (T > 1) __syncthreads();
+      // recurrence chains
+      for (int c = t; c < 3 * NR; c += T) {
+        const int axis = c / NR;
+        const int r = c - axis * NR;
+        build_chain<LIc, LJ, LK, LL, R>(g, axis, r, uw[r], uw[NR + r], p, rij[axis],
+                                        rkl[axis]);
+      }
+      if constexpr (T > 1) __syncthreads();
+      // Cartesian elements
+      if constexpr (!DERIV) {
+#pragma unroll
+        for (int kk = 0; kk < NFT; ++kk) {
+          if (kk * T + t < NF) {
+            const R* gx = g + (pos[kk] & 1023) * NR;
+            const R* gy = g + (G4 + ((pos[kk] >> 10) & 1023)) * NR;
+            const R* gz = g + (2 * G4 + (pos[kk] >> 20)) * NR;
+            R s = 0;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) s += gx[r] * gy[r] * gz[r];
+            acc[kk] += (double)s;
+          }
+        }
+      } else {
+        const R ai2 = (R)(-2.0 * a.ai[iidx]);
+        constexpr int SI = DJ * DK * DL;  // stride of the i power
+#pragma unroll
+        for (int kk = 0; kk < NFT; ++kk) {
+          const int e = kk * T + t;
+          if (e < NF) {
+            const int px = pos[kk] & 1023, py = (pos[kk] >> 10) & 1023, pz = pos[kk] >> 20;
+            const int fi = e % NFI;
+            int ix, iy, iz;
+            cart_decode(LI, fi, ix, iy, iz);
+            const R* gx = g + px * NR;
+            const R* gy = g + (G4 + py) * NR;
+            const R* gz = g + (2 * G4 + pz) * NR;
+            R sx = 0, sy = 0, sz = 0;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+              R fx = ai2 * gx[SI * NR + r];
+              R fy = ai2 * gy[SI * NR + r];
+              R fz = ai2 * gz[SI * NR + r];
+              if (ix > 0) fx += (R)ix * gx[r - SI * NR];
+              if (iy > 0) fy += (R)iy * gy[r - SI * NR];
+              if (iz > 0) fz += (R)iz * gz[r - SI * NR];
+              sx += fx * gy[r] * gz[r];
+              sy += gx[r] * fy * gz[r];
+              sz += gx[r] * gy[r] * fz;
+            }
+            acc[kk] += (double)sx;
+            acc[NFT + kk] += (double)sy;
+            acc[2 * NFT + kk] += (double)sz;
+          }
+        }
+      }
+    }
+  }
+  if (a.counters && t == 0 && nsurv) {
+    atomicAdd(a.counters, (unsigned long long)nsurv);
+    atomicAdd(a.counters + 1, 1ull);
+  }
+
+  // Cartesian -> spherical, then the strided write.
+  if constexpr (T > 1) __syncthreads();
+  double* b0 = sbuf[slot][0];
+  double* b1 = sbuf[slot][1];
+  constexpr int NSI = nsph(LI), NSJ = nsph(LJ), NSK = nsph(LK), NSL = nsph(LL);
+  constexpr int NS = NSI * NSJ * NSK * NSL;
+  for (int comp = 0; comp < NCOMP; ++comp) {
+#pragma unroll
+    for (int kk = 0; kk < NFT; ++kk) {
+      const int e = kk * T + t;
+      if (e < NF) b0[e] = acc[comp * NFT + kk];
+    }
+    if constexpr (T > 1) __syncthreads();
+    int dims[4] = {NFI, NFJ, NFK, NFL};
+    const int ls[4] = {LI, LJ, LK, LL};
+    double* src = b0;
+    double* dst = b1;
+#pragma unroll
+    for (int ax = 0; ax < 4; ++ax) {
+      if (ls[ax] < 2) continue;  // s and p: identity
+      c2s_pass(dst, src, dims, ax, ls[ax], a.c2s, t, T);
+      dims[ax] = nsph(ls[ax]);
+      double* tmp = src;
+      src = dst;
+      dst = tmp;
+      if constexpr (T > 1) __syncthreads();
+    }
+    if (active) {
+      double* o = a.out + a.out_off[q] + comp * a.st_comp;
+      for (int e = t; e < NS; e += T) {
+        int mi = e % NSI, e1 = e / NSI;
+        int mj = e1 % NSJ, e2 = e1 / NSJ;
+        int mk = e2 % NSK, ml = e2 / NSK;
+        o[mi * a.st[0] + mj * a.st[1] + mk * a.st[2] + ml * a.st[3]] = src[e];
+      }
+    }
+    if constexpr (T > 1) __syncthreads();
+  }
+}
+
+}  // namespace cx
+
+namespace cx {
+
+template <int LI, int LJ, int LK, int LL, int DERIV, typename R>
+hipError_t launch_class(const EriArgs& a, hipStream_t s) {
+  using S = ClassShape<LI, LJ, LK, LL, DERIV>;
+  constexpr int Q = S::template Q<R>();
+  const int64_t nblk = (a.nq + Q - 1) / Q;
+  hipLaunchKernelGGL((eri_class_kernel<LI, LJ, LK, LL, DERIV, R>), dim3((unsigned)nblk),
+                     dim3(S::T * Q), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int LI, int LJ, int LK, int LL, int DERIV, typename R>
+KernelInfo class_info() {
+  using S = ClassShape<LI, LJ, LK, LL, DERIV>;
+  return KernelInfo{&launch_class<LI, LJ, LK, LL, DERIV, R>, S::template Q<R>(), S::T};
+}
+
+}  // namespace cx

@@ -1,0 +1,1381 @@
+// Host runtime of libcintx_hip: input validation, device-resident basis and
+// primitive-pair data, class binning of shell tuples, kernel launches,
+// hipGraph replay, and the three C-ABI layers declared in cintx_hip.h.
+//
+// Reference behaviour mirrored here:
+//   * libcint raw entry points and their out == NULL cache query
+//     (libcint-master/src/cint2e.c:794-869, cint3c2e.c:556-625);
+//   * cintx-compat eval_raw validation and dims contract
+//     (crates/cintx-compat/src/raw.rs:874-1125, layout.rs:25-134);
+//   * cintx-capi status codes / thread-local last error
+//     (crates/cintx-capi/src/shim.rs:313-450, errors.rs:9-245).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/cintx_hip.h"
+#include "common.h"
+#include "launch.h"
+#include "rys_table.h"
+
+namespace cx {
+
+hipError_t launch_rys_eval(const RysTable& tab, int n, const double* x, int64_t nx, double* u,
+                           double* w, hipStream_t s);
+
+// ------------------------------------------------------------------
+// Errors: fail-closed, typed, thread-local last error.
+// ------------------------------------------------------------------
+struct Failure {
+  int status;
+  std::string msg;
+};
+
+static thread_local int t_status = 0;
+static thread_local std::string t_msg;
+
+static void set_error(int st, const std::string& m) {
+  t_status = st;
+  t_msg = m;
+}
+static void clear_error() {
+  t_status = 0;
+  t_msg.clear();
+}
+
+[[noreturn]] static void fail(int st, const std::string& m) { throw Failure{st, m}; }
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess)                                                                  \
+      fail(CINTX_STATUS_EXECUTION_FAILED, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename F>
+static int guarded(F&& f) {
+  try {
+    f();
+    clear_error();
+    return CINTX_STATUS_SUCCESS;
+  } catch (const Failure& e) {
+    set_error(e.status, e.msg);
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    set_error(CINTX_STATUS_ALLOCATION_FAILED, "host allocation failed");
+    return CINTX_STATUS_ALLOCATION_FAILED;
+  } catch (...) {
+    set_error(CINTX_STATUS_PANIC, "unexpected exception");
+    return CINTX_STATUS_PANIC;
+  }
+}
+
+// ------------------------------------------------------------------
+// Families
+// ------------------------------------------------------------------
+struct FamilyInfo {
+  bool three;
+  int deriv;
+  int ncomp;
+  int arity;
+  const char* name;
+};
+static const FamilyInfo kFam[4] = {{false, 0, 1, 4, "int2e_sph"},
+                                   {false, 1, 3, 4, "int2e_ip1_sph"},
+                                   {true, 0, 1, 3, "int3c2e_sph"},
+                                   {true, 1, 3, 3, "int3c2e_ip1_sph"}};
+
+static constexpr double kExpCutoff = 60.0;     // cint_config.h.in EXPCUTOFF
+static constexpr double kMinExpCutoff = 40.0;  // cint_config.h.in MIN_EXPCUTOFF
+
+static double common_fac_sp(int l) {  // g1e.c:566-573
+  if (l == 0) return 0.282094791773878143;
+  if (l == 1) return 0.488602511902919921;
+  return 1.0;
+}
+
+// ------------------------------------------------------------------
+// Real solid harmonics table (same convention as cart2sph.c:21):
+// coefficient of x^a y^b z^c (libcint Cartesian order) in r^l Y_lm with
+// unit-sphere-normalised real Y_lm, rows m = -l..l; s and p are identity.
+// Closed form: Helgaker, Jorgensen, Olsen, Molecular Electronic-Structure
+// Theory, eqs. 6.4.47-6.4.50.
+// ------------------------------------------------------------------
+static void build_c2s_table(std::vector<double>& tab) {
+  tab.assign((size_t)(kC2sL + 1) * kC2sM * kC2sC, 0.0);
+  auto at = [&](int l, int m, int c) -> double& {
+    return tab[((size_t)l * kC2sM + m) * kC2sC + c];
+  };
+  at(0, 0, 0) = 1.0;
+  for (int i = 0; i < 3; ++i) at(1, i, i) = 1.0;
+  auto binom = [](int n, int k) -> long double {
+    if (k < 0 || k > n) return 0.0L;
+    long double r = 1;
+    for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+    return r;
+  };
+  auto fact = [](int n) -> long double {
+    long double r = 1;
+    for (int i = 2; i <= n; ++i) r *= i;
+    return r;
+  };
+  auto cidx = [](int l, int lx, int ly) {
+    int pos = 0;
+    for (int ax = l; ax > lx; --ax) pos += l - ax + 1;
+    return pos + (l - lx - ly);
+  };
+  const long double pi = 3.14159265358979323846264338327950288L;
+  for (int l = 2; l <= kC2sL; ++l) {
+    long double ynorm = sqrtl((2 * l + 1) / (4 * pi));
+    for (int m = -l; m <= l; ++m) {
+      int am = std::abs(m);
+      int vm2 = m < 0 ? 1 : 0;
+      long double nlm = 1.0L / (powl(2.0L, am) * fact(l)) *
+                        sqrtl(2.0L * fact(l + am) * fact(l - am) / (m == 0 ? 2.0L : 1.0L));
+      for (int t = 0; t <= (l - am) / 2; ++t)
+        for (int uu = 0; uu <= t; ++uu) {
+          int vvmax = am - vm2 >= 0 ? (am - vm2) / 2 : -1;
+          for (int vv = 0; vv <= vvmax; ++vv) {
+            int v2 = 2 * vv + vm2;
+            long double c = (((t + vv) & 1) ? -1.0L : 1.0L) * powl(0.25L, t) * binom(l, t) *
+                            binom(l - t, am + t) * binom(t, uu) * binom(am, v2);
+            int px = 2 * t + am - 2 * uu - v2, py = 2 * uu + v2;
+            if (px < 0) continue;
+            at(l, m + l, cidx(l, px, py)) += (double)(ynorm * nlm * c);
+          }
+        }
+    }
+  }
+}
+
+// ------------------------------------------------------------------
+// Device context (one per process; current device at first use).
+// ------------------------------------------------------------------
+struct DevCtx {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  RysTable rys{};
+  double* d_rys = nullptr;
+  double* d_large = nullptr;
+  double* d_c2s = nullptr;
+  Registry* reg = nullptr;
+};
+
+static std::mutex g_mu;  // guards context creation and the per-call arena
+static DevCtx* g_ctx = nullptr;
+
+static DevCtx& ctx() {
+  static std::mutex init_mu;
+  std::lock_guard<std::mutex> lk(init_mu);
+  if (g_ctx) return *g_ctx;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    fail(CINTX_STATUS_EXECUTION_FAILED, "no HIP device visible");
+  DevCtx* c = new DevCtx();
+  HIPCHK(hipGetDevice(&c->device));
+  HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  std::vector<double> coef, large;
+  rys_table_build(coef, large, c->rys.off);
+  HIPCHK(hipMalloc(&c->d_rys, coef.size() * sizeof(double)));
+  HIPCHK(hipMalloc(&c->d_large, large.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(c->d_rys, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_large, large.data(), large.size() * sizeof(double),
+                   hipMemcpyHostToDevice));
+  c->rys.coef = c->d_rys;
+  c->rys.large = c->d_large;
+  std::vector<double> c2s;
+  build_c2s_table(c2s);
+  HIPCHK(hipMalloc(&c->d_c2s, c2s.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(c->d_c2s, c2s.data(), c2s.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->reg = new Registry();
+  std::memset(c->reg, 0, sizeof(Registry));
+  register_class_kernels(*c->reg);
+  g_ctx = c;
+  return *c;
+}
+
+// ------------------------------------------------------------------
+// Basis ingestion (libcint slots, cint.h.in:49-67).  General contractions
+// are split into virtual shells, one per contraction column.
+// ------------------------------------------------------------------
+struct VShell {
+  int l, nprim, orig, ctr;
+  double r[3];
+  int off;  // into exps / coefs
+};
+
+struct HostBasis {
+  std::vector<VShell> v;
+  std::vector<double> exps, coefs;
+  std::vector<int> first_v;  // original shell -> first virtual shell
+  std::vector<int> l, nctr, nprim;
+  int nbas = 0;
+};
+
+static HostBasis ingest(const int* atm, int natm, const int* bas, int nbas, const double* env,
+                        size_t env_len /* 0: unknown */) {
+  if (!atm) fail(CINTX_STATUS_NULL_POINTER, "atm is null");
+  if (!bas) fail(CINTX_STATUS_NULL_POINTER, "bas is null");
+  if (!env) fail(CINTX_STATUS_NULL_POINTER, "env is null");
+  if (natm <= 0) fail(CINTX_STATUS_INVALID_INPUT, "natm must be positive");
+  if (nbas <= 0) fail(CINTX_STATUS_INVALID_INPUT, "nbas must be positive");
+  HostBasis B;
+  B.nbas = nbas;
+  B.first_v.resize(nbas);
+  B.l.resize(nbas);
+  B.nctr.resize(nbas);
+  B.nprim.resize(nbas);
+  auto env_ok = [&](long p, long n) {
+    return p >= 0 && (env_len == 0 || (size_t)(p + n) <= env_len);
+  };
+  for (int s = 0; s < nbas; ++s) {
+    const int* b = bas + 8 * (size_t)s;
+    int ia = b[0], l = b[1], np = b[2], nc = b[3], pe = b[5], pc = b[6];
+    if (ia < 0 || ia >= natm)
+      fail(CINTX_STATUS_INVALID_INPUT, "bas[" + std::to_string(s) + "] ATOM_OF out of range");
+    if (l < 0) fail(CINTX_STATUS_INVALID_INPUT, "negative angular momentum");
+    if (l > kMaxL)
+      fail(CINTX_STATUS_UNSUPPORTED_API,
+           "angular momentum " + std::to_string(l) + " above the supported maximum 4");
+    if (np <= 0 || nc <= 0) fail(CINTX_STATUS_INVALID_INPUT, "NPRIM_OF/NCTR_OF must be positive");
+    if (np > 64) fail(CINTX_STATUS_UNSUPPORTED_API, "more than 64 primitives in a shell");
+    const int* a = atm + 6 * (size_t)ia;
+    if (!env_ok(a[1], 3) || !env_ok(pe, np) || !env_ok(pc, (long)np * nc))
+      fail(CINTX_STATUS_INVALID_INPUT, "env offset out of range for shell " + std::to_string(s));
+    B.first_v[s] = (int)B.v.size();
+    B.l[s] = l;
+    B.nctr[s] = nc;
+    B.nprim[s] = np;
+    for (int c = 0; c < nc; ++c) {
+      VShell vs;
+      vs.l = l;
+      vs.nprim = np;
+      vs.orig = s;
+      vs.ctr = c;
+      for (int d = 0; d < 3; ++d) vs.r[d] = env[a[1] + d];
+      vs.off = (int)B.exps.size();
+      for (int p = 0; p < np; ++p) {
+        B.exps.push_back(env[pe + p]);
+        B.coefs.push_back(env[pc + (size_t)c * np + p]);  // column-major [ctr][prim]
+      }
+      B.v.push_back(vs);
+    }
+  }
+  return B;
+}
+
+// ------------------------------------------------------------------
+// Primitive-pair data (CINTset_pairdata, optimizer.c:288-342; screening
+// keys cint2e.c:93-123, 199-236).  cce stored without the l-dependent
+// l_ij*log(|R_ij|+1) term, which the kernel subtracts per class.
+// ------------------------------------------------------------------
+struct PairSet {
+  std::vector<double> aij, px, py, pz, kfac, ai, cce;
+  std::vector<int> off, np;
+  std::vector<double> geo;
+  std::unordered_map<uint64_t, int> index;
+
+  int add(const HostBasis& B, int va, int vb) {
+    uint64_t key = ((uint64_t)(uint32_t)va << 32) | (uint32_t)vb;
+    auto it = index.find(key);
+    if (it != index.end()) return it->second;
+    const VShell& A = B.v[va];
+    const VShell& Bs = B.v[vb];
+    int id = (int)off.size();
+    off.push_back((int)aij.size());
+    np.push_back(A.nprim * Bs.nprim);
+    double rr = 0;
+    for (int d = 0; d < 3; ++d) rr += (A.r[d] - Bs.r[d]) * (A.r[d] - Bs.r[d]);
+    double aij_last = B.exps[A.off + A.nprim - 1] + B.exps[Bs.off + Bs.nprim - 1];
+    double log_rr0 = 1.7 - 1.5 * std::log(aij_last);
+    for (int jp = 0; jp < Bs.nprim; ++jp)
+      for (int ip = 0; ip < A.nprim; ++ip) {
+        double a_i = B.exps[A.off + ip], a_j = B.exps[Bs.off + jp];
+        double ci = B.coefs[A.off + ip], cj = B.coefs[Bs.off + jp];
+        double inv = 1.0 / (a_i + a_j);
+        double eij = rr * a_i * a_j * inv;
+        double wj = a_j * inv;
+        aij.push_back(a_i + a_j);
+        px.push_back(A.r[0] + wj * (Bs.r[0] - A.r[0]));
+        py.push_back(A.r[1] + wj * (Bs.r[1] - A.r[1]));
+        pz.push_back(A.r[2] + wj * (Bs.r[2] - A.r[2]));
+        kfac.push_back(std::exp(-eij) * ci * cj);
+        ai.push_back(a_i);
+        cce.push_back(eij - log_rr0 - std::log(std::fabs(ci)) - std::log(std::fabs(cj)));
+      }
+    for (int d = 0; d < 3; ++d) geo.push_back(A.r[d]);
+    for (int d = 0; d < 3; ++d) geo.push_back(Bs.r[d]);
+    geo.push_back(std::log(std::sqrt(rr) + 1.0));
+    geo.push_back(0.0);
+    index.emplace(key, id);
+    return id;
+  }
+
+  // int3c2e ket: shell k alone (the l slot of g3c2e.c:68-69 carries k, al=0).
+  int add_pseudo(const HostBasis& B, int vk) {
+    uint64_t key = ((uint64_t)0xffffffffu << 32) | (uint32_t)vk;
+    auto it = index.find(key);
+    if (it != index.end()) return it->second;
+    const VShell& K = B.v[vk];
+    int id = (int)off.size();
+    off.push_back((int)aij.size());
+    np.push_back(K.nprim);
+    for (int kp = 0; kp < K.nprim; ++kp) {
+      aij.push_back(B.exps[K.off + kp]);
+      px.push_back(K.r[0]);
+      py.push_back(K.r[1]);
+      pz.push_back(K.r[2]);
+      kfac.push_back(B.coefs[K.off + kp]);
+      ai.push_back(B.exps[K.off + kp]);
+      cce.push_back(0.0);
+    }
+    for (int d = 0; d < 3; ++d) geo.push_back(K.r[d]);
+    for (int d = 0; d < 3; ++d) geo.push_back(K.r[d]);
+    geo.push_back(0.0);
+    geo.push_back(0.0);
+    index.emplace(key, id);
+    return id;
+  }
+};
+
+// ------------------------------------------------------------------
+// Plans
+// ------------------------------------------------------------------
+struct Bin {
+  int l[4];
+  int prec;
+  int npij, npkl;
+  std::vector<int2> q;
+  int64_t out_base = 0;  // element offset of the bin's first block
+  int64_t block = 0;     // elements per block (NS * ncomp)
+  // device pointers (inside the plan blob)
+  int2* d_q = nullptr;
+  int64_t* d_off = nullptr;
+};
+
+struct VQ {  // one virtual (segmented) tuple of a caller tuple
+  int bin;
+  int64_t idx;
+  uint8_t cc[4];  // contraction index per canonical axis
+};
+
+struct TupleRec {
+  int shl[4];
+  int perm[4];  // canonical axis c -> caller axis perm[c]
+  int64_t first_vq, nvq;
+  int64_t nat_off;  // natural-layout offset in the host output
+  int64_t nat_size;
+};
+
+}  // namespace cx
+
+struct cintx_plan {
+  int family = 0;
+  double expcutoff = 60.0;
+  cx::HostBasis basis;
+  cx::PairSet pairs;
+  std::vector<cx::Bin> bins;
+  std::vector<cx::VQ> vqs;
+  std::vector<cx::TupleRec> tuples;
+  int64_t total = 0;  // output elements
+  char* blob = nullptr;
+  size_t blob_bytes = 0;
+  double* d_out = nullptr;
+  unsigned long long* d_counters = nullptr;  // 2 per bin
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  double* graph_out = nullptr;
+  // device pointers of the pair SoA
+  double *d_aij = nullptr, *d_px = nullptr, *d_py = nullptr, *d_pz = nullptr, *d_kfac = nullptr,
+         *d_ai = nullptr, *d_cce = nullptr, *d_geo = nullptr;
+  int* d_poff = nullptr;
+};
+
+namespace cx {
+
+static void canon_perm(int fam, const int* l, int p[4]) {
+  for (int c = 0; c < 4; ++c) p[c] = c;
+  const FamilyInfo& f = kFam[fam];
+  if (!f.deriv && l[0] < l[1]) std::swap(p[0], p[1]);
+  if (!f.three && l[2] < l[3]) std::swap(p[2], p[3]);
+  if (!f.three && !f.deriv) {
+    int b0 = l[p[0]], b1 = l[p[1]], k0 = l[p[2]], k1 = l[p[3]];
+    if (b0 < k0 || (b0 == k0 && b1 < k1)) {
+      std::swap(p[0], p[2]);
+      std::swap(p[1], p[3]);
+    }
+  }
+}
+
+static int64_t nat_size(int fam, const HostBasis& B, const int* shl) {
+  const FamilyInfo& f = kFam[fam];
+  int64_t n = f.ncomp;
+  for (int a = 0; a < f.arity; ++a) n *= (int64_t)nsph(B.l[shl[a]]) * B.nctr[shl[a]];
+  return n;
+}
+
+static double fam_expcutoff(int fam, const double* env) {
+  double e0 = env[0];
+  if (e0 == 0) return kExpCutoff;
+  double c = std::max(kMinExpCutoff, e0);
+  if (!kFam[fam].three) c += 1.0;  // g2e.c:57-62 vs g3c2e.c:56-60
+  return c;
+}
+
+static void plan_add_tuples(cintx_plan* P, const int* tuples, int64_t nt, int prec) {
+  const FamilyInfo& f = kFam[P->family];
+  const HostBasis& B = P->basis;
+  std::unordered_map<uint64_t, int> bin_of;
+  for (int64_t t = 0; t < nt; ++t) {
+    TupleRec tr;
+    int l[4] = {0, 0, 0, 0};
+    for (int a = 0; a < 4; ++a) tr.shl[a] = 0;
+    for (int a = 0; a < f.arity; ++a) {
+      int s = tuples[4 * t + a];
+      if (s < 0 || s >= B.nbas) fail(CINTX_STATUS_INVALID_INPUT, "shell index out of range");
+      tr.shl[a] = s;
+      l[a] = B.l[s];
+    }
+    canon_perm(P->family, l, tr.perm);
+    int cl[4], cn[4], cs[4];
+    for (int c = 0; c < 4; ++c) {
+      int a = tr.perm[c];
+      bool dummy = f.three && a == 3;
+      cl[c] = dummy ? 0 : l[a];
+      cn[c] = dummy ? 1 : B.nctr[tr.shl[a]];
+      cs[c] = dummy ? -1 : tr.shl[a];
+    }
+    tr.first_vq = (int64_t)P->vqs.size();
+    tr.nvq = (int64_t)cn[0] * cn[1] * cn[2] * cn[3];
+    tr.nat_off = P->total;
+    tr.nat_size = nat_size(P->family, B, tr.shl);
+    P->total += tr.nat_size;
+    int npij = B.nprim[cs[0]] * B.nprim[cs[1]];
+    int npkl = f.three ? B.nprim[cs[2]] : B.nprim[cs[2]] * B.nprim[cs[3]];
+    uint64_t key = 0;
+    for (int c = 0; c < 4; ++c) key = key * 8 + cl[c];
+    key = ((key * 2 + prec) * 128 + npij) * 128 + npkl;
+    int bi;
+    auto it = bin_of.find(key);
+    if (it == bin_of.end()) {
+      Bin b;
+      for (int c = 0; c < 4; ++c) b.l[c] = cl[c];
+      b.prec = prec;
+      b.npij = npij;
+      b.npkl = npkl;
+      b.block = (int64_t)f.ncomp * nsph(cl[0]) * nsph(cl[1]) * nsph(cl[2]) * nsph(cl[3]);
+      bi = (int)P->bins.size();
+      P->bins.push_back(b);
+      bin_of.emplace(key, bi);
+    } else {
+      bi = it->second;
+    }
+    for (int c3 = 0; c3 < cn[3]; ++c3)
+      for (int c2 = 0; c2 < cn[2]; ++c2)
+        for (int c1 = 0; c1 < cn[1]; ++c1)
+          for (int c0 = 0; c0 < cn[0]; ++c0) {
+            int v0 = B.first_v[cs[0]] + c0, v1 = B.first_v[cs[1]] + c1;
+            int bra = P->pairs.add(B, v0, v1);
+            int ket;
+            if (f.three)
+              ket = P->pairs.add_pseudo(B, B.first_v[cs[2]] + c2);
+            else
+              ket = P->pairs.add(B, B.first_v[cs[2]] + c2, B.first_v[cs[3]] + c3);
+            VQ vq;
+            vq.bin = bi;
+            vq.idx = (int64_t)P->bins[bi].q.size();
+            vq.cc[0] = (uint8_t)c0;
+            vq.cc[1] = (uint8_t)c1;
+            vq.cc[2] = (uint8_t)c2;
+            vq.cc[3] = (uint8_t)c3;
+            P->bins[bi].q.push_back(make_int2(bra, ket));
+            P->vqs.push_back(vq);
+          }
+    P->tuples.push_back(tr);
+  }
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Lay out and upload pairs, bins and offsets in one device blob.
+static void plan_upload(cintx_plan* P) {
+  DevCtx& C = ctx();
+  int64_t base = 0;
+  for (auto& b : P->bins) {
+    b.out_base = base;
+    base += b.block * (int64_t)b.q.size();
+  }
+  if (base != P->total) fail(CINTX_STATUS_PANIC, "output size mismatch");
+  const PairSet& S = P->pairs;
+  size_t npp = S.aij.size(), npair = S.off.size();
+  std::vector<std::pair<size_t, size_t>> segs;  // (offset, bytes)
+  size_t cur = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = cur;
+    cur = align256(cur + bytes);
+    return o;
+  };
+  size_t o_aij = take(npp * 8), o_px = take(npp * 8), o_py = take(npp * 8), o_pz = take(npp * 8),
+         o_k = take(npp * 8), o_ai = take(npp * 8), o_cce = take(npp * 8);
+  size_t o_poff = take(npair * 4), o_geo = take(npair * 64);
+  std::vector<size_t> o_q(P->bins.size()), o_off(P->bins.size());
+  for (size_t i = 0; i < P->bins.size(); ++i) {
+    o_q[i] = take(P->bins[i].q.size() * sizeof(int2));
+    o_off[i] = take(P->bins[i].q.size() * 8);
+  }
+  size_t o_cnt = take(P->bins.size() * 2 * 8 + 8);
+  size_t o_out = take((size_t)P->total * 8);
+  size_t host_bytes = o_out;  // everything before the output is uploaded
+  P->blob_bytes = cur;
+  HIPCHK(hipMalloc(&P->blob, P->blob_bytes));
+  std::vector<char> h(host_bytes, 0);
+  auto put = [&](size_t o, const void* src, size_t bytes) {
+    if (bytes) std::memcpy(h.data() + o, src, bytes);
+  };
+  put(o_aij, S.aij.data(), npp * 8);
+  put(o_px, S.px.data(), npp * 8);
+  put(o_py, S.py.data(), npp * 8);
+  put(o_pz, S.pz.data(), npp * 8);
+  put(o_k, S.kfac.data(), npp * 8);
+  put(o_ai, S.ai.data(), npp * 8);
+  put(o_cce, S.cce.data(), npp * 8);
+  put(o_poff, S.off.data(), npair * 4);
+  put(o_geo, S.geo.data(), npair * 64);
+  for (size_t i = 0; i < P->bins.size(); ++i) {
+    Bin& b = P->bins[i];
+    put(o_q[i], b.q.data(), b.q.size() * sizeof(int2));
+    std::vector<int64_t> offs(b.q.size());
+    for (size_t k = 0; k < b.q.size(); ++k) offs[k] = b.out_base + (int64_t)k * b.block;
+    put(o_off[i], offs.data(), offs.size() * 8);
+    b.d_q = (int2*)(P->blob + o_q[i]);
+    b.d_off = (int64_t*)(P->blob + o_off[i]);
+  }
+  HIPCHK(hipMemcpyAsync(P->blob, h.data(), host_bytes, hipMemcpyHostToDevice, C.stream));
+  HIPCHK(hipStreamSynchronize(C.stream));
+  P->d_aij = (double*)(P->blob + o_aij);
+  P->d_px = (double*)(P->blob + o_px);
+  P->d_py = (double*)(P->blob + o_py);
+  P->d_pz = (double*)(P->blob + o_pz);
+  P->d_kfac = (double*)(P->blob + o_k);
+  P->d_ai = (double*)(P->blob + o_ai);
+  P->d_cce = (double*)(P->blob + o_cce);
+  P->d_poff = (int*)(P->blob + o_poff);
+  P->d_geo = (double*)(P->blob + o_geo);
+  P->d_counters = (unsigned long long*)(P->blob + o_cnt);
+  P->d_out = (double*)(P->blob + o_out);
+  HIPCHK(hipEventCreate(&P->ev0));
+  HIPCHK(hipEventCreate(&P->ev1));
+}
+
+static void plan_check_kernels(cintx_plan* P) {
+  DevCtx& C = ctx();
+  const FamilyInfo& f = kFam[P->family];
+  for (auto& b : P->bins) {
+    const KernelInfo& k = C.reg->k[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
+    if (!k.fn) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "%s class (%d%d|%d%d) precision %d has no kernel in this build",
+               f.name, b.l[0], b.l[1], b.l[2], b.l[3], b.prec);
+      fail(CINTX_STATUS_UNSUPPORTED_API, buf);
+    }
+  }
+}
+
+static void plan_launch_all(cintx_plan* P, double* out, bool count) {
+  DevCtx& C = ctx();
+  const FamilyInfo& f = kFam[P->family];
+  double common_base = (M_PI * M_PI * M_PI) * 2.0 / std::sqrt(M_PI);
+  for (size_t i = 0; i < P->bins.size(); ++i) {
+    Bin& b = P->bins[i];
+    const KernelInfo& k = C.reg->k[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
+    EriArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.aij = P->d_aij;
+    a.px = P->d_px;
+    a.py = P->d_py;
+    a.pz = P->d_pz;
+    a.kfac = P->d_kfac;
+    a.ai = P->d_ai;
+    a.cce = P->d_cce;
+    a.pair_off = P->d_poff;
+    a.pair_geo = P->d_geo;
+    a.npij = b.npij;
+    a.npkl = b.npkl;
+    a.common = common_base * common_fac_sp(b.l[0]) * common_fac_sp(b.l[1]) *
+               common_fac_sp(b.l[2]) * (f.three ? 1.0 : common_fac_sp(b.l[3]));
+    a.expcutoff = P->expcutoff;
+    a.lij_ceil = b.l[0] + b.l[1] + f.deriv;
+    a.lkl_ceil = b.l[2] + b.l[3];
+    a.three_center = f.three ? 1 : 0;
+    int ns0 = nsph(b.l[0]), ns1 = nsph(b.l[1]), ns2 = nsph(b.l[2]);
+    a.st[0] = 1;
+    a.st[1] = ns0;
+    a.st[2] = (int64_t)ns0 * ns1;
+    a.st[3] = (int64_t)ns0 * ns1 * ns2;
+    a.st_comp = b.block / f.ncomp;
+    a.out = out;
+    a.rys = C.rys;
+    a.c2s = C.d_c2s;
+    a.counters = count ? P->d_counters + 2 * i : nullptr;
+    // grid dimension limit: launch in chunks of at most 2^28 tuples
+    const int64_t chunk = (int64_t)1 << 28;
+    for (int64_t s = 0; s < (int64_t)b.q.size(); s += chunk) {
+      a.quartets = b.d_q + s;
+      a.out_off = b.d_off + s;
+      a.nq = std::min(chunk, (int64_t)b.q.size() - s);
+      hipError_t e = k.fn(a, C.stream);
+      if (e != hipSuccess)
+        fail(CINTX_STATUS_EXECUTION_FAILED, std::string("kernel launch: ") + hipGetErrorString(e));
+    }
+  }
+}
+
+static cintx_plan* plan_new(int family, const int* atm, int natm, const int* bas, int nbas,
+                            const double* env, size_t env_len) {
+  if (family < 0 || family > 3) fail(CINTX_STATUS_UNSUPPORTED_API, "unknown family");
+  cintx_plan* P = new cintx_plan();
+  P->family = family;
+  try {
+    P->basis = ingest(atm, natm, bas, nbas, env, env_len);
+    P->expcutoff = fam_expcutoff(family, env);
+  } catch (...) {
+    delete P;
+    throw;
+  }
+  return P;
+}
+
+static void plan_free(cintx_plan* P) {
+  if (!P) return;
+  if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  if (P->ev0) (void)hipEventDestroy(P->ev0);
+  if (P->ev1) (void)hipEventDestroy(P->ev1);
+  if (P->blob) (void)hipFree(P->blob);
+  delete P;
+}
+
+// Convert the canonical device blocks (already on the host in `canon`) into
+// the libcint layout of every tuple (cart2sph.c:5359-5374 placement).
+static void plan_to_natural(const cintx_plan* P, const double* canon, double* nat) {
+  const FamilyInfo& f = kFam[P->family];
+  const HostBasis& B = P->basis;
+  for (const TupleRec& tr : P->tuples) {
+    int n[4] = {1, 1, 1, 1};
+    for (int a = 0; a < f.arity; ++a) n[a] = nsph(B.l[tr.shl[a]]) * B.nctr[tr.shl[a]];
+    int64_t so[4] = {1, n[0], (int64_t)n[0] * n[1], (int64_t)n[0] * n[1] * n[2]};
+    int64_t nout = (int64_t)n[0] * n[1] * n[2] * n[3];
+    double* dst = nat + tr.nat_off;
+    for (int64_t k = 0; k < tr.nvq; ++k) {
+      const VQ& vq = P->vqs[tr.first_vq + k];
+      const Bin& b = P->bins[vq.bin];
+      const double* src = canon + b.out_base + vq.idx * b.block;
+      int ns[4], base[4];
+      int64_t st[4];
+      for (int c = 0; c < 4; ++c) {
+        ns[c] = nsph(b.l[c]);
+        int a = tr.perm[c];
+        base[c] = vq.cc[c] * ns[c];
+        st[c] = (f.three && a == 3) ? 0 : so[a];
+      }
+      int64_t nsall = (int64_t)ns[0] * ns[1] * ns[2] * ns[3];
+      for (int comp = 0; comp < f.ncomp; ++comp) {
+        const double* s = src + comp * nsall;
+        double* d = dst + comp * nout;
+        int64_t e = 0;
+        for (int m3 = 0; m3 < ns[3]; ++m3)
+          for (int m2 = 0; m2 < ns[2]; ++m2)
+            for (int m1 = 0; m1 < ns[1]; ++m1)
+              for (int m0 = 0; m0 < ns[0]; ++m0, ++e)
+                d[(base[0] + m0) * st[0] + (base[1] + m1) * st[1] + (base[2] + m2) * st[2] +
+                  (base[3] + m3) * st[3]] = s[e];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------
+// libcint cache-size formulas (cint2e.c:780-816, cint3c2e.c:543-567).
+// ------------------------------------------------------------------
+static int cache_size(int fam, const int* shls, const int* bas) {
+  const FamilyInfo& f = kFam[fam];
+  auto B = [&](int s, int slot) { return bas[8 * (size_t)s + slot]; };
+  int l[4] = {0, 0, 0, 0}, np[4] = {1, 1, 1, 1}, nc[4] = {1, 1, 1, 1};
+  for (int a = 0; a < f.arity; ++a) {
+    l[a] = B(shls[a], 1);
+    np[a] = B(shls[a], 2);
+    nc[a] = B(shls[a], 3);
+  }
+  size_t nf = 1;
+  for (int a = 0; a < f.arity; ++a) nf *= ncart(l[a]);
+  size_t nctr = (size_t)nc[0] * nc[1] * nc[2] * nc[3];
+  size_t ncomp = f.ncomp;
+  int gbits = f.deriv ? 1 : 0;
+  int lic = l[0] + f.deriv, ljc = l[1];
+  size_t g_size;
+  if (!f.three) {
+    int lkc = l[2], llc = l[3];
+    int nroots = (lic + ljc + lkc + llc) / 2 + 1;
+    bool ib = lic > ljc, kb = lkc > llc;
+    int dli = ib ? lic + ljc + 1 : lic + 1, dlj = ib ? ljc + 1 : lic + ljc + 1;
+    int dlk = kb ? lkc + llc + 1 : lkc + 1, dll = kb ? llc + 1 : lkc + llc + 1;
+    g_size = (size_t)nroots * dli * dlk * dll * dlj;
+    size_t pdata = (size_t)(np[0] * np[1] + np[2] * np[3]) * 5 + np[0] * nc[0] + np[1] * nc[1] +
+                   np[2] * nc[2] + np[3] * nc[3] + (np[0] + np[1] + np[2] + np[3]) * 2 + nf * 3;
+    size_t leng = g_size * 3 * ((1 << gbits) + 1);
+    size_t len0 = nf * ncomp;
+    size_t nc_all = nf * nctr;
+    size_t a = leng + len0 + nc_all * ncomp * 3 + pdata;
+    size_t b = nc_all * ncomp + nf * 4;
+    return (int)std::max(a, b);
+  } else {
+    int llc = l[2];
+    int nroots = (lic + ljc + llc) / 2 + 1;
+    bool ib = lic > ljc;
+    int dli = ib ? lic + ljc + 1 : lic + 1, dlj = ib ? ljc + 1 : lic + ljc + 1;
+    int dlk = llc + 1;
+    g_size = (size_t)nroots * dli * dlk * dlj;
+    size_t pdata = (size_t)np[0] * np[1] * 5 + np[0] * nc[0] + np[1] * nc[1] + np[2] * nc[2] +
+                   (np[0] + np[1]) * 2 + np[2] + nf * 3 + 16;
+    size_t leng = g_size * 3 * ((1 << gbits) + 1);
+    size_t len0 = nf * ncomp;
+    size_t nc_all = nf * nctr;
+    size_t a = leng + len0 + nc_all * ncomp * 3 + pdata;
+    size_t b = nc_all * ncomp + nf * 3;
+    return (int)std::max(a, b);
+  }
+}
+
+// One libcint-signature call.  Returns not0.
+static int eval_one(int fam, double* out, const int* dims, const int* shls, const int* atm,
+                    int natm, const int* bas, int nbas, const double* env, size_t env_len) {
+  const FamilyInfo& f = kFam[fam];
+  if (!shls) fail(CINTX_STATUS_NULL_POINTER, "shls is null");
+  std::lock_guard<std::mutex> lk(g_mu);
+  DevCtx& C = ctx();
+  cintx_plan* P = plan_new(fam, atm, natm, bas, nbas, env, env_len);
+  struct Guard {
+    cintx_plan* p;
+    ~Guard() { plan_free(p); }
+  } guard{P};
+  int tup[4] = {shls[0], shls[1], shls[2], f.arity == 4 ? shls[3] : 0};
+  plan_add_tuples(P, tup, 1, CINTX_PREC_FP64);
+  plan_check_kernels(P);
+  plan_upload(P);
+  HIPCHK(hipMemsetAsync(P->d_counters, 0, P->bins.size() * 16, C.stream));
+  plan_launch_all(P, P->d_out, true);
+  std::vector<double> canon((size_t)P->total), nat((size_t)P->total);
+  HIPCHK(hipMemcpyAsync(canon.data(), P->d_out, (size_t)P->total * 8, hipMemcpyDeviceToHost,
+                        C.stream));
+  std::vector<unsigned long long> cnt(P->bins.size() * 2);
+  HIPCHK(hipMemcpyAsync(cnt.data(), P->d_counters, cnt.size() * 8, hipMemcpyDeviceToHost,
+                        C.stream));
+  HIPCHK(hipStreamSynchronize(C.stream));
+  plan_to_natural(P, canon.data(), nat.data());
+  unsigned long long surv = 0;
+  for (size_t i = 0; i < P->bins.size(); ++i) surv += cnt[2 * i];
+  const TupleRec& tr = P->tuples[0];
+  int n[4] = {1, 1, 1, 1};
+  for (int a = 0; a < f.arity; ++a) n[a] = nsph(P->basis.l[tr.shl[a]]) * P->basis.nctr[tr.shl[a]];
+  int d[4] = {n[0], n[1], n[2], n[3]};
+  if (dims)
+    for (int a = 0; a < f.arity; ++a) d[a] = dims[a];
+  for (int a = 0; a < f.arity; ++a)
+    if (d[a] < n[a]) fail(CINTX_STATUS_INVALID_INPUT, "dims smaller than the shell extents");
+  int64_t nout_d = (int64_t)d[0] * d[1] * d[2] * d[3];
+  int64_t nout_n = (int64_t)n[0] * n[1] * n[2] * n[3];
+  for (int comp = 0; comp < f.ncomp; ++comp)
+    for (int l = 0; l < n[3]; ++l)
+      for (int k = 0; k < n[2]; ++k)
+        for (int j = 0; j < n[1]; ++j)
+          for (int i = 0; i < n[0]; ++i)
+            out[comp * nout_d + i + (int64_t)d[0] * (j + (int64_t)d[1] * (k + (int64_t)d[2] * l))] =
+                surv ? nat[comp * nout_n + i + (int64_t)n[0] * (j + (int64_t)n[1] * (k + (int64_t)n[2] * l))]
+                     : 0.0;
+  return surv ? 1 : 0;
+}
+
+static int family_of_api(int api) {
+  switch (api) {
+    case CINTX_API_INT2E_SPH:
+      return 0;
+    case CINTX_API_INT2E_IP1_SPH:
+      return 1;
+    case CINTX_API_INT3C2E_SPH:
+      return 2;
+    case CINTX_API_INT3C2E_IP1_SPH:
+      return 3;
+    default:
+      return -1;
+  }
+}
+
+}  // namespace cx
+
+using namespace cx;
+
+// ------------------------------------------------------------------
+// 1. libcint raw signature
+// ------------------------------------------------------------------
+struct CINTOpt {
+  int family;
+};
+
+static int raw_entry(int fam, double* out, int* dims, int* shls, int* atm, int natm, int* bas,
+                     int nbas, double* env) {
+  if (!out) {
+    if (!shls || !bas) return 0;
+    return cache_size(fam, shls, bas);
+  }
+  int not0 = 0;
+  int st = guarded([&] { not0 = eval_one(fam, out, dims, shls, atm, natm, bas, nbas, env, 0); });
+  if (st != CINTX_STATUS_SUCCESS) {
+    fprintf(stderr, "libcintx_hip %s: %s\n", kFam[fam].name, t_msg.c_str());
+    return 0;
+  }
+  return not0;
+}
+
+extern "C" {
+
+int int2e_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
+              double* env, CINTOpt* opt, double* cache) {
+  (void)opt;
+  (void)cache;
+  return raw_entry(0, out, dims, shls, atm, natm, bas, nbas, env);
+}
+int int2e_ip1_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
+                  double* env, CINTOpt* opt, double* cache) {
+  (void)opt;
+  (void)cache;
+  return raw_entry(1, out, dims, shls, atm, natm, bas, nbas, env);
+}
+int int3c2e_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
+                double* env, CINTOpt* opt, double* cache) {
+  (void)opt;
+  (void)cache;
+  return raw_entry(2, out, dims, shls, atm, natm, bas, nbas, env);
+}
+int int3c2e_ip1_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
+                    double* env, CINTOpt* opt, double* cache) {
+  (void)opt;
+  (void)cache;
+  return raw_entry(3, out, dims, shls, atm, natm, bas, nbas, env);
+}
+
+static void make_opt(CINTOpt** opt, int fam) {
+  if (!opt) return;
+  *opt = new CINTOpt{fam};
+}
+void int2e_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 0); }
+void int2e_ip1_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 1); }
+void int3c2e_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 2); }
+void int3c2e_ip1_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 3); }
+void CINTdel_optimizer(CINTOpt** opt) {
+  if (opt && *opt) {
+    delete *opt;
+    *opt = nullptr;
+  }
+}
+
+// ------------------------------------------------------------------
+// 2. cintx C shim
+// ------------------------------------------------------------------
+static void shim_common(int fam, const int* dims, size_t dims_len, const int* shls,
+                        size_t shls_len, const int* atm, size_t atm_len, const int* bas,
+                        size_t bas_len, const double* env, size_t env_len, int64_t* required) {
+  const FamilyInfo& f = kFam[fam];
+  if (!shls) fail(CINTX_STATUS_NULL_POINTER, "shls is null");
+  if (!atm) fail(CINTX_STATUS_NULL_POINTER, "atm is null");
+  if (!bas) fail(CINTX_STATUS_NULL_POINTER, "bas is null");
+  if (!env) fail(CINTX_STATUS_NULL_POINTER, "env is null");
+  if (shls_len != (size_t)f.arity)
+    fail(CINTX_STATUS_INVALID_INPUT, "shls length does not match the operator arity");
+  if (atm_len % 6 || bas_len % 8 || atm_len == 0 || bas_len == 0)
+    fail(CINTX_STATUS_INVALID_INPUT, "atm/bas lengths must be multiples of 6/8");
+  int natm = (int)(atm_len / 6), nbas = (int)(bas_len / 8);
+  HostBasis B = ingest(atm, natm, bas, nbas, env, env_len);
+  int n[4] = {1, 1, 1, 1};
+  for (int a = 0; a < f.arity; ++a) {
+    if (shls[a] < 0 || shls[a] >= nbas) fail(CINTX_STATUS_INVALID_INPUT, "shell index out of range");
+    n[a] = nsph(B.l[shls[a]]) * B.nctr[shls[a]];
+  }
+  if (dims) {
+    // crates/cintx-compat/src/layout.rs:25-64: dims must equal the natural extents
+    if (dims_len != (size_t)f.arity) fail(CINTX_STATUS_INVALID_INPUT, "invalid dims length");
+    for (int a = 0; a < f.arity; ++a)
+      if (dims[a] != n[a]) fail(CINTX_STATUS_INVALID_INPUT, "dims differ from natural extents");
+  }
+  *required = (int64_t)f.ncomp * n[0] * n[1] * n[2] * n[3];
+}
+
+int cintrs_query_workspace(int api, const int* dims, size_t dims_len, const int* shls,
+                           size_t shls_len, const int* atm, size_t atm_len, const int* bas,
+                           size_t bas_len, const double* env, size_t env_len, const void* opt,
+                           CintxWorkspaceQuery* query_out) {
+  (void)opt;
+  int fam = family_of_api(api);
+  if (fam < 0) {
+    set_error(CINTX_STATUS_UNSUPPORTED_API, "api id " + std::to_string(api) + " is not served");
+    return CINTX_STATUS_UNSUPPORTED_API;
+  }
+  return guarded([&] {
+    if (!query_out) fail(CINTX_STATUS_NULL_POINTER, "query_out is null");
+    int64_t req = 0;
+    shim_common(fam, dims, dims_len, shls, shls_len, atm, atm_len, bas, bas_len, env, env_len,
+                &req);
+    CintxWorkspaceQuery q;
+    q.bytes = (size_t)cache_size(fam, shls, bas) * 8;
+    q.alignment = 64;
+    q.required_bytes = (size_t)req * 8;
+    q.chunk_count = 1;
+    q.work_units = 1;
+    *query_out = q;
+  });
+}
+
+int cintrs_eval(int api, double* out, size_t out_len, const int* dims, size_t dims_len,
+                const int* shls, size_t shls_len, const int* atm, size_t atm_len, const int* bas,
+                size_t bas_len, const double* env, size_t env_len, const void* opt, double* cache,
+                size_t cache_len, CintxEvalSummary* summary_out) {
+  (void)opt;
+  int fam = family_of_api(api);
+  if (fam < 0) {
+    set_error(CINTX_STATUS_UNSUPPORTED_API, "api id " + std::to_string(api) + " is not served");
+    return CINTX_STATUS_UNSUPPORTED_API;
+  }
+  return guarded([&] {
+    if (!summary_out) fail(CINTX_STATUS_NULL_POINTER, "summary_out is null");
+    if (!out && out_len > 0) fail(CINTX_STATUS_NULL_POINTER, "out is null");
+    if (!cache && cache_len > 0) fail(CINTX_STATUS_NULL_POINTER, "cache is null");
+    if (out && cache && out == cache)
+      fail(CINTX_STATUS_INVALID_INPUT, "`out` and `cache` pointers must not alias");
+    int64_t req = 0;
+    shim_common(fam, dims, dims_len, shls, shls_len, atm, atm_len, bas, bas_len, env, env_len,
+                &req);
+    size_t ws = (size_t)cache_size(fam, shls, bas) * 8;
+    CintxEvalSummary s{0, 0, ws};
+    if (!out) {
+      *summary_out = s;
+      return;
+    }
+    if (out_len < (size_t)req)
+      fail(CINTX_STATUS_BUFFER_TOO_SMALL, "out holds " + std::to_string(out_len) + " elements, " +
+                                              std::to_string(req) + " required");
+    if (cache && cache_len * 8 < ws) fail(CINTX_STATUS_BUFFER_TOO_SMALL, "cache too small");
+    // evaluate into a staging buffer first: no partial writes on failure
+    std::vector<double> staging((size_t)req);
+    int not0 = eval_one(fam, staging.data(), nullptr, shls, atm, (int)(atm_len / 6), bas,
+                        (int)(bas_len / 8), env, env_len);
+    std::memcpy(out, staging.data(), (size_t)req * 8);
+    s.not0 = not0;
+    s.bytes_written = (size_t)req * 8;
+    *summary_out = s;
+  });
+}
+
+int cintrs_last_error_code(void) { return t_status; }
+void cintrs_clear_last_error(void) { clear_error(); }
+size_t cintrs_copy_last_error_message(char* out, size_t out_len) {
+  size_t need = t_msg.size() + 1;
+  if (out && out_len) {
+    size_t n = std::min(out_len - 1, t_msg.size());
+    std::memcpy(out, t_msg.data(), n);
+    out[n] = 0;
+  }
+  return need;
+}
+
+// ------------------------------------------------------------------
+// 3. Plans
+// ------------------------------------------------------------------
+int cintx_plan_create(cintx_plan** plan, int family, const int* atm, int natm, const int* bas,
+                      int nbas, const double* env, const int* tuples, int64_t ntuples,
+                      int precision, double mixed_threshold) {
+  (void)mixed_threshold;
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    if (!tuples && ntuples > 0) fail(CINTX_STATUS_NULL_POINTER, "tuples is null");
+    if (precision != CINTX_PREC_FP64 && precision != CINTX_PREC_MIXED)
+      fail(CINTX_STATUS_INVALID_INPUT, "unknown precision mode");
+    cintx_plan* P = plan_new(family, atm, natm, bas, nbas, env, 0);
+    try {
+      plan_add_tuples(P, tuples, ntuples, precision == CINTX_PREC_MIXED ? 1 : 0);
+      plan_check_kernels(P);
+      plan_upload(P);
+    } catch (...) {
+      plan_free(P);
+      throw;
+    }
+    *plan = P;
+  });
+}
+
+int cintx_plan_destroy(cintx_plan* plan) {
+  return guarded([&] { plan_free(plan); });
+}
+
+int cintx_plan_sizes(const cintx_plan* plan, int64_t* nelem, int64_t* ntuples) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    if (nelem) *nelem = plan->total;
+    if (ntuples) *ntuples = (int64_t)plan->tuples.size();
+  });
+}
+
+int cintx_plan_tuples(const cintx_plan* plan, int* tuples, int64_t* offsets) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    for (size_t t = 0; t < plan->tuples.size(); ++t) {
+      if (tuples)
+        for (int a = 0; a < 4; ++a) tuples[4 * t + a] = plan->tuples[t].shl[a];
+      if (offsets) offsets[t] = plan->tuples[t].nat_off;
+    }
+  });
+}
+
+int cintx_plan_execute(cintx_plan* plan, double* out_dev) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    DevCtx& C = ctx();
+    HIPCHK(hipEventRecord(plan->ev0, C.stream));
+    plan_launch_all(plan, out_dev ? out_dev : plan->d_out, false);
+    HIPCHK(hipEventRecord(plan->ev1, C.stream));
+  });
+}
+
+int cintx_plan_execute_graph(cintx_plan* plan, double* out_dev) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    DevCtx& C = ctx();
+    double* out = out_dev ? out_dev : plan->d_out;
+    if (plan->graph_exec && plan->graph_out != out) {
+      HIPCHK(hipGraphExecDestroy(plan->graph_exec));
+      HIPCHK(hipGraphDestroy(plan->graph));
+      plan->graph_exec = nullptr;
+      plan->graph = nullptr;
+    }
+    if (!plan->graph_exec) {
+      HIPCHK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+      try {
+        plan_launch_all(plan, out, false);
+      } catch (...) {
+        hipGraph_t g;
+        (void)hipStreamEndCapture(C.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(C.stream, &plan->graph));
+      HIPCHK(hipGraphInstantiate(&plan->graph_exec, plan->graph, nullptr, nullptr, 0));
+      plan->graph_out = out;
+    }
+    HIPCHK(hipEventRecord(plan->ev0, C.stream));
+    HIPCHK(hipGraphLaunch(plan->graph_exec, C.stream));
+    HIPCHK(hipEventRecord(plan->ev1, C.stream));
+  });
+}
+
+int cintx_plan_sync(cintx_plan* plan) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    DevCtx& C = ctx();
+    HIPCHK(hipStreamSynchronize(C.stream));
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, plan->ev0, plan->ev1) == hipSuccess) plan->last_ms = ms;
+  });
+}
+
+int cintx_plan_copy_output(cintx_plan* plan, double* host_out) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    if (!host_out) fail(CINTX_STATUS_NULL_POINTER, "host_out is null");
+    DevCtx& C = ctx();
+    std::vector<double> canon((size_t)plan->total);
+    HIPCHK(hipMemcpyAsync(canon.data(), plan->d_out, (size_t)plan->total * 8,
+                          hipMemcpyDeviceToHost, C.stream));
+    HIPCHK(hipStreamSynchronize(C.stream));
+    plan_to_natural(plan, canon.data(), host_out);
+  });
+}
+
+// Algorithmic work per surviving primitive tuple and per tuple (c2s), the
+// trip counts of libcint's loops (same model as the oracle's work counter).
+static void class_work(int fam, const int* l, double* per_prim, double* per_tuple, int* nroots) {
+  const FamilyInfo& f = kFam[fam];
+  int Lc[4] = {l[0] + f.deriv, l[1], l[2], l[3]};
+  int nr = (Lc[0] + Lc[1] + Lc[2] + Lc[3]) / 2 + 1;
+  int nmax = Lc[0] + Lc[1], mmax = Lc[2] + Lc[3];
+  double nf = (double)ncart(l[0]) * ncart(l[1]) * ncart(l[2]) * ncart(l[3]);
+  double per = 24.0 * nr + 3.0 * nr * (4.0 * (nmax + mmax) + 7.0 * nmax * mmax);
+  int A = std::max(Lc[0], Lc[1]), Bv = std::min(Lc[0], Lc[1]);
+  int Cv = std::max(Lc[2], Lc[3]), D = std::min(Lc[2], Lc[3]);
+  double hb = 0, hk = 0;
+  for (int b = 1; b <= Bv; ++b) hb += (A + Bv - b + 1) * (double)(mmax + 1);
+  for (int d = 1; d <= D; ++d) hk += (Cv + D - d + 1) * (double)((Lc[0] + 1) * (Lc[1] + 1));
+  per += 6.0 * nr * (hb + hk);
+  if (f.deriv)
+    per += 9.0 * nf * nr + 9.0 * nr * (l[0] + 1) * (Lc[1] + 1) * (Lc[2] + 1) * (Lc[3] + 1);
+  else
+    per += 3.0 * nf * nr;
+  per += 2.0 * nf * f.ncomp;
+  double c2s = 0, dim[4] = {(double)ncart(l[0]), (double)ncart(l[1]), (double)ncart(l[2]),
+                            (double)ncart(l[3])};
+  for (int a = 0; a < 4; ++a) {
+    double other = 1;
+    for (int b = 0; b < 4; ++b)
+      if (b != a) other *= dim[b];
+    c2s += 2.0 * other * nsph(l[a]) * ncart(l[a]);
+    dim[a] = nsph(l[a]);
+  }
+  *per_prim = per;
+  *per_tuple = c2s * f.ncomp;
+  *nroots = nr;
+}
+
+int cintx_plan_last_time_ms(const cintx_plan* plan, double* ms) {
+  return guarded([&] {
+    if (!plan || !ms) fail(CINTX_STATUS_NULL_POINTER, "null argument");
+    *ms = plan->last_ms;
+  });
+}
+
+int cintx_plan_work(cintx_plan* plan, double* prim_tuples, double* root_evals, double* flops) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    DevCtx& C = ctx();
+    HIPCHK(hipMemsetAsync(plan->d_counters, 0, plan->bins.size() * 16 + 8, C.stream));
+    plan_launch_all(plan, plan->d_out, true);
+    std::vector<unsigned long long> cnt(plan->bins.size() * 2);
+    HIPCHK(hipMemcpyAsync(cnt.data(), plan->d_counters, cnt.size() * 8, hipMemcpyDeviceToHost,
+                          C.stream));
+    HIPCHK(hipStreamSynchronize(C.stream));
+    double np = 0, nr = 0, fl = 0;
+    for (size_t i = 0; i < plan->bins.size(); ++i) {
+      double per, pt;
+      int r;
+      class_work(plan->family, plan->bins[i].l, &per, &pt, &r);
+      np += (double)cnt[2 * i];
+      nr += (double)cnt[2 * i] * r;
+      fl += (double)cnt[2 * i] * per + (double)cnt[2 * i + 1] * pt;
+    }
+    if (prim_tuples) *prim_tuples = np;
+    if (root_evals) *root_evals = nr;
+    if (flops) *flops = fl;
+  });
+}
+
+int cintx_plan_create_screened(cintx_plan** plan, const int* atm, int natm, const int* bas,
+                               int nbas, const double* env, double threshold, int bra_stride,
+                               int precision, double mixed_threshold) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    if (bra_stride < 1) fail(CINTX_STATUS_INVALID_INPUT, "bra_stride must be >= 1");
+    if (precision != CINTX_PREC_FP64 && precision != CINTX_PREC_MIXED)
+      fail(CINTX_STATUS_INVALID_INPUT, "unknown precision mode");
+    // Schwarz bounds Q_ij = sqrt(max |(ij|ij)|) over i >= j, from a diagonal plan.
+    std::vector<int> diag;
+    for (int i = 0; i < nbas; ++i)
+      for (int j = 0; j <= i; ++j) {
+        diag.push_back(i);
+        diag.push_back(j);
+        diag.push_back(i);
+        diag.push_back(j);
+      }
+    int64_t npair_all = (int64_t)diag.size() / 4;
+    std::vector<double> qv((size_t)npair_all, 0.0);
+    {
+      cintx_plan* D = plan_new(0, atm, natm, bas, nbas, env, 0);
+      struct G {
+        cintx_plan* p;
+        ~G() { plan_free(p); }
+      } g{D};
+      for (int s = 0; s < nbas; ++s)
+        if (D->basis.nctr[s] != 1)
+          fail(CINTX_STATUS_UNSUPPORTED_API, "screened plans need segmented shells (nctr == 1)");
+      plan_add_tuples(D, diag.data(), npair_all, 0);
+      plan_check_kernels(D);
+      plan_upload(D);
+      DevCtx& C = ctx();
+      plan_launch_all(D, D->d_out, false);
+      std::vector<double> nat((size_t)D->total);
+      std::vector<double> canon((size_t)D->total);
+      HIPCHK(hipMemcpyAsync(canon.data(), D->d_out, (size_t)D->total * 8, hipMemcpyDeviceToHost,
+                            C.stream));
+      HIPCHK(hipStreamSynchronize(C.stream));
+      for (int64_t t = 0; t < npair_all; ++t) {
+        const TupleRec& tr = D->tuples[t];
+        double m = 0;
+        for (int64_t k = 0; k < tr.nvq; ++k) {
+          const VQ& vq = D->vqs[tr.first_vq + k];
+          const Bin& b = D->bins[vq.bin];
+          const double* src = canon.data() + b.out_base + vq.idx * b.block;
+          for (int64_t e = 0; e < b.block; ++e) m = std::max(m, std::fabs(src[e]));
+        }
+        qv[t] = std::sqrt(m);
+      }
+    }
+    cintx_plan* P = plan_new(0, atm, natm, bas, nbas, env, 0);
+    try {
+      const HostBasis& B = P->basis;
+      double qmax = 0;
+      for (double q : qv) qmax = std::max(qmax, q);
+      struct SP {
+        int a, b;  // oriented: l[a] >= l[b]
+        double q;
+        int64_t id;
+      };
+      std::vector<SP> sp;
+      for (int64_t t = 0; t < npair_all; ++t) {
+        if (qv[t] * qmax < threshold) continue;
+        int i = diag[4 * t], j = diag[4 * t + 1];
+        SP s;
+        s.a = B.l[i] >= B.l[j] ? i : j;
+        s.b = B.l[i] >= B.l[j] ? j : i;
+        s.q = qv[t];
+        s.id = t;
+        sp.push_back(s);
+      }
+      std::vector<int> order(sp.size());
+      for (size_t k = 0; k < sp.size(); ++k) order[k] = (int)k;
+      std::sort(order.begin(), order.end(), [&](int x, int y) { return sp[x].q > sp[y].q; });
+      std::vector<int> tuples;
+      for (size_t ip = 0; ip < sp.size(); ++ip) {
+        const SP& p = sp[ip];
+        if (p.id % bra_stride) continue;
+        double need = threshold / std::max(p.q, 1e-300);
+        for (int oq : order) {
+          const SP& q = sp[oq];
+          if (q.q < need) break;
+          if (q.id > p.id) continue;
+          tuples.push_back(p.a);
+          tuples.push_back(p.b);
+          tuples.push_back(q.a);
+          tuples.push_back(q.b);
+        }
+      }
+      int64_t nt = (int64_t)tuples.size() / 4;
+      if (precision == CINTX_PREC_MIXED) {
+        // split by bound: FP32 bins for tuples below mixed_threshold
+        std::vector<int> lo, hi;
+        std::unordered_map<uint64_t, double> qof;
+        for (const SP& s : sp) qof[((uint64_t)s.a << 32) | (uint32_t)s.b] = s.q;
+        for (int64_t t = 0; t < nt; ++t) {
+          double b = qof[((uint64_t)tuples[4 * t] << 32) | (uint32_t)tuples[4 * t + 1]] *
+                     qof[((uint64_t)tuples[4 * t + 2] << 32) | (uint32_t)tuples[4 * t + 3]];
+          auto& dst = b < mixed_threshold ? lo : hi;
+          dst.insert(dst.end(), tuples.begin() + 4 * t, tuples.begin() + 4 * t + 4);
+        }
+        plan_add_tuples(P, hi.data(), (int64_t)hi.size() / 4, 0);
+        plan_add_tuples(P, lo.data(), (int64_t)lo.size() / 4, 1);
+      } else {
+        plan_add_tuples(P, tuples.data(), nt, 0);
+      }
+      plan_check_kernels(P);
+      plan_upload(P);
+    } catch (...) {
+      plan_free(P);
+      throw;
+    }
+    *plan = P;
+  });
+}
+
+// Rys rule from the Chebyshev table, evaluated on the host (same table and
+// Clenshaw recurrence as rys_eval in eri_device.h).
+int cintx_rys_eval_host(int n, const double* x, int64_t nx, double* u, double* w) {
+  return guarded([&] {
+    if (n < 1 || n > kMaxRoots) fail(CINTX_STATUS_INVALID_INPUT, "nroots out of range");
+    if (!x || !u || !w) fail(CINTX_STATUS_NULL_POINTER, "null argument");
+    static std::mutex mu;
+    static std::vector<double> coef, large;
+    static int off[kMaxRoots + 1];
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (coef.empty()) rys_table_build(coef, large, off);
+    }
+    for (int64_t i = 0; i < nx; ++i) {
+      double xi = x[i];
+      for (int r = 0; r < n; ++r) {
+        if (!(xi < rys_largex(n))) {
+          const double* L = large.data() + n * (n - 1);
+          u[i * n + r] = L[r] / (xi - L[r]);
+          w[i * n + r] = L[n + r] / std::sqrt(xi);
+          continue;
+        }
+        int iv = (int)xi;
+        double t = 2.0 * (xi - iv) - 1.0, t2 = t + t;
+        const double* cu = coef.data() + off[n] + ((size_t)iv * 2 * n + r) * kRysNc;
+        const double* cw = cu + (size_t)n * kRysNc;
+        double bu1 = 0, bu2 = 0, bw1 = 0, bw2 = 0;
+        for (int k = kRysDeg; k >= 1; --k) {
+          double bu0 = std::fma(t2, bu1, cu[k] - bu2);
+          double bw0 = std::fma(t2, bw1, cw[k] - bw2);
+          bu2 = bu1;
+          bu1 = bu0;
+          bw2 = bw1;
+          bw1 = bw0;
+        }
+        u[i * n + r] = std::fma(t, bu1, cu[0] - bu2);
+        w[i * n + r] = std::fma(t, bw1, cw[0] - bw2);
+      }
+    }
+  });
+}
+
+// The same rule evaluated by the device (rys_eval of the class kernels).
+int cintx_rys_eval_device(int n, const double* x, int64_t nx, double* u, double* w) {
+  return guarded([&] {
+    if (n < 1 || n > kMaxRoots) fail(CINTX_STATUS_INVALID_INPUT, "nroots out of range");
+    if (!x || !u || !w) fail(CINTX_STATUS_NULL_POINTER, "null argument");
+    DevCtx& C = ctx();
+    double *dx = nullptr, *du = nullptr, *dw = nullptr;
+    size_t bx = (size_t)std::max<int64_t>(nx, 1) * 8, bu = bx * n;
+    HIPCHK(hipMalloc(&dx, bx));
+    HIPCHK(hipMalloc(&du, bu));
+    HIPCHK(hipMalloc(&dw, bu));
+    struct F {
+      double *a, *b, *c;
+      ~F() {
+        (void)hipFree(a);
+        (void)hipFree(b);
+        (void)hipFree(c);
+      }
+    } f{dx, du, dw};
+    HIPCHK(hipMemcpy(dx, x, (size_t)nx * 8, hipMemcpyHostToDevice));
+    HIPCHK(launch_rys_eval(C.rys, n, dx, nx, du, dw, C.stream));
+    HIPCHK(hipStreamSynchronize(C.stream));
+    HIPCHK(hipMemcpy(u, du, (size_t)nx * n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(w, dw, (size_t)nx * n * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+int cintx_device_alloc(double** ptr, int64_t nelem) {
+  return guarded([&] {
+    if (!ptr) fail(CINTX_STATUS_NULL_POINTER, "ptr is null");
+    ctx();
+    hipError_t e = hipMalloc(ptr, (size_t)std::max<int64_t>(nelem, 1) * 8);
+    if (e != hipSuccess) fail(CINTX_STATUS_ALLOCATION_FAILED, hipGetErrorString(e));
+  });
+}
+int cintx_device_free(double* ptr) {
+  return guarded([&] { HIPCHK(hipFree(ptr)); });
+}
+int cintx_device_count(int* n) {
+  return guarded([&] {
+    if (!n) fail(CINTX_STATUS_NULL_POINTER, "n is null");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,182 @@
+/*
+ * cintx_hip.h -- C ABI of libcintx_hip.so, the MI355X (gfx950) Rys
+ * two-electron engine that replaces cintx's CubeCL backend for the families
+ * int2e_sph, int2e_ip1_sph, int3c2e_sph and int3c2e_ip1_sph.
+ *
+ * Three layers, each a drop-in for a reference interface:
+ *
+ *   1. libcint raw signature (pyscf-style callers, cintx-compat legacy
+ *      wrappers): int2e_sph & co.  Same arguments, output layout, return value
+ *      (not0) and out == NULL cache-size query as libcint 6.1.3.
+ *   2. cintx C shim: cintrs_eval / cintrs_query_workspace / last-error
+ *      accessors with the same status codes and api ids as
+ *      crates/cintx-capi/src/shim.rs and errors.rs.
+ *   3. Batched plans (the executor boundary, crates/cintx-runtime/src/
+ *      dispatch.rs:187-195 BackendExecutor): device-resident basis + pair
+ *      data (CINTOpt analogue), class-binned tuple lists, device output.
+ *
+ * Integer types follow libcint's FINT = int build (crates/cintx-oracle/
+ * build.rs:184-193) and CACHE_SIZE_T = int.
+ */
+#ifndef CINTX_HIP_H
+#define CINTX_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* 1. libcint raw signature                                            */
+/* ------------------------------------------------------------------ */
+
+/* Opaque optimizer handle; replaces CINTOpt (libcint-master/include/
+ * cint.h.in:145-152) and cintx's RawOptimizerHandle
+ * (crates/cintx-compat/src/optimizer.rs:15). */
+typedef struct CINTOpt CINTOpt;
+
+/* libcint-master/src/cint2e.c:1186-1193 */
+int int2e_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+              double *env, CINTOpt *opt, double *cache);
+/* libcint-master/src/autocode/grad2.c:81-88 */
+int int2e_ip1_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                  double *env, CINTOpt *opt, double *cache);
+/* libcint-master/src/cint3c2e.c:693-700 */
+int int3c2e_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                double *env, CINTOpt *opt, double *cache);
+/* libcint-master/src/autocode/int3c2e.c:80-87 */
+int int3c2e_ip1_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                    double *env, CINTOpt *opt, double *cache);
+
+/* libcint-master/src/cint2e.c:1195, autocode/grad2.c:69,
+ * cint3c2e.c:702, autocode/int3c2e.c:68 */
+void int2e_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
+void int2e_ip1_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
+void int3c2e_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
+void int3c2e_ip1_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
+/* libcint-master/src/optimizer.c:69 */
+void CINTdel_optimizer(CINTOpt **opt);
+
+/* ------------------------------------------------------------------ */
+/* 2. cintx C shim (crates/cintx-capi/src/shim.rs, errors.rs)           */
+/* ------------------------------------------------------------------ */
+
+/* api ids: crates/cintx-capi/src/shim.rs:8-49.  The reference enum has no
+ * int3c2e_sph entry; CINTX_API_INT3C2E_SPH is this library's extension. */
+#define CINTX_API_INT2E_SPH 10
+#define CINTX_API_INT3C2E_IP1_SPH 19
+#define CINTX_API_INT2E_IP1_SPH 36
+#define CINTX_API_INT3C2E_SPH 40
+
+/* status codes: crates/cintx-capi/src/errors.rs:9-21 */
+#define CINTX_STATUS_SUCCESS 0
+#define CINTX_STATUS_INVALID_INPUT 1
+#define CINTX_STATUS_UNSUPPORTED_API 2
+#define CINTX_STATUS_UNSUPPORTED_REPRESENTATION 3
+#define CINTX_STATUS_BUFFER_TOO_SMALL 4
+#define CINTX_STATUS_MEMORY_LIMIT_EXCEEDED 5
+#define CINTX_STATUS_ALLOCATION_FAILED 6
+#define CINTX_STATUS_EXECUTION_FAILED 7
+#define CINTX_STATUS_NULL_POINTER 8
+#define CINTX_STATUS_PANIC 9
+#define CINTX_STATUS_BACKEND_NOT_COMPILED 10
+
+/* crates/cintx-capi/src/shim.rs:51-66 */
+typedef struct {
+  size_t bytes;
+  size_t alignment;
+  size_t required_bytes;
+  size_t chunk_count;
+  size_t work_units;
+} CintxWorkspaceQuery;
+
+typedef struct {
+  int32_t not0;
+  size_t bytes_written;
+  size_t workspace_bytes;
+} CintxEvalSummary;
+
+/* crates/cintx-capi/src/shim.rs:313-327 */
+int cintrs_query_workspace(int api, const int *dims, size_t dims_len, const int *shls,
+                           size_t shls_len, const int *atm, size_t atm_len, const int *bas,
+                           size_t bas_len, const double *env, size_t env_len, const void *opt,
+                           CintxWorkspaceQuery *query_out);
+/* crates/cintx-capi/src/shim.rs:371-389 */
+int cintrs_eval(int api, double *out, size_t out_len, const int *dims, size_t dims_len,
+                const int *shls, size_t shls_len, const int *atm, size_t atm_len, const int *bas,
+                size_t bas_len, const double *env, size_t env_len, const void *opt, double *cache,
+                size_t cache_len, CintxEvalSummary *summary_out);
+/* crates/cintx-capi/src/errors.rs:211-245 */
+int cintrs_last_error_code(void);
+void cintrs_clear_last_error(void);
+size_t cintrs_copy_last_error_message(char *out, size_t out_len);
+
+/* ------------------------------------------------------------------ */
+/* 3. Batched plans (executor boundary)                                 */
+/* ------------------------------------------------------------------ */
+
+/* Family ids of a plan (the hot-path rows of crates/cintx-ops/
+ * src/generated/api_manifest.csv:12, 22, 25, 225). */
+#define CINTX_FAMILY_INT2E_SPH 0
+#define CINTX_FAMILY_INT2E_IP1_SPH 1
+#define CINTX_FAMILY_INT3C2E_SPH 2
+#define CINTX_FAMILY_INT3C2E_IP1_SPH 3
+
+/* precision modes: FP64 everywhere, or the mixed mode (FP32 recurrences and
+ * primitive sums for tuples whose Schwarz bound is below the plan's
+ * mixed_threshold, FP64 for the rest and for roots/weights/accumulation). */
+#define CINTX_PREC_FP64 0
+#define CINTX_PREC_MIXED 1
+
+typedef struct cintx_plan cintx_plan;
+
+/* Explicit tuple list.  tuples holds ntuples x 4 ints (3c2e: the 4th is
+ * ignored).  Block t of the output holds the tuple's libcint-layout result
+ * (natural dims, components leading) at element offset offsets[t]
+ * (cintx_plan_offsets).  Replaces cintx-cubecl's per-tuple launch
+ * (crates/cintx-cubecl/src/kernels/two_electron.rs:1309-1430). */
+int cintx_plan_create(cintx_plan **plan, int family, const int *atm, int natm, const int *bas,
+                      int nbas, const double *env, const int *tuples, int64_t ntuples,
+                      int precision, double mixed_threshold);
+/* Unique 8-fold int2e_sph tuples (i>=j, k>=l, ij>=kl) whose Schwarz bound
+ * sqrt((ij|ij)) sqrt((kl|kl)) is at least `threshold`; bra pairs restricted to
+ * every `bra_stride`-th screened pair (1 = all).  Tuples are generated on the
+ * host and kept in HBM. */
+int cintx_plan_create_screened(cintx_plan **plan, const int *atm, int natm, const int *bas,
+                               int nbas, const double *env, double threshold, int bra_stride,
+                               int precision, double mixed_threshold);
+int cintx_plan_destroy(cintx_plan *plan);
+/* total output elements and tuples */
+int cintx_plan_sizes(const cintx_plan *plan, int64_t *nelem, int64_t *ntuples);
+/* host copies of the tuple list (ntuples x 4) and offsets (ntuples) */
+int cintx_plan_tuples(const cintx_plan *plan, int *tuples, int64_t *offsets);
+/* run all class bins on the plan's stream into device memory `out_dev`
+ * (NULL: the plan's own buffer); asynchronous */
+int cintx_plan_execute(cintx_plan *plan, double *out_dev);
+int cintx_plan_sync(cintx_plan *plan);
+int cintx_plan_copy_output(cintx_plan *plan, double *host_out);
+/* Record the plan's launches into a hipGraph once and replay it. */
+int cintx_plan_execute_graph(cintx_plan *plan, double *out_dev);
+/* Timing of the last execute (HIP events on the plan's stream), the
+ * per-kernel-launch records, and the algorithmic work of the plan
+ * (surviving primitive tuples, root evaluations, FLOPs). */
+int cintx_plan_last_time_ms(const cintx_plan *plan, double *ms);
+int cintx_plan_work(cintx_plan *plan, double *prim_tuples, double *root_evals, double *flops);
+
+/* The n-point Rys rule (roots u = t^2/(1-t^2), weights) that the kernels use,
+ * evaluated for nx arguments on the host or on the device; replaces
+ * CINTrys_roots (libcint-master/src/rys_roots.c:57-123).  u, w: nx x n. */
+int cintx_rys_eval_host(int n, const double *x, int64_t nx, double *u, double *w);
+int cintx_rys_eval_device(int n, const double *x, int64_t nx, double *u, double *w);
+
+/* Device memory helpers for callers without a runtime of their own. */
+int cintx_device_alloc(double **ptr, int64_t nelem);
+int cintx_device_free(double *ptr);
+int cintx_device_count(int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

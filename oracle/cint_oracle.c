@@ -1,0 +1,994 @@
+/*
+ * cint_oracle.c -- CPU oracle for the Rys two-electron path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see cint_oracle.h).  Restates, in plain C, the
+ * libcint 6.1.3 algorithm that cintx's CubeCL kernels port:
+ *
+ *   Boys function                 fmt.c:186-226
+ *   Rys roots and weights         rys_roots.c:57-123 (mathematical object;
+ *                                 computed here by the discretised Stieltjes
+ *                                 procedure in 80-bit long double)
+ *   primitive pair data, screens  optimizer.c:248-342, cint2e.c:57-250,
+ *                                 cint3c2e.c:56-210
+ *   per-root coefficients         g2e.c:4425-4571 (CINTg0_2e)
+ *   2D VRR                        g2e.c:272-411 (CINTg0_2e_2d)
+ *   4D HRR (ibase/kbase choice)   g2e.c:102-150, 428-690
+ *   gout / nabla_i                cint2e.c:961-1180, g2e.c:4574-4620,
+ *                                 autocode/grad2.c:19-74,
+ *                                 autocode/int3c2e.c:18-73
+ *   contraction                   cint2e.c PRIM2CTR
+ *   cart -> sph                   cart2sph.c:21 (table), 5324-5376, 5884-5923
+ *   entry points / zero fill      cint2e.c:794-869, 1186-1193,
+ *                                 cint3c2e.c:556-625, 693-700
+ *
+ * The memory layout of the intermediate g tensor is this file's own
+ * ([axis][root][i][j][k][l]); only the arithmetic and the output layout follow
+ * libcint.
+ */
+#include <math.h>
+#include <float.h>
+#include <stdlib.h>
+#include <string.h>
+#include "cint_oracle.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define EXPCUTOFF 60.0        /* cint_config.h.in: EXPCUTOFF */
+#define MIN_EXPCUTOFF 40.0    /* cint_config.h.in: MIN_EXPCUTOFF */
+#define LMAX_ORC 8            /* largest l handled (ceil, i.e. incl. +1) */
+#define NCART_MAX ((LMAX_ORC + 1) * (LMAX_ORC + 2) / 2)
+#define MAXPRIM 64
+
+static inline int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+static inline int nsph(int l) { return 2 * l + 1; }
+
+/* ------------------------------------------------------------------ */
+/* Boys function (fmt.c:186-226): series + downward recursion for small
+ * x, erf start + upward recursion for large x.  Long double.          */
+/* ------------------------------------------------------------------ */
+static void boys_ld(int mmax, long double x, long double *f)
+{
+    int i;
+    if (x == 0.0L) {
+        for (i = 0; i <= mmax; i++) f[i] = 1.0L / (2 * i + 1);
+        return;
+    }
+    long double e = expl(-x);
+    if (x < 40.0L + 2.0L * mmax) {
+        /* F_m(x) = e^{-x} sum_k (2x)^k / ((2m+1)(2m+3)...(2m+2k+1)) */
+        long double b = mmax + 0.5L;
+        long double term = 1.0L / b;
+        long double s = term;
+        long double bi = b;
+        for (int k = 0; k < 4000; k++) {
+            bi += 1.0L;
+            term *= x / bi;
+            s += term;
+            if (term < s * 1e-22L) break;
+        }
+        f[mmax] = 0.5L * e * s;
+        for (i = mmax; i > 0; i--) {
+            f[i - 1] = (2.0L * x * f[i] + e) / (2 * i - 1);
+        }
+    } else {
+        long double sx = sqrtl(x);
+        f[0] = 0.5L * sqrtl(3.14159265358979323846264338327950288L) / sx * erfl(sx);
+        for (i = 1; i <= mmax; i++) {
+            f[i] = ((2 * i - 1) * f[i - 1] - e) / (2.0L * x);
+        }
+    }
+}
+
+void orc_boys(int mmax, double x, double *f)
+{
+    long double fl[128];
+    if (mmax > 127) mmax = 127;
+    boys_ld(mmax, (long double)x, fl);
+    for (int i = 0; i <= mmax; i++) f[i] = (double)fl[i];
+}
+
+/* ------------------------------------------------------------------ */
+/* Rys quadrature.  The n-point Rys rule for argument x is the Gauss rule
+ * of the measure exp(-x t^2) dt on t in [0,1], written in s = t^2:
+ *     sum_i w_i s_i^m = F_m(x),  m = 0..2n-1,   u_i = s_i / (1 - s_i).
+ * It is computed by discretising the measure with Gauss-Legendre nodes in t
+ * and running the Stieltjes procedure (Gautschi, "On generating orthogonal
+ * polynomials", SIAM J. Sci. Stat. Comput. 3 (1982)) followed by the
+ * Golub-Welsch eigen-solve of the Jacobi matrix (implicit QL with Wilkinson
+ * shifts).  Everything in 80-bit long double.                          */
+/* ------------------------------------------------------------------ */
+#define GL_MAXN 256
+static long double gl_t[GL_MAXN + 1][GL_MAXN];
+static long double gl_w[GL_MAXN + 1][GL_MAXN];
+static volatile int gl_ready[GL_MAXN + 1];
+
+static void gauss_legendre_01(int M, long double *t, long double *w)
+{
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for (int k = 0; k < (M + 1) / 2; k++) {
+        long double z = cosl(pi * (k + 0.75L) / (M + 0.5L));
+        long double dp = 1;
+        for (int it = 0; it < 100; it++) {
+            long double p0 = 1, p1 = z;
+            for (int j = 2; j <= M; j++) {
+                long double p2 = ((2 * j - 1) * z * p1 - (j - 1) * p0) / j;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = M * (z * p1 - p0) / (z * z - 1);
+            long double dz = p1 / dp;
+            z -= dz;
+            if (fabsl(dz) < 1e-21L) break;
+        }
+        {
+            long double p0 = 1, p1 = z;
+            for (int j = 2; j <= M; j++) {
+                long double p2 = ((2 * j - 1) * z * p1 - (j - 1) * p0) / j;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = M * (z * p1 - p0) / (z * z - 1);
+        }
+        long double wk = 2.0L / ((1 - z * z) * dp * dp);
+        /* map [-1,1] -> [0,1] */
+        t[k] = 0.5L * (1 - z);
+        w[k] = 0.5L * wk;
+        t[M - 1 - k] = 0.5L * (1 + z);
+        w[M - 1 - k] = 0.5L * wk;
+    }
+}
+
+static const long double *gl_nodes(int M, const long double **wout)
+{
+    if (!gl_ready[M]) {
+#pragma omp critical(orc_gl)
+        {
+            if (!gl_ready[M]) {
+                gauss_legendre_01(M, gl_t[M], gl_w[M]);
+                __sync_synchronize();
+                gl_ready[M] = 1;
+            }
+        }
+    }
+    *wout = gl_w[M];
+    return gl_t[M];
+}
+
+/* Symmetric tridiagonal eigen-solve by implicit QL; only the first row of the
+ * eigenvector matrix is tracked (that is all Golub-Welsch needs). */
+static void tridiag_ql(int n, long double *d, long double *e, long double *z0)
+{
+    for (int i = 0; i < n; i++) z0[i] = (i == 0);
+    if (n == 1) return;
+    e[n - 1] = 0;
+    for (int l = 0; l < n; l++) {
+        int iter = 0;
+        int m;
+        do {
+            for (m = l; m < n - 1; m++) {
+                long double dd = fabsl(d[m]) + fabsl(d[m + 1]);
+                if (fabsl(e[m]) <= LDBL_EPSILON * dd) break;
+            }
+            if (m != l) {
+                if (iter++ > 200) break;
+                long double g = (d[l + 1] - d[l]) / (2.0L * e[l]);
+                long double r = hypotl(g, 1.0L);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+                long double s = 1, c = 1, p = 0;
+                int i;
+                int early = 0;
+                for (i = m - 1; i >= l; i--) {
+                    long double f = s * e[i];
+                    long double b = c * e[i];
+                    r = hypotl(f, g);
+                    e[i + 1] = r;
+                    if (r == 0) {
+                        d[i + 1] -= p;
+                        e[m] = 0;
+                        early = 1;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0L * c * b;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - b;
+                    long double zf = z0[i + 1];
+                    z0[i + 1] = s * z0[i] + c * zf;
+                    z0[i] = c * z0[i] - s * zf;
+                }
+                if (early) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0;
+            }
+        } while (m != l);
+    }
+}
+
+int orc_rys_roots_m(int n, double xd, double *u, double *w, int nodes)
+{
+    long double x = xd;
+    /* the measure beyond t_max carries < e^{-(100+4n)} of any moment used */
+    long double T = 100.0L + 4.0L * n;
+    long double tmax = 1.0L;
+    if (x * tmax * tmax > T) tmax = sqrtl(T / x);
+    long double xeff = x * tmax * tmax;
+    int M = nodes;
+    if (M <= 0) {
+        M = 24 + 2 * n + (int)(1.5L * xeff);
+        if (M > GL_MAXN) M = GL_MAXN;
+    }
+    const long double *gw;
+    const long double *gt = gl_nodes(M, &gw);
+    long double s[GL_MAXN], W[GL_MAXN], p0[GL_MAXN], p1[GL_MAXN];
+    long double mu0 = 0;
+    for (int k = 0; k < M; k++) {
+        long double t = gt[k] * tmax;
+        s[k] = t * t;
+        W[k] = gw[k] * tmax * expl(-x * s[k]);
+        p0[k] = 0;
+        p1[k] = 1;
+        mu0 += W[k];
+    }
+    long double alpha[ORC_MAX_ROOTS], beta[ORC_MAX_ROOTS];
+    long double norm = mu0;
+    for (int j = 0; j < n; j++) {
+        long double num = 0;
+        for (int k = 0; k < M; k++) num += W[k] * s[k] * p1[k] * p1[k];
+        alpha[j] = num / norm;
+        if (j == n - 1) break;
+        long double bj = (j == 0) ? 0.0L : beta[j];
+        long double nn = 0;
+        for (int k = 0; k < M; k++) {
+            long double pn = (s[k] - alpha[j]) * p1[k] - bj * p0[k];
+            p0[k] = p1[k];
+            p1[k] = pn;
+            nn += W[k] * pn * pn;
+        }
+        beta[j + 1] = nn / norm;
+        norm = nn;
+    }
+    long double d[ORC_MAX_ROOTS], e[ORC_MAX_ROOTS], z0[ORC_MAX_ROOTS];
+    for (int j = 0; j < n; j++) {
+        d[j] = alpha[j];
+        e[j] = (j + 1 < n) ? sqrtl(beta[j + 1]) : 0.0L;
+    }
+    tridiag_ql(n, d, e, z0);
+    /* sort ascending */
+    int idx[ORC_MAX_ROOTS];
+    for (int i = 0; i < n; i++) idx[i] = i;
+    for (int i = 1; i < n; i++) {
+        int v = idx[i], j = i - 1;
+        while (j >= 0 && d[idx[j]] > d[v]) {
+            idx[j + 1] = idx[j];
+            j--;
+        }
+        idx[j + 1] = v;
+    }
+    for (int i = 0; i < n; i++) {
+        long double si = d[idx[i]];
+        long double wi = mu0 * z0[idx[i]] * z0[idx[i]];
+        u[i] = (double)(si / (1.0L - si));
+        w[i] = (double)wi;
+    }
+    return 0;
+}
+
+int orc_rys_roots(int n, double x, double *u, double *w)
+{
+    return orc_rys_roots_m(n, x, u, w, 0);
+}
+
+/* ------------------------------------------------------------------ */
+/* Cartesian -> real spherical harmonics (cart2sph.c:21 table).  Rows m =
+ * -l..l, columns in libcint Cartesian order (cint_bas.c:166-180: lx from l
+ * down, then ly from l-lx down).  s and p are the identity; libcint folds
+ * their Y_lm normalisation into CINTcommon_fac_sp (g1e.c:566).  For l >= 2
+ * the coefficient of x^a y^b z^c in r^l Y_lm (unit-sphere normalised real
+ * harmonics, sin(|m| phi) for m < 0) is built from the closed form of the
+ * real solid harmonics (Helgaker, Jorgensen, Olsen, "Molecular
+ * Electronic-Structure Theory", eq. 6.4.47-6.4.50) times sqrt((2l+1)/4pi). */
+/* ------------------------------------------------------------------ */
+static double c2s_tab[LMAX_ORC + 1][2 * LMAX_ORC + 1][NCART_MAX];
+static volatile int c2s_ready = 0;
+
+static long double binoml(int n, int k)
+{
+    if (k < 0 || k > n) return 0;
+    long double r = 1;
+    for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+    return r;
+}
+static long double factl(int n)
+{
+    long double r = 1;
+    for (int i = 2; i <= n; i++) r *= i;
+    return r;
+}
+static int cart_index(int l, int lx, int ly)
+{
+    /* position of (lx, ly, l-lx-ly) in libcint order */
+    int pos = 0;
+    for (int ax = l; ax > lx; ax--) pos += l - ax + 1;
+    return pos + (l - lx - ly);
+}
+
+static void build_c2s(void)
+{
+    const long double pi = 3.14159265358979323846264338327950288L;
+    memset(c2s_tab, 0, sizeof(c2s_tab));
+    c2s_tab[0][0][0] = 1.0;
+    for (int i = 0; i < 3; i++) c2s_tab[1][i][i] = 1.0;
+    for (int l = 2; l <= LMAX_ORC; l++) {
+        long double ynorm = sqrtl((2 * l + 1) / (4 * pi));
+        for (int m = -l; m <= l; m++) {
+            int am = abs(m);
+            int vm2 = (m < 0) ? 1 : 0; /* 2*v_m */
+            long double nlm = 1.0L / (powl(2.0L, am) * factl(l)) *
+                              sqrtl(2.0L * factl(l + am) * factl(l - am) /
+                                    (m == 0 ? 2.0L : 1.0L));
+            for (int t = 0; t <= (l - am) / 2; t++) {
+                for (int uu = 0; uu <= t; uu++) {
+                    /* v = vv + v_m, vv = 0 .. floor(|m|/2 - v_m) */
+                    int vvmax = (am - vm2) / 2;
+                    if (am - vm2 < 0) vvmax = -1;
+                    for (int vv = 0; vv <= vvmax; vv++) {
+                        int v2 = 2 * vv + vm2; /* 2v */
+                        long double c = ((t + vv) % 2 ? -1.0L : 1.0L) * powl(0.25L, t) *
+                                        binoml(l, t) * binoml(l - t, am + t) * binoml(t, uu) *
+                                        binoml(am, v2);
+                        int px = 2 * t + am - 2 * uu - v2;
+                        int py = 2 * uu + v2;
+                        if (px < 0) continue;
+                        int col = cart_index(l, px, py);
+                        c2s_tab[l][m + l][col] += (double)(ynorm * nlm * c);
+                    }
+                }
+            }
+        }
+    }
+}
+
+static void ensure_c2s(void)
+{
+    if (!c2s_ready) {
+#pragma omp critical(orc_c2s)
+        {
+            if (!c2s_ready) {
+                build_c2s();
+                __sync_synchronize();
+                c2s_ready = 1;
+            }
+        }
+    }
+}
+
+double orc_c2s_coef(int l, int m, int c)
+{
+    ensure_c2s();
+    if (l < 0 || l > LMAX_ORC || m < 0 || m >= nsph(l) || c < 0 || c >= ncart(l)) return 0;
+    return c2s_tab[l][m][c];
+}
+
+/* ------------------------------------------------------------------ */
+/* Shell access (cint.h.in:49-67 slots)                               */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int l, nprim, nctr;
+    const double *r, *a, *c; /* c column-major: c[ctr*nprim + prim] */
+} Shl;
+
+static void get_shl(Shl *s, int sh, const int *atm, const int *bas, const double *env)
+{
+    const int *b = bas + sh * ORC_BAS_SLOTS;
+    s->l = b[1];
+    s->nprim = b[2];
+    s->nctr = b[3];
+    s->r = env + atm[b[0] * ORC_ATM_SLOTS + 1];
+    s->a = env + b[5];
+    s->c = env + b[6];
+}
+
+static double common_fac_sp(int l) /* g1e.c:566-573 */
+{
+    if (l == 0) return 0.282094791773878143;
+    if (l == 1) return 0.488602511902919921;
+    return 1.0;
+}
+
+static void log_max_coeff(double *out, const Shl *s) /* optimizer.c:248-261 */
+{
+    for (int ip = 0; ip < s->nprim; ip++) {
+        double m = 0;
+        for (int ic = 0; ic < s->nctr; ic++) {
+            double v = fabs(s->c[ic * s->nprim + ip]);
+            if (v > m) m = v;
+        }
+        out[ip] = log(m);
+    }
+}
+
+static void cart_comp(int l, int *nx, int *ny, int *nz) /* cint_bas.c:166-180 */
+{
+    int n = 0;
+    for (int lx = l; lx >= 0; lx--)
+        for (int ly = l - lx; ly >= 0; ly--) {
+            nx[n] = lx;
+            ny[n] = ly;
+            nz[n] = l - lx - ly;
+            n++;
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* One primitive tuple: Rys roots, per-root coefficients, 2D VRR and the
+ * two HRR transfers.  G[((axis*nr + r)*NI + i)*NJ + j)*NK + k)*NL + l].   */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int li, lj, lk, ll; /* ceil values */
+    int nroots;
+    int ibase, kbase;
+    const double *ri, *rj, *rk, *rl;
+} GDesc;
+
+static inline size_t gidx(const GDesc *d, int ax, int r, int i, int j, int k, int l)
+{
+    size_t NI = d->li + 1, NJ = d->lj + 1, NK = d->lk + 1, NL = d->ll + 1;
+    return ((((size_t)(ax * d->nroots + r) * NI + i) * NJ + j) * NK + k) * NL + l;
+}
+
+/* H(a, b+1) = H(a+1, b) + dr * H(a, b): a runs over the base centre's powers,
+ * b over the other centre's (g2e.c:428-690).  src[a] for a = 0..A+B with
+ * stride sstride; dst[a*(B+1)+b]. */
+static void hrr_1d(double *dst, const double *src, int sstride, int A, int B, double dr)
+{
+    double tmp[(2 * LMAX_ORC + 2) * (LMAX_ORC + 2)];
+    int W = A + B + 1;
+    for (int a = 0; a < W; a++) tmp[a * (B + 1) + 0] = src[a * sstride];
+    for (int b = 0; b < B; b++)
+        for (int a = 0; a < W - b - 1; a++)
+            tmp[a * (B + 1) + b + 1] = tmp[(a + 1) * (B + 1) + b] + dr * tmp[a * (B + 1) + b];
+    for (int a = 0; a <= A; a++)
+        for (int b = 0; b <= B; b++) dst[a * (B + 1) + b] = tmp[a * (B + 1) + b];
+}
+
+static void build_g(double *G, const GDesc *d, double aij, double akl, const double *P,
+                    const double *Q, double fac)
+{
+    const int nr = d->nroots;
+    const int nmax = d->li + d->lj, mmax = d->lk + d->ll;
+    double dx[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
+    double rr = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+    double a1 = aij * akl;
+    double a0 = a1 / (aij + akl);
+    double fac1 = sqrt(a0 / (a1 * a1 * a1)) * fac;
+    double x = a0 * rr;
+    double u[ORC_MAX_ROOTS], w[ORC_MAX_ROOTS];
+    orc_rys_roots(nr, x, u, w);
+    const double *Rb = d->ibase ? d->ri : d->rj;
+    const double *Rq = d->kbase ? d->rk : d->rl;
+    double g2[(2 * LMAX_ORC + 2) * (2 * LMAX_ORC + 2)];
+    double h1[(LMAX_ORC + 1) * (LMAX_ORC + 1) * (2 * LMAX_ORC + 2)];
+    const int W2 = mmax + 1;
+
+    for (int r = 0; r < nr; r++) {
+        /* g2e.c:4523-4543 */
+        double u2 = a0 * u[r];
+        double tmp4 = .5 / (u2 * (aij + akl) + a1);
+        double tmp5 = u2 * tmp4;
+        double tmp1 = 2. * tmp5;
+        double tmp2 = tmp1 * akl;
+        double tmp3 = tmp1 * aij;
+        double b00 = tmp5;
+        double b10 = tmp5 + tmp4 * akl;
+        double b01 = tmp5 + tmp4 * aij;
+        for (int ax = 0; ax < 3; ax++) {
+            double c00 = (P[ax] - Rb[ax]) - tmp2 * dx[ax];
+            double c0p = (Q[ax] - Rq[ax]) + tmp3 * dx[ax];
+            /* 2D VRR (g2e.c:272-411), g2[n*W2 + m] */
+            g2[0] = (ax == 2) ? w[r] * fac1 : 1.0;
+            if (nmax > 0) g2[1 * W2] = c00 * g2[0];
+            for (int n = 1; n < nmax; n++)
+                g2[(n + 1) * W2] = c00 * g2[n * W2] + n * b10 * g2[(n - 1) * W2];
+            for (int m = 0; m < mmax; m++) {
+                for (int n = 0; n <= nmax; n++) {
+                    double v = c0p * g2[n * W2 + m];
+                    if (m > 0) v += m * b01 * g2[n * W2 + m - 1];
+                    if (n > 0) v += n * b00 * g2[(n - 1) * W2 + m];
+                    g2[n * W2 + m + 1] = v;
+                }
+            }
+            /* bra HRR: for every ket power m, transfer from the bra base */
+            int A = d->ibase ? d->li : d->lj;
+            int B = d->ibase ? d->lj : d->li;
+            double drb = d->ibase ? d->ri[ax] - d->rj[ax] : d->rj[ax] - d->ri[ax];
+            /* h1[(i*(lj+1) + j)*W2 + m] */
+            for (int m = 0; m <= mmax; m++) {
+                double hb[(LMAX_ORC + 1) * (LMAX_ORC + 1)];
+                hrr_1d(hb, g2 + m, W2, A, B, drb);
+                for (int a = 0; a <= A; a++)
+                    for (int b = 0; b <= B; b++) {
+                        int i = d->ibase ? a : b;
+                        int j = d->ibase ? b : a;
+                        h1[(i * (d->lj + 1) + j) * W2 + m] = hb[a * (B + 1) + b];
+                    }
+            }
+            /* ket HRR */
+            int C = d->kbase ? d->lk : d->ll;
+            int D = d->kbase ? d->ll : d->lk;
+            double drk = d->kbase ? d->rk[ax] - d->rl[ax] : d->rl[ax] - d->rk[ax];
+            for (int i = 0; i <= d->li; i++)
+                for (int j = 0; j <= d->lj; j++) {
+                    double hk[(LMAX_ORC + 1) * (LMAX_ORC + 1)];
+                    hrr_1d(hk, h1 + (i * (d->lj + 1) + j) * W2, 1, C, D, drk);
+                    for (int c = 0; c <= C; c++)
+                        for (int e = 0; e <= D; e++) {
+                            int k = d->kbase ? c : e;
+                            int l = d->kbase ? e : c;
+                            G[gidx(d, ax, r, i, j, k, l)] = hk[c * (D + 1) + e];
+                        }
+                }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Shell-tuple driver shared by the four families.                      */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    int three_center; /* int3c2e: ket is one Gaussian on shell k */
+    int deriv_i;      /* 1: nabla_i, three components (grad2.c:19-88) */
+    int spherical;    /* 1: sph output, 0: cart */
+} Family;
+
+typedef struct {
+    double cce;
+    double e;
+    double P[3];
+} PairPrim;
+
+/* CINTset_pairdata (optimizer.c:288-342); returns 1 if all pairs screened. */
+static int set_pairdata(PairPrim *pd, const Shl *si, const Shl *sj, int li_ceil, int lj_ceil,
+                        double expcutoff)
+{
+    double lci[MAXPRIM], lcj[MAXPRIM];
+    log_max_coeff(lci, si);
+    log_max_coeff(lcj, sj);
+    double rr = 0;
+    for (int d = 0; d < 3; d++) rr += (si->r[d] - sj->r[d]) * (si->r[d] - sj->r[d]);
+    double aij_last = si->a[si->nprim - 1] + sj->a[sj->nprim - 1];
+    double log_rr_ij = 1.7 - 1.5 * log(aij_last);
+    int lij = li_ceil + lj_ceil;
+    if (lij > 0) log_rr_ij += lij * log(sqrt(rr) + 1.);
+    int empty = 1;
+    int n = 0;
+    for (int jp = 0; jp < sj->nprim; jp++)
+        for (int ip = 0; ip < si->nprim; ip++, n++) {
+            double ainv = 1.0 / (si->a[ip] + sj->a[jp]);
+            double eij = rr * si->a[ip] * sj->a[jp] * ainv;
+            double cce = eij - log_rr_ij - lci[ip] - lcj[jp];
+            pd[n].cce = cce;
+            if (cce < expcutoff) {
+                empty = 0;
+                double wj = sj->a[jp] * ainv;
+                for (int d = 0; d < 3; d++) pd[n].P[d] = si->r[d] + wj * (sj->r[d] - si->r[d]);
+                pd[n].e = exp(-eij);
+            } else {
+                pd[n].P[0] = pd[n].P[1] = pd[n].P[2] = 1e18;
+                pd[n].e = 0;
+            }
+        }
+    return empty;
+}
+
+/* Apply per-axis transforms: src[l][k][j][i] (i fastest) with sizes
+ * (ci,cj,ck,cl) -> dst with sizes (si,sj,sk,sl) using T_x[s][c]. */
+static void transform4(double *dst, const double *src, const int lv[4], int spherical, double *work)
+{
+    int cin[4], cout[4];
+    for (int a = 0; a < 4; a++) {
+        cin[a] = ncart(lv[a]);
+        cout[a] = spherical ? nsph(lv[a]) : ncart(lv[a]);
+    }
+    if (!spherical) {
+        size_t n = (size_t)cin[0] * cin[1] * cin[2] * cin[3];
+        memcpy(dst, src, n * sizeof(double));
+        return;
+    }
+    ensure_c2s();
+    size_t nall = (size_t)cin[0] * cin[1] * cin[2] * cin[3];
+    double *bufA = work, *bufB = work + nall;
+    const double *cur = src;
+    int dim[4] = {cin[0], cin[1], cin[2], cin[3]};
+    for (int a = 0; a < 4; a++) {
+        double *nxt = (a == 3) ? dst : ((a % 2 == 0) ? bufA : bufB);
+        size_t inner = 1, outer = 1;
+        for (int b = 0; b < a; b++) inner *= dim[b];
+        for (int b = a + 1; b < 4; b++) outer *= dim[b];
+        int nc = dim[a], ns = cout[a];
+        for (size_t o = 0; o < outer; o++)
+            for (int s = 0; s < ns; s++)
+                for (size_t in = 0; in < inner; in++) {
+                    double acc = 0;
+                    for (int c = 0; c < nc; c++) {
+                        double t = c2s_tab[lv[a]][s][c];
+                        if (t != 0.0) acc += t * cur[(o * nc + c) * inner + in];
+                    }
+                    nxt[(o * ns + s) * inner + in] = acc;
+                }
+        dim[a] = ns;
+        cur = nxt;
+    }
+}
+
+static int tuple_driver(const Family *fam, double *out, const int *dims, const int *shls,
+                        const int *atm, const int *bas, const double *env)
+{
+    Shl s[4];
+    int nsh = fam->three_center ? 3 : 4;
+    for (int a = 0; a < nsh; a++) get_shl(&s[a], shls[a], atm, bas, env);
+    static const double zero3[3] = {0, 0, 0};
+    static const double one = 1.0;
+    if (fam->three_center) {
+        /* a fourth "shell": s-type, exponent 0, coefficient 1, on k's centre */
+        s[3].l = 0;
+        s[3].nprim = 1;
+        s[3].nctr = 1;
+        s[3].r = s[2].r;
+        s[3].a = zero3;
+        s[3].c = &one;
+    }
+    int lv[4] = {s[0].l, s[1].l, s[2].l, s[3].l};
+    int nctr[4] = {s[0].nctr, s[1].nctr, s[2].nctr, s[3].nctr};
+    int ncomp = fam->deriv_i ? 3 : 1;
+    int cnt[4];
+    for (int a = 0; a < 4; a++)
+        cnt[a] = (fam->spherical ? nsph(lv[a]) : ncart(lv[a])) * nctr[a];
+    int dd[4];
+    if (dims) {
+        for (int a = 0; a < nsh; a++) dd[a] = dims[a];
+        if (nsh == 3) dd[3] = 1;
+    } else {
+        for (int a = 0; a < 4; a++) dd[a] = cnt[a];
+    }
+    size_t nout = (size_t)dd[0] * dd[1] * dd[2] * dd[3];
+
+    GDesc gd;
+    gd.li = lv[0] + fam->deriv_i;
+    gd.lj = lv[1];
+    gd.lk = lv[2];
+    gd.ll = lv[3];
+    gd.nroots = (gd.li + gd.lj + gd.lk + gd.ll) / 2 + 1;
+    gd.ibase = gd.li > gd.lj;
+    gd.kbase = gd.lk > gd.ll;
+    gd.ri = s[0].r;
+    gd.rj = s[1].r;
+    gd.rk = s[2].r;
+    gd.rl = s[3].r;
+
+    double common = (M_PI * M_PI * M_PI) * 2 / sqrt(M_PI) * common_fac_sp(lv[0]) *
+                    common_fac_sp(lv[1]) * common_fac_sp(lv[2]);
+    if (!fam->three_center) common *= common_fac_sp(lv[3]);
+    double expcutoff;
+    if (env[ORC_PTR_EXPCUTOFF] == 0)
+        expcutoff = EXPCUTOFF;
+    else {
+        expcutoff = env[ORC_PTR_EXPCUTOFF] > MIN_EXPCUTOFF ? env[ORC_PTR_EXPCUTOFF] : MIN_EXPCUTOFF;
+        if (!fam->three_center) expcutoff += 1; /* g2e.c:57-62 vs g3c2e.c:56-60 */
+    }
+
+    int nf[4];
+    for (int a = 0; a < 4; a++) nf[a] = ncart(lv[a]);
+    size_t nfall = (size_t)nf[0] * nf[1] * nf[2] * nf[3];
+    size_t nctr_all = (size_t)nctr[0] * nctr[1] * nctr[2] * nctr[3];
+    size_t gsz = (size_t)3 * gd.nroots * (gd.li + 1) * (gd.lj + 1) * (gd.lk + 1) * (gd.ll + 1);
+
+    PairPrim *pdij = malloc(sizeof(PairPrim) * s[0].nprim * s[1].nprim);
+    double *G = malloc(sizeof(double) * gsz);
+    double *gout = malloc(sizeof(double) * nfall * ncomp);
+    double *gctr = calloc(nfall * nctr_all * ncomp, sizeof(double));
+    double *work = malloc(sizeof(double) * 2 * nfall);
+    double *blk = malloc(sizeof(double) * nfall);
+
+    int empty = set_pairdata(pdij, &s[0], &s[1], gd.li, gd.lj, expcutoff);
+    int not0 = 0;
+    if (!empty) {
+        int ix[4][NCART_MAX], iy[4][NCART_MAX], iz[4][NCART_MAX];
+        for (int a = 0; a < 4; a++) cart_comp(lv[a], ix[a], iy[a], iz[a]);
+        double rr_kl = 0;
+        for (int d = 0; d < 3; d++) rr_kl += (s[2].r[d] - s[3].r[d]) * (s[2].r[d] - s[3].r[d]);
+        double lck[MAXPRIM], lcl[MAXPRIM];
+        log_max_coeff(lck, &s[2]);
+        log_max_coeff(lcl, &s[3]);
+        double log_rr_kl = 0;
+        if (!fam->three_center) {
+            double akl_last = s[2].a[s[2].nprim - 1] + s[3].a[s[3].nprim - 1];
+            log_rr_kl = 1.7 - 1.5 * log(akl_last);
+            int lkl = gd.lk + gd.ll;
+            if (lkl > 0) log_rr_kl += lkl * log(sqrt(rr_kl) + 1.);
+        }
+        for (int lp = 0; lp < s[3].nprim; lp++)
+            for (int kp = 0; kp < s[2].nprim; kp++) {
+                double ak = s[2].a[kp], al = s[3].a[lp];
+                double akl = ak + al;
+                double Q[3], ekl_exp, eijcutoff;
+                if (fam->three_center) {
+                    for (int d = 0; d < 3; d++) Q[d] = s[2].r[d];
+                    ekl_exp = 1.0;
+                    eijcutoff = expcutoff;
+                } else {
+                    double ekl = rr_kl * ak * al / akl;
+                    double ccekl = ekl - log_rr_kl - lck[kp] - lcl[lp];
+                    if (ccekl > expcutoff) continue;
+                    for (int d = 0; d < 3; d++) Q[d] = (ak * s[2].r[d] + al * s[3].r[d]) / akl;
+                    eijcutoff = expcutoff - ccekl;
+                    ekl_exp = exp(-ekl);
+                }
+                for (int jp = 0; jp < s[1].nprim; jp++)
+                    for (int ip = 0; ip < s[0].nprim; ip++) {
+                        const PairPrim *pp = &pdij[jp * s[0].nprim + ip];
+                        if (pp->cce > eijcutoff) continue;
+                        double ai = s[0].a[ip], aj = s[1].a[jp];
+                        double fac = common * pp->e * ekl_exp;
+                        build_g(G, &gd, ai + aj, akl, pp->P, Q, fac);
+                        not0 = 1;
+                        /* gout over Cartesian components, element order
+                         * (i fastest, j, k, l) */
+                        double ai2 = -2.0 * ai;
+                        size_t n = 0;
+                        for (int fl = 0; fl < nf[3]; fl++)
+                            for (int fk = 0; fk < nf[2]; fk++)
+                                for (int fj = 0; fj < nf[1]; fj++)
+                                    for (int fi = 0; fi < nf[0]; fi++, n++) {
+                                        int px[4] = {ix[0][fi], ix[1][fj], ix[2][fk], ix[3][fl]};
+                                        int py[4] = {iy[0][fi], iy[1][fj], iy[2][fk], iy[3][fl]};
+                                        int pz[4] = {iz[0][fi], iz[1][fj], iz[2][fk], iz[3][fl]};
+                                        if (!fam->deriv_i) {
+                                            double acc = 0;
+                                            for (int r = 0; r < gd.nroots; r++)
+                                                acc += G[gidx(&gd, 0, r, px[0], px[1], px[2], px[3])] *
+                                                       G[gidx(&gd, 1, r, py[0], py[1], py[2], py[3])] *
+                                                       G[gidx(&gd, 2, r, pz[0], pz[1], pz[2], pz[3])];
+                                            gout[n] = acc;
+                                        } else {
+                                            /* nabla_i: f(i) = i g(i-1) - 2 ai g(i+1)
+                                             * (g2e.c:4574-4620) */
+                                            double sx = 0, sy = 0, sz = 0;
+                                            for (int r = 0; r < gd.nroots; r++) {
+                                                double gx = G[gidx(&gd, 0, r, px[0], px[1], px[2], px[3])];
+                                                double gy = G[gidx(&gd, 1, r, py[0], py[1], py[2], py[3])];
+                                                double gz = G[gidx(&gd, 2, r, pz[0], pz[1], pz[2], pz[3])];
+                                                double fx = ai2 * G[gidx(&gd, 0, r, px[0] + 1, px[1], px[2], px[3])];
+                                                double fy = ai2 * G[gidx(&gd, 1, r, py[0] + 1, py[1], py[2], py[3])];
+                                                double fz = ai2 * G[gidx(&gd, 2, r, pz[0] + 1, pz[1], pz[2], pz[3])];
+                                                if (px[0] > 0) fx += px[0] * G[gidx(&gd, 0, r, px[0] - 1, px[1], px[2], px[3])];
+                                                if (py[0] > 0) fy += py[0] * G[gidx(&gd, 1, r, py[0] - 1, py[1], py[2], py[3])];
+                                                if (pz[0] > 0) fz += pz[0] * G[gidx(&gd, 2, r, pz[0] - 1, pz[1], pz[2], pz[3])];
+                                                sx += fx * gy * gz;
+                                                sy += gx * fy * gz;
+                                                sz += gx * gy * fz;
+                                            }
+                                            gout[n] = sx;
+                                            gout[nfall + n] = sy;
+                                            gout[2 * nfall + n] = sz;
+                                        }
+                                    }
+                        /* contraction (PRIM2CTR): gctr[comp][lc][kc][jc][ic][f] */
+                        size_t blkc = 0;
+                        for (int lc = 0; lc < nctr[3]; lc++)
+                            for (int kc = 0; kc < nctr[2]; kc++)
+                                for (int jc = 0; jc < nctr[1]; jc++)
+                                    for (int ic = 0; ic < nctr[0]; ic++, blkc++) {
+                                        double c = s[0].c[ic * s[0].nprim + ip] *
+                                                   s[1].c[jc * s[1].nprim + jp] *
+                                                   s[2].c[kc * s[2].nprim + kp] *
+                                                   s[3].c[lc * s[3].nprim + lp];
+                                        if (c == 0.0) continue;
+                                        for (int cp = 0; cp < ncomp; cp++) {
+                                            double *dst = gctr + (cp * nctr_all + blkc) * nfall;
+                                            const double *src = gout + cp * nfall;
+                                            for (size_t f = 0; f < nfall; f++) dst[f] += c * src[f];
+                                        }
+                                    }
+                    }
+            }
+    }
+    /* cart -> sph and scatter into out (cart2sph.c:5324-5376) */
+    int ns[4];
+    for (int a = 0; a < 4; a++) ns[a] = fam->spherical ? nsph(lv[a]) : ncart(lv[a]);
+    for (int cp = 0; cp < ncomp; cp++) {
+        double *o = out + cp * nout;
+        if (!not0) {
+            for (int l = 0; l < cnt[3]; l++)
+                for (int k = 0; k < cnt[2]; k++)
+                    for (int j = 0; j < cnt[1]; j++)
+                        for (int i = 0; i < cnt[0]; i++)
+                            o[i + (size_t)dd[0] * (j + (size_t)dd[1] * (k + (size_t)dd[2] * l))] = 0;
+            continue;
+        }
+        size_t blkc = 0;
+        for (int lc = 0; lc < nctr[3]; lc++)
+            for (int kc = 0; kc < nctr[2]; kc++)
+                for (int jc = 0; jc < nctr[1]; jc++)
+                    for (int ic = 0; ic < nctr[0]; ic++, blkc++) {
+                        transform4(blk, gctr + (cp * nctr_all + blkc) * nfall, lv, fam->spherical, work);
+                        size_t q = 0;
+                        for (int ml = 0; ml < ns[3]; ml++)
+                            for (int mk = 0; mk < ns[2]; mk++)
+                                for (int mj = 0; mj < ns[1]; mj++)
+                                    for (int mi = 0; mi < ns[0]; mi++, q++) {
+                                        size_t I = ic * ns[0] + mi, J = jc * ns[1] + mj;
+                                        size_t K = kc * ns[2] + mk, L = lc * ns[3] + ml;
+                                        o[I + dd[0] * (J + dd[1] * (K + dd[2] * L))] = blk[q];
+                                    }
+                    }
+    }
+    free(pdij);
+    free(G);
+    free(gout);
+    free(gctr);
+    free(work);
+    free(blk);
+    return not0;
+}
+
+int orc_int2e_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                  double *env, void *opt, double *cache)
+{
+    (void)natm; (void)nbas; (void)opt; (void)cache;
+    if (!out) return 0;
+    Family f = {0, 0, 1};
+    return tuple_driver(&f, out, dims, shls, atm, bas, env);
+}
+int orc_int2e_cart(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                   double *env, void *opt, double *cache)
+{
+    (void)natm; (void)nbas; (void)opt; (void)cache;
+    if (!out) return 0;
+    Family f = {0, 0, 0};
+    return tuple_driver(&f, out, dims, shls, atm, bas, env);
+}
+int orc_int2e_ip1_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                      double *env, void *opt, double *cache)
+{
+    (void)natm; (void)nbas; (void)opt; (void)cache;
+    if (!out) return 0;
+    Family f = {0, 1, 1};
+    return tuple_driver(&f, out, dims, shls, atm, bas, env);
+}
+int orc_int3c2e_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                    double *env, void *opt, double *cache)
+{
+    (void)natm; (void)nbas; (void)opt; (void)cache;
+    if (!out) return 0;
+    Family f = {1, 0, 1};
+    return tuple_driver(&f, out, dims, shls, atm, bas, env);
+}
+int orc_int3c2e_ip1_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas,
+                        int nbas, double *env, void *opt, double *cache)
+{
+    (void)natm; (void)nbas; (void)opt; (void)cache;
+    if (!out) return 0;
+    Family f = {1, 1, 1};
+    return tuple_driver(&f, out, dims, shls, atm, bas, env);
+}
+
+typedef int (*orc_fn)(double *, int *, int *, int *, int, int *, int, double *, void *, double *);
+
+long orc_eval_list(int api, int nq, const int *shls_list, const long *offsets, double *out,
+                   int *atm, int natm, int *bas, int nbas, double *env, int nthreads)
+{
+    orc_fn fns[5] = {orc_int2e_sph, orc_int2e_ip1_sph, orc_int3c2e_sph, orc_int3c2e_ip1_sph,
+                     orc_int2e_cart};
+    if (api < 0 || api > 4) return -1;
+    orc_fn fn = fns[api];
+    ensure_c2s();
+    long nnz = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads) reduction(+ : nnz)
+#endif
+    for (int q = 0; q < nq; q++) {
+        int sh[4];
+        for (int a = 0; a < 4; a++) sh[a] = shls_list[4 * q + a];
+        nnz += fn(out + offsets[q], NULL, sh, atm, natm, bas, nbas, env, NULL, NULL);
+    }
+    (void)nthreads;
+    return nnz;
+}
+
+/* ------------------------------------------------------------------ */
+/* Algorithmic work model: trip counts of the loops above (which follow
+ * libcint's), per shell tuple.                                          */
+/* ------------------------------------------------------------------ */
+void orc_work_model(int api, int *shls, int *atm, int natm, int *bas, int nbas, double *env,
+                    double *counts)
+{
+    (void)natm; (void)nbas;
+    Family fam = {api == 2 || api == 3, api == 1 || api == 3, api != 4};
+    Shl s[4];
+    int nsh = fam.three_center ? 3 : 4;
+    for (int a = 0; a < nsh; a++) get_shl(&s[a], shls[a], atm, bas, env);
+    static const double zero3[3] = {0, 0, 0};
+    static const double one = 1.0;
+    if (fam.three_center) {
+        s[3].l = 0; s[3].nprim = 1; s[3].nctr = 1; s[3].r = s[2].r; s[3].a = zero3; s[3].c = &one;
+    }
+    int lv[4] = {s[0].l, s[1].l, s[2].l, s[3].l};
+    int Lc[4] = {lv[0] + fam.deriv_i, lv[1], lv[2], lv[3]};
+    int nr = (Lc[0] + Lc[1] + Lc[2] + Lc[3]) / 2 + 1;
+    double expcutoff;
+    if (env[ORC_PTR_EXPCUTOFF] == 0)
+        expcutoff = EXPCUTOFF;
+    else {
+        expcutoff = env[ORC_PTR_EXPCUTOFF] > MIN_EXPCUTOFF ? env[ORC_PTR_EXPCUTOFF] : MIN_EXPCUTOFF;
+        if (!fam.three_center) expcutoff += 1;
+    }
+    PairPrim *pd = malloc(sizeof(PairPrim) * s[0].nprim * s[1].nprim);
+    long nprimq = 0;
+    if (!set_pairdata(pd, &s[0], &s[1], Lc[0], Lc[1], expcutoff)) {
+        double rr_kl = 0;
+        for (int d = 0; d < 3; d++) rr_kl += (s[2].r[d] - s[3].r[d]) * (s[2].r[d] - s[3].r[d]);
+        double lck[MAXPRIM], lcl[MAXPRIM];
+        log_max_coeff(lck, &s[2]);
+        log_max_coeff(lcl, &s[3]);
+        double log_rr_kl = 0;
+        if (!fam.three_center) {
+            log_rr_kl = 1.7 - 1.5 * log(s[2].a[s[2].nprim - 1] + s[3].a[s[3].nprim - 1]);
+            if (Lc[2] + Lc[3] > 0) log_rr_kl += (Lc[2] + Lc[3]) * log(sqrt(rr_kl) + 1.);
+        }
+        for (int lp = 0; lp < s[3].nprim; lp++)
+            for (int kp = 0; kp < s[2].nprim; kp++) {
+                double eijcutoff = expcutoff;
+                if (!fam.three_center) {
+                    double ak = s[2].a[kp], al = s[3].a[lp];
+                    double ccekl = rr_kl * ak * al / (ak + al) - log_rr_kl - lck[kp] - lcl[lp];
+                    if (ccekl > expcutoff) continue;
+                    eijcutoff = expcutoff - ccekl;
+                }
+                for (int n = 0; n < s[0].nprim * s[1].nprim; n++)
+                    if (pd[n].cce <= eijcutoff) nprimq++;
+            }
+    }
+    free(pd);
+    int nmax = Lc[0] + Lc[1], mmax = Lc[2] + Lc[3];
+    double nf = (double)ncart(lv[0]) * ncart(lv[1]) * ncart(lv[2]) * ncart(lv[3]);
+    double per = 24.0 * nr;                                      /* g0_2e coefficients */
+    per += 3.0 * nr * (4.0 * (nmax + mmax) + 7.0 * nmax * mmax); /* 2D VRR */
+    {
+        int A = Lc[0] > Lc[1] ? Lc[0] : Lc[1], B = Lc[0] > Lc[1] ? Lc[1] : Lc[0];
+        int C = Lc[2] > Lc[3] ? Lc[2] : Lc[3], D = Lc[2] > Lc[3] ? Lc[3] : Lc[2];
+        double hb = 0, hk = 0;
+        for (int b = 1; b <= B; b++) hb += (A + B - b + 1) * (double)(mmax + 1);
+        for (int d = 1; d <= D; d++) hk += (C + D - d + 1) * (double)((Lc[0] + 1) * (Lc[1] + 1));
+        per += 2.0 * 3.0 * nr * (hb + hk); /* 4D HRR */
+    }
+    int ncomp = fam.deriv_i ? 3 : 1;
+    if (fam.deriv_i) {
+        per += 9.0 * nf * nr;                                           /* gout_ip1 */
+        per += 3.0 * 3.0 * nr * (lv[0] + 1) * (Lc[1] + 1) * (Lc[2] + 1) * (Lc[3] + 1); /* nabla */
+    } else {
+        per += 3.0 * nf * nr; /* gout */
+    }
+    per += 2.0 * nf * ncomp; /* contraction */
+    double nctr = (double)s[0].nctr * s[1].nctr * s[2].nctr * s[3].nctr;
+    double c2s = 0;
+    if (fam.spherical) {
+        double dim[4] = {ncart(lv[0]), ncart(lv[1]), ncart(lv[2]), ncart(lv[3])};
+        for (int a = 0; a < 4; a++) {
+            double other = 1;
+            for (int b = 0; b < 4; b++)
+                if (b != a) other *= dim[b];
+            c2s += 2.0 * other * nsph(lv[a]) * ncart(lv[a]);
+            dim[a] = nsph(lv[a]);
+        }
+    }
+    counts[0] = (double)nprimq;
+    counts[1] = (double)nprimq * nr;
+    counts[2] = nprimq * per + (nprimq > 0 ? c2s * nctr * ncomp : 0.0);
+}

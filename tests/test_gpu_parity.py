@@ -1,0 +1,168 @@
+"""GPU parity: libcintx_hip.so (HIP kernels on the MI355X) against the oracle
+on the same seeded inputs.  FP64 bar: max abs error <= 1e-12 (the reference's
+UNIFIED_ATOL, crates/cintx-oracle/src/compare.rs:21-22)."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from cintx_amd import capi
+from cintx_amd import molecule as M
+
+pytestmark = pytest.mark.gpu
+ATOL = 1e-12
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_blocks(O, name, tuples, mol):
+    out, offs, sizes = O.eval_list(name, tuples, mol.atm, mol.bas, mol.env)
+    return out, offs, sizes
+
+
+def check_plan(O, name, tuples, mol, atol=ATOL):
+    say(f"plan {name} ntuples={len(tuples)}")
+    p = capi.Plan(name, mol.atm, mol.bas, mol.env, tuples)
+    p.execute()
+    p.sync()
+    got = p.output()
+    ref, offs, sizes = oracle_blocks(O, name, tuples, mol)
+    t, o = p.tuples()
+    assert np.array_equal(o, offs)
+    err = np.abs(got - ref)
+    say(f"  max abs err {err.max():.3e}  max |ref| {np.abs(ref).max():.3e}")
+    assert err.max() <= atol
+    p.close()
+    return got
+
+
+def test_device_rys_rule_matches_oracle(oracle):
+    L = capi.lib()
+    L.cintx_rys_eval_device.argtypes = [ctypes.c_int, capi.DP, ctypes.c_int64, capi.DP, capi.DP]
+    for n in range(1, 15):
+        xs = np.concatenate([np.linspace(0.0, 40.0 + 5 * n, 301), [1e-12, 3e-7, 1e4]])
+        u = np.zeros(len(xs) * n)
+        w = np.zeros(len(xs) * n)
+        assert L.cintx_rys_eval_device(n, capi._dp(xs), len(xs), capi._dp(u), capi._dp(w)) == 0
+        u, w = u.reshape(-1, n), w.reshape(-1, n)
+        for k, x in enumerate(xs):
+            uo, wo = oracle.rys_roots(n, x)
+            tol = 1e-13 if x >= 35.0 + 5 * n else 2e-14
+            assert np.max(np.abs(u[k] - uo) / uo) < tol
+            assert np.max(np.abs(w[k] - wo) / wo) < tol
+
+
+def test_int2e_sph_cfg1_full_tensor(oracle):
+    mol = M.config(1)
+    n = mol.nbas
+    tuples = [(i, j, k, l) for l in range(n) for k in range(n) for j in range(n) for i in range(n)]
+    check_plan(oracle, "int2e_sph", tuples, mol)
+
+
+def test_int2e_sph_cfg2_sampled(oracle):
+    mol = M.config(2)
+    rng = np.random.default_rng(2)
+    tuples = [tuple(int(s) for s in rng.integers(0, mol.nbas, 4)) for _ in range(3000)]
+    check_plan(oracle, "int2e_sph", tuples, mol)
+
+
+def test_int2e_ip1_sph_cfg1(oracle):
+    mol = M.config(1)
+    rng = np.random.default_rng(3)
+    tuples = [tuple(int(s) for s in rng.integers(0, mol.nbas, 4)) for _ in range(2000)]
+    check_plan(oracle, "int2e_ip1_sph", tuples, mol)
+
+
+def test_int3c2e_sph_cfg4_sampled(oracle):
+    mol = M.config(4)
+    rng = np.random.default_rng(4)
+    tuples = [(int(rng.integers(0, mol.aux_start)), int(rng.integers(0, mol.aux_start)),
+               int(rng.integers(mol.aux_start, mol.nbas))) for _ in range(3000)]
+    check_plan(oracle, "int3c2e_sph", tuples, mol)
+
+
+def test_int3c2e_ip1_sph_cfg4_sampled(oracle):
+    mol = M.config(4)
+    rng = np.random.default_rng(5)
+    tuples = [(int(rng.integers(0, mol.aux_start)), int(rng.integers(0, mol.aux_start)),
+               int(rng.integers(mol.aux_start, mol.nbas))) for _ in range(1500)]
+    check_plan(oracle, "int3c2e_ip1_sph", tuples, mol)
+
+
+def test_raw_signature_dims_and_general_contraction(oracle):
+    mol = M.general_contraction_h2()
+    low = [s for s in range(mol.nbas) if mol.bas[s][1] <= 2]
+    rng = np.random.default_rng(6)
+    for _ in range(25):
+        shls = [int(low[i]) for i in rng.integers(0, len(low), 4)]
+        got, not0 = capi.int2e_sph(shls, mol.atm, mol.bas, mol.env)
+        ref, rnot0 = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env)
+        assert not0 == rnot0
+        assert np.abs(got - ref).max() <= 1e-11
+        d0 = capi.natural_dims("int2e_sph", shls, mol.bas)
+        dims = [d + 1 for d in d0]
+        gd, _ = capi.int2e_sph(shls, mol.atm, mol.bas, mol.env, dims=dims)
+        rd, _ = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env, dims=dims)
+        assert np.abs(gd - rd).max() <= 1e-11
+    shls = [int(low[0]), int(low[1]), int(low[2])]
+    for name in ("int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph"):
+        sh = shls + [int(low[3])] if name.startswith("int2e") else shls
+        got, not0 = capi.raw_call(name, sh, mol.atm, mol.bas, mol.env)
+        ref, rnot0 = oracle.eval_tuple(name, sh, mol.atm, mol.bas, mol.env)
+        assert not0 == rnot0
+        assert np.abs(got - ref).max() <= 1e-11
+
+
+def test_cintrs_eval_on_device(oracle):
+    mol = M.config(1)
+    shls = [5, 3, 4, 8]
+    req = int(np.prod(capi.natural_dims("int2e_sph", shls, mol.bas)))
+    out = np.zeros(req)
+    s = capi.eval_raw(capi.INT2E_SPH, out, None, shls, mol.atm, mol.bas, mol.env)
+    ref, not0 = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env)
+    assert s["not0"] == not0 and s["bytes_written"] == req * 8
+    assert np.abs(out - ref).max() <= ATOL
+
+
+def test_graph_replay_equals_eager(oracle):
+    mol = M.config(1)
+    n = mol.nbas
+    tuples = [(i, j, k, l) for i in range(n) for j in range(i + 1) for k in range(n)
+              for l in range(k + 1)]
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, tuples)
+    p.execute()
+    p.sync()
+    eager = p.output()
+    for rep in range(4):
+        p.execute(graph=True)
+        p.sync()
+        again = p.output()
+        assert np.array_equal(eager, again), rep
+        p.execute()
+        p.sync()
+    p.close()
+
+
+def test_screened_plan_cfg2(oracle):
+    mol = M.config(2)
+    say("screened plan cfg2")
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, screened=(1e-12, 97))
+    p.execute()
+    p.sync()
+    got = p.output()
+    t, offs = p.tuples()
+    say(f"  tuples {p.ntuples}")
+    assert p.ntuples > 1000
+    # unique-quartet invariant: i>=j, k>=l and bra pair id >= ket pair id
+    rng = np.random.default_rng(7)
+    pick = rng.choice(p.ntuples, size=min(2000, p.ntuples), replace=False)
+    sub = [tuple(int(v) for v in t[q]) for q in pick]
+    ref, roffs, sizes = oracle.eval_list("int2e_sph", sub, mol.atm, mol.bas, mol.env)
+    for k, q in enumerate(pick):
+        blk = got[offs[q]:offs[q] + sizes[k]]
+        assert np.abs(blk - ref[roffs[k]:roffs[k] + sizes[k]]).max() <= ATOL
+    p.close()

@@ -1,0 +1,111 @@
+"""CPU checks of libcintx_hip.so: it loads, exports every declared symbol,
+answers libcint's cache-size query, fails closed with the cintx status codes
+before touching a device, and its Rys table reproduces the oracle's rule."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cintx_amd import capi
+from cintx_amd import molecule as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "cintx_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:int|void|size_t)\s+\**\s*(\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = capi.lib()
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), n
+    for n in capi.EXPORTS:
+        assert n in names
+
+
+def test_cache_size_query_matches_libcint_formula():
+    mol = M.Molecule()
+    a = mol.add_atom(1, (0, 0, 0))
+    mol.add_shell(a, 0, [1.0], [[1.0]])
+    mol.add_shell(a, 1, [1.0, 0.5], [[1.0, 0.3], [0.2, 1.0]])
+    mol.build()
+    # (ss|ss), 1 primitive: g_size 1 -> leng 6, len0 1, nc 1, pdata 25 -> 35
+    assert capi.cache_size("int2e_sph", [0, 0, 0, 0], mol.atm, mol.bas, mol.env) == 35
+    # (ps|ss) ip1: li_ceil 2, nroots 2, g_size 2*3*1*1*1=6, leng 6*3*3=54,
+    # nf 3, ncomp 3 -> len0 9, nc 6 (2 contractions) -> 6*3*3=54,
+    # pdata (2+1)*5 + 4+1+1+1 + (2+1+1+1)*2 + 9 = 15+7+10+9 = 41 -> 158
+    assert capi.cache_size("int2e_ip1_sph", [1, 0, 0, 0], mol.atm, mol.bas, mol.env) == 158
+    # int3c2e (ss|s): g_size 1, leng 6, len0 1, nc 1 -> 3,
+    # pdata 5 + 1+1+1 + 4 + 1 + 3 + 16 = 32 -> 42
+    assert capi.cache_size("int3c2e_sph", [0, 0, 0], mol.atm, mol.bas, mol.env) == 42
+
+
+def _eval(api, out, dims, shls, mol):
+    return capi.eval_raw(api, out, dims, shls, mol.atm, mol.bas, mol.env)
+
+
+def test_shim_fails_closed_without_touching_the_device():
+    mol = M.config(1)
+    with pytest.raises(capi.CintxError) as e:
+        _eval(99, np.zeros(10), None, [0, 0, 0, 0], mol)
+    assert e.value.status == 2
+    shls = [5, 5, 5, 5]  # d shell: 5**4 elements
+    with pytest.raises(capi.CintxError) as e:
+        _eval(capi.INT2E_SPH, np.zeros(10), None, shls, mol)
+    assert e.value.status == 4  # BufferTooSmall
+    with pytest.raises(capi.CintxError) as e:
+        _eval(capi.INT2E_SPH, np.zeros(625), [5, 5, 5, 4], shls, mol)
+    assert e.value.status == 1  # dims differ from natural extents
+    with pytest.raises(capi.CintxError) as e:
+        _eval(capi.INT2E_SPH, np.zeros(625), None, [0, 0, 0, 99], mol)
+    assert e.value.status == 1
+    with pytest.raises(capi.CintxError) as e:
+        _eval(capi.INT2E_SPH, np.zeros(625), None, [0, 0, 0], mol)
+    assert e.value.status == 1
+    bad = mol.bas.copy()
+    bad[0, 1] = 5
+    with pytest.raises(capi.CintxError) as e:
+        capi.eval_raw(capi.INT2E_SPH, np.zeros(11 ** 4), None, [0, 0, 0, 0], mol.atm, bad, mol.env)
+    assert e.value.status == 2  # l above the supported maximum
+    # workspace query (out = NULL) needs no device
+    s = _eval(capi.INT2E_SPH, None, None, shls, mol)
+    assert s["bytes_written"] == 0 and s["workspace_bytes"] > 0
+    q = capi.query_workspace_raw(capi.INT2E_IP1_SPH, None, shls, mol.atm, mol.bas, mol.env)
+    assert q["required_bytes"] == 3 * 625 * 8
+
+
+def test_rys_table_matches_oracle(oracle):
+    L = capi.lib()
+    L.cintx_rys_eval_host.argtypes = [ctypes.c_int, capi.DP, ctypes.c_int64, capi.DP, capi.DP]
+    for n in range(1, 15):
+        xs = np.concatenate([np.linspace(0.0, 40.0 + 5 * n, 211),
+                             [1e-10, 3e-7, 35.0 + 5 * n - 1e-9, 35.0 + 5 * n, 500.0, 1e5]])
+        u = np.zeros(len(xs) * n)
+        w = np.zeros(len(xs) * n)
+        assert L.cintx_rys_eval_host(n, capi._dp(xs), len(xs), capi._dp(u), capi._dp(w)) == 0
+        u = u.reshape(-1, n)
+        w = w.reshape(-1, n)
+        for k, x in enumerate(xs):
+            uo, wo = oracle.rys_roots(n, x)
+            # table: ~4e-15; the asymptotic rule above 35+5n (libcint's LARGEX
+            # regime, rys_roots.c:67-77) is within 1e-13
+            tol = 1e-13 if x >= 35.0 + 5 * n else 2e-14
+            assert np.max(np.abs(u[k] - uo) / uo) < tol, (n, x)
+            assert np.max(np.abs(w[k] - wo) / wo) < tol, (n, x)
+
+
+def test_generated_molecules_have_the_quoted_sizes():
+    sizes = {1: (3, 24), 2: (64, 509), 3: (256, 2819), 5: (32, 1088)}
+    for cfg, (natm, nao) in sizes.items():
+        m = M.config(cfg)
+        assert len(m.coords) == natm and m.nao() == nao
+    m4 = M.config(4)
+    assert m4.nao(range(m4.aux_start, m4.nbas)) == 4544
