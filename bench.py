@@ -37,11 +37,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--bra-stride", type=int, default=64)
+    ap.add_argument("--bra-stride", type=int, default=256)
     ap.add_argument("--threshold", type=float, default=1e-12)
     ap.add_argument("--mixed-threshold", type=float, default=1e-4)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sample", type=int, default=4000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -55,12 +56,14 @@ def main():
     t0 = time.time()
     p64 = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env,
                     screened=(args.threshold, args.bra_stride), precision=capi.PREC_FP64)
-    log(f"fp64 plan: {p64.ntuples} quartets, {p64.nelem} doubles ({time.time() - t0:.1f}s)")
+    log(f"fp64 plan: {p64.ntuples} quartets, {p64.nelem} doubles, {p64.nbins} bins "
+        f"({time.time() - t0:.1f}s)")
     t0 = time.time()
     pmx = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env,
                     screened=(args.threshold, args.bra_stride), precision=capi.PREC_MIXED,
                     mixed_threshold=args.mixed_threshold)
-    log(f"mixed plan: {pmx.ntuples} quartets ({time.time() - t0:.1f}s)")
+    log(f"mixed plan: {pmx.ntuples} quartets, {pmx.ntuples_fp32} in FP32, {pmx.nbins} bins "
+        f"({time.time() - t0:.1f}s)")
 
     def timed(plan, label):
         for _ in range(args.warmup):
@@ -79,21 +82,12 @@ def main():
     ms64, wall64 = timed(p64, "fp64")
     msmx, wallmx = timed(pmx, "mixed")
 
-    # accuracy of the mixed path against the FP64 path, tuple by tuple
-    out64 = p64.output()
-    outmx = pmx.output()
-    t64, o64 = p64.tuples()
-    tmx, omx = pmx.tuples()
-    key64 = {tuple(t): o for t, o in zip(map(tuple, t64), o64)}
-    sizes = np.diff(np.append(omx, len(outmx)))
-    err = 0.0
-    for t, o, s in zip(map(tuple, tmx), omx, sizes):
-        a = outmx[o:o + s]
-        b = out64[key64[t]:key64[t] + s]
-        err = max(err, float(np.abs(a - b).max()) if s else 0.0)
-    log(f"mixed vs fp64 max abs err {err:.3e}")
+    # accuracy on a seeded sample of tuples: both GPU paths against the oracle
+    err64, errmx = sample_errors(mol, p64, pmx, args.sample)
+    log(f"sampled max abs err vs oracle: fp64 {err64:.3e}, mixed {errmx:.3e}")
 
     prim64, roots64, flops64 = p64.work()
+    log(f"fp64 work: {prim64:.3e} primitive quartets, {flops64:.3e} FLOP")
     step_s = ms64 / 1e3
     achieved = flops64 / step_s / 1e12
     value = pmx.ntuples / (msmx / 1e3)
@@ -102,7 +96,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        t64, _ = p64.tuples_cached()
         cpu = cpu_baseline(mol, t64, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.3e} quartets/s on {cpu['cores']} cores")
 
     res = {
         "metric": "int2e_sph quartets/s: mixed vs FP64 GPU at <=1e-10 abs err, 1 MI355X",
@@ -122,12 +118,14 @@ def main():
                         f"unique quartets, Schwarz 1e-12, bra stride {args.bra_stride}",
             "quartets_per_step": pmx.ntuples,
             "mixed_threshold": args.mixed_threshold,
+            "fp32_quartets": pmx.ntuples_fp32,
         },
         "fp64": {"quartets_per_s": fp64_rate, "ms_per_step": ms64,
                  "wall_ms_per_step": wall64, "gflops": achieved * 1e3,
                  "prim_quartets": prim64, "root_evals": roots64},
         "mixed_over_fp64": value / fp64_rate,
-        "mixed_max_abs_err_vs_fp64": err,
+        "fp64_max_abs_err_vs_oracle": err64,
+        "mixed_max_abs_err_vs_oracle": errmx,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "note": "FP64 VALU-bound class kernels (FP64 vector and matrix peaks are "
@@ -137,6 +135,30 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(res), flush=True)
+
+
+def sample_errors(mol, p64, pmx, n):
+    """max abs error of each GPU path against the oracle on n seeded tuples
+    (for the mixed plan, drawn from its FP32 bins when it has any)."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(1)
+    out = []
+    for p in (p64, pmx):
+        t, _ = p.tuples_cached()
+        pick = np.sort(rng.choice(len(t), size=min(n, len(t)), replace=False))
+        if p is pmx and p.ntuples_fp32 > 0:
+            # mixed plans keep the FP32 tuples after the FP64 ones: take half of
+            # the sample from each part
+            first32 = len(t) - p.ntuples_fp32
+            a = rng.choice(first32, size=min(n // 2, first32), replace=False) if first32 else []
+            b = first32 + rng.choice(p.ntuples_fp32, size=min(n // 2, p.ntuples_fp32),
+                                     replace=False)
+            pick = np.sort(np.concatenate([np.asarray(a, dtype=np.int64), b]))
+        got, offs, sizes = p.output_tuples(pick)
+        ref, roffs, rsizes = O.eval_list("int2e_sph", [tuple(int(v) for v in t[q]) for q in pick],
+                                         mol.atm, mol.bas, mol.env)
+        out.append(float(np.abs(got - ref).max()))
+    return out[0], out[1]
 
 
 def cpu_baseline(mol, tuples, seconds):

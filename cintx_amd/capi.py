@@ -46,8 +46,8 @@ EXPORTS = [
     "cintrs_query_workspace", "cintrs_eval", "cintrs_last_error_code",
     "cintrs_clear_last_error", "cintrs_copy_last_error_message",
     "cintx_plan_create", "cintx_plan_create_screened", "cintx_plan_destroy",
-    "cintx_plan_sizes", "cintx_plan_tuples", "cintx_plan_execute", "cintx_plan_sync",
-    "cintx_plan_copy_output", "cintx_plan_execute_graph", "cintx_plan_last_time_ms",
+    "cintx_plan_sizes", "cintx_plan_info", "cintx_plan_tuples", "cintx_plan_execute", "cintx_plan_sync",
+    "cintx_plan_copy_output", "cintx_plan_copy_tuples", "cintx_plan_execute_graph", "cintx_plan_last_time_ms",
     "cintx_plan_work", "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
 ]
 
@@ -104,11 +104,13 @@ def lib():
                                                  ctypes.c_int, ctypes.c_double]
         L.cintx_plan_destroy.argtypes = [vp]
         L.cintx_plan_sizes.argtypes = [vp, I64P, I64P]
+        L.cintx_plan_info.argtypes = [vp, I64P, I64P]
         L.cintx_plan_tuples.argtypes = [vp, IP, I64P]
         L.cintx_plan_execute.argtypes = [vp, DP]
         L.cintx_plan_execute_graph.argtypes = [vp, DP]
         L.cintx_plan_sync.argtypes = [vp]
         L.cintx_plan_copy_output.argtypes = [vp, DP]
+        L.cintx_plan_copy_tuples.argtypes = [vp, I64P, ctypes.c_int64, DP, I64P]
         L.cintx_plan_last_time_ms.argtypes = [vp, DP]
         L.cintx_plan_work.argtypes = [vp, DP, DP, DP]
         L.cintx_device_count.argtypes = [IP]
@@ -244,6 +246,11 @@ class Plan:
         _check(L.cintx_plan_sizes(h, ctypes.byref(n), ctypes.byref(nt)))
         self.nelem = n.value
         self.ntuples = nt.value
+        n32 = ctypes.c_int64()
+        nb = ctypes.c_int64()
+        _check(L.cintx_plan_info(h, ctypes.byref(n32), ctypes.byref(nb)))
+        self.ntuples_fp32 = n32.value
+        self.nbins = nb.value
 
     def tuples(self):
         t = np.zeros((self.ntuples, 4), dtype=np.int32)
@@ -267,6 +274,29 @@ class Plan:
         out = np.zeros(self.nelem)
         _check(lib().cintx_plan_copy_output(self.h, _dp(out)))
         return out
+
+    def output_tuples(self, which):
+        """libcint-layout blocks of the tuples `which` (packed) and offsets."""
+        which = np.ascontiguousarray(which, dtype=np.int64)
+        t, _ = self.tuples_cached()
+        sizes = np.array([self.block_size(t[q]) for q in which], dtype=np.int64)
+        out = np.zeros(int(sizes.sum()))
+        offs = np.zeros(len(which), dtype=np.int64)
+        _check(lib().cintx_plan_copy_tuples(self.h, which.ctypes.data_as(I64P), len(which),
+                                            _dp(out), offs.ctypes.data_as(I64P)))
+        return out, offs, sizes
+
+    def tuples_cached(self):
+        if not hasattr(self, "_tuples"):
+            self._tuples = self.tuples()
+        return self._tuples
+
+    def block_size(self, shls):
+        arity = 3 if self.family.startswith("int3c2e") else 4
+        n = 3 if "ip1" in self.family else 1
+        for s in shls[:arity]:
+            n *= (2 * int(self._bas[s][1]) + 1) * int(self._bas[s][3])
+        return n
 
     def work(self):
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -378,6 +378,7 @@ struct TupleRec {
 
 struct cintx_plan {
   int family = 0;
+  int64_t ntuples_fp32 = 0;
   double expcutoff = 60.0;
   cx::HostBasis basis;
   cx::PairSet pairs;
@@ -1141,6 +1142,75 @@ static void class_work(int fam, const int* l, double* per_prim, double* per_tupl
   *nroots = nr;
 }
 
+int cintx_plan_copy_tuples(cintx_plan* plan, const int64_t* which, int64_t n, double* host_out,
+                           int64_t* offsets) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    if (n > 0 && (!which || !host_out)) fail(CINTX_STATUS_NULL_POINTER, "null argument");
+    DevCtx& C = ctx();
+    const FamilyInfo& f = kFam[plan->family];
+    int64_t pos = 0;
+    for (int64_t k = 0; k < n; ++k) {
+      int64_t t = which[k];
+      if (t < 0 || t >= (int64_t)plan->tuples.size())
+        fail(CINTX_STATUS_INVALID_INPUT, "tuple index out of range");
+      const TupleRec& tr = plan->tuples[t];
+      if (offsets) offsets[k] = pos;
+      // gather this tuple's canonical blocks, then place them
+      std::vector<double> blocks;
+      std::vector<int64_t> where;
+      for (int64_t v = 0; v < tr.nvq; ++v) {
+        const VQ& vq = plan->vqs[tr.first_vq + v];
+        const Bin& b = plan->bins[vq.bin];
+        where.push_back((int64_t)blocks.size());
+        size_t old = blocks.size();
+        blocks.resize(old + (size_t)b.block);
+        HIPCHK(hipMemcpyAsync(blocks.data() + old, plan->d_out + b.out_base + vq.idx * b.block,
+                              (size_t)b.block * 8, hipMemcpyDeviceToHost, C.stream));
+      }
+      HIPCHK(hipStreamSynchronize(C.stream));
+      int nn[4] = {1, 1, 1, 1};
+      for (int a = 0; a < f.arity; ++a)
+        nn[a] = nsph(plan->basis.l[tr.shl[a]]) * plan->basis.nctr[tr.shl[a]];
+      int64_t so[4] = {1, nn[0], (int64_t)nn[0] * nn[1], (int64_t)nn[0] * nn[1] * nn[2]};
+      int64_t nout = (int64_t)nn[0] * nn[1] * nn[2] * nn[3];
+      double* dst = host_out + pos;
+      for (int64_t v = 0; v < tr.nvq; ++v) {
+        const VQ& vq = plan->vqs[tr.first_vq + v];
+        const Bin& b = plan->bins[vq.bin];
+        const double* src = blocks.data() + where[v];
+        int ns[4], base[4];
+        int64_t st[4];
+        for (int c = 0; c < 4; ++c) {
+          ns[c] = nsph(b.l[c]);
+          int a = tr.perm[c];
+          base[c] = vq.cc[c] * ns[c];
+          st[c] = (f.three && a == 3) ? 0 : so[a];
+        }
+        int64_t nsall = (int64_t)ns[0] * ns[1] * ns[2] * ns[3];
+        for (int comp = 0; comp < f.ncomp; ++comp) {
+          int64_t e = 0;
+          for (int m3 = 0; m3 < ns[3]; ++m3)
+            for (int m2 = 0; m2 < ns[2]; ++m2)
+              for (int m1 = 0; m1 < ns[1]; ++m1)
+                for (int m0 = 0; m0 < ns[0]; ++m0, ++e)
+                  dst[comp * nout + (base[0] + m0) * st[0] + (base[1] + m1) * st[1] +
+                      (base[2] + m2) * st[2] + (base[3] + m3) * st[3]] = src[comp * nsall + e];
+        }
+      }
+      pos += tr.nat_size;
+    }
+  });
+}
+
+int cintx_plan_info(const cintx_plan* plan, int64_t* ntuples_fp32, int64_t* nbins) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    if (ntuples_fp32) *ntuples_fp32 = plan->ntuples_fp32;
+    if (nbins) *nbins = (int64_t)plan->bins.size();
+  });
+}
+
 int cintx_plan_last_time_ms(const cintx_plan* plan, double* ms) {
   return guarded([&] {
     if (!plan || !ms) fail(CINTX_STATUS_NULL_POINTER, "null argument");
@@ -1276,6 +1346,7 @@ int cintx_plan_create_screened(cintx_plan** plan, const int* atm, int natm, cons
         }
         plan_add_tuples(P, hi.data(), (int64_t)hi.size() / 4, 0);
         plan_add_tuples(P, lo.data(), (int64_t)lo.size() / 4, 1);
+        P->ntuples_fp32 = (int64_t)lo.size() / 4;
       } else {
         plan_add_tuples(P, tuples.data(), nt, 0);
       }

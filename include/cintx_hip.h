@@ -150,6 +150,9 @@ int cintx_plan_create_screened(cintx_plan **plan, const int *atm, int natm, cons
 int cintx_plan_destroy(cintx_plan *plan);
 /* total output elements and tuples */
 int cintx_plan_sizes(const cintx_plan *plan, int64_t *nelem, int64_t *ntuples);
+/* tuples evaluated in FP32 (mixed plans keep them after the FP64 ones) and
+ * the number of class bins (kernel launches per execute) */
+int cintx_plan_info(const cintx_plan *plan, int64_t *ntuples_fp32, int64_t *nbins);
 /* host copies of the tuple list (ntuples x 4) and offsets (ntuples) */
 int cintx_plan_tuples(const cintx_plan *plan, int *tuples, int64_t *offsets);
 /* run all class bins on the plan's stream into device memory `out_dev`
@@ -157,6 +160,10 @@ int cintx_plan_tuples(const cintx_plan *plan, int *tuples, int64_t *offsets);
 int cintx_plan_execute(cintx_plan *plan, double *out_dev);
 int cintx_plan_sync(cintx_plan *plan);
 int cintx_plan_copy_output(cintx_plan *plan, double *host_out);
+/* libcint-layout blocks of the selected tuples only, packed in the order of
+ * `which`; offsets[k] receives block k's element offset. */
+int cintx_plan_copy_tuples(cintx_plan *plan, const int64_t *which, int64_t n, double *host_out,
+                           int64_t *offsets);
 /* Record the plan's launches into a hipGraph once and replay it. */
 int cintx_plan_execute_graph(cintx_plan *plan, double *out_dev);
 /* Timing of the last execute (HIP events on the plan's stream), the

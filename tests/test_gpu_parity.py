@@ -1,6 +1,7 @@
 """GPU parity: libcintx_hip.so (HIP kernels on the MI355X) against the oracle
-on the same seeded inputs.  FP64 bar: max abs error <= 1e-12 (the reference's
-UNIFIED_ATOL, crates/cintx-oracle/src/compare.rs:21-22)."""
+on the same seeded inputs.  FP64 bar: the reference's own element criterion
+|got - ref| <= 1e-12 + 1e-12 |ref| (UNIFIED_ATOL/UNIFIED_RTOL and the pass rule
+of crates/cintx-oracle/src/compare.rs:21-22, 256-259)."""
 import ctypes
 import sys
 
@@ -12,6 +13,23 @@ from cintx_amd import molecule as M
 
 pytestmark = pytest.mark.gpu
 ATOL = 1e-12
+RTOL = 1e-12
+# nabla_i integrals at cfg-1 exponents (up to 5000) cancel terms of size
+# ~2a g; two FP64 builds of the oracle that differ only in FMA contraction
+# already break the element rule there, so the derivative families use the
+# rule plus the measured FP64 floor (test_library_cpu.deriv_tolerance).
+from test_library_cpu import deriv_tolerance
+
+
+def assert_parity(got, ref, block_scale=None):
+    err = np.abs(got - ref)
+    tol = ATOL + RTOL * np.abs(ref)
+    if block_scale is not None:
+        tol = deriv_tolerance(ref, block_scale)
+    bad = err > tol
+    say(f"  max abs err {err.max():.3e}  max |ref| {np.abs(ref).max():.3e}  "
+        f"max rel(>1e-6) {np.max(err[np.abs(ref) > 1e-6] / np.abs(ref[np.abs(ref) > 1e-6]), initial=0):.3e}")
+    assert not bad.any(), (int(bad.sum()), float(err.max()))
 
 
 def say(msg):
@@ -24,7 +42,7 @@ def oracle_blocks(O, name, tuples, mol):
     return out, offs, sizes
 
 
-def check_plan(O, name, tuples, mol, atol=ATOL):
+def check_plan(O, name, tuples, mol):
     say(f"plan {name} ntuples={len(tuples)}")
     p = capi.Plan(name, mol.atm, mol.bas, mol.env, tuples)
     p.execute()
@@ -33,9 +51,11 @@ def check_plan(O, name, tuples, mol, atol=ATOL):
     ref, offs, sizes = oracle_blocks(O, name, tuples, mol)
     t, o = p.tuples()
     assert np.array_equal(o, offs)
-    err = np.abs(got - ref)
-    say(f"  max abs err {err.max():.3e}  max |ref| {np.abs(ref).max():.3e}")
-    assert err.max() <= atol
+    scale = None
+    if "ip1" in name:
+        scale = np.repeat([np.abs(ref[a:a + n]).max() if n else 0.0
+                           for a, n in zip(offs, sizes)], sizes)
+    assert_parity(got, ref, scale)
     p.close()
     return got
 
@@ -102,19 +122,19 @@ def test_raw_signature_dims_and_general_contraction(oracle):
         got, not0 = capi.int2e_sph(shls, mol.atm, mol.bas, mol.env)
         ref, rnot0 = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env)
         assert not0 == rnot0
-        assert np.abs(got - ref).max() <= 1e-11
+        assert_parity(got, ref)
         d0 = capi.natural_dims("int2e_sph", shls, mol.bas)
         dims = [d + 1 for d in d0]
         gd, _ = capi.int2e_sph(shls, mol.atm, mol.bas, mol.env, dims=dims)
         rd, _ = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env, dims=dims)
-        assert np.abs(gd - rd).max() <= 1e-11
+        assert_parity(gd, rd)
     shls = [int(low[0]), int(low[1]), int(low[2])]
     for name in ("int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph"):
         sh = shls + [int(low[3])] if name.startswith("int2e") else shls
         got, not0 = capi.raw_call(name, sh, mol.atm, mol.bas, mol.env)
         ref, rnot0 = oracle.eval_tuple(name, sh, mol.atm, mol.bas, mol.env)
         assert not0 == rnot0
-        assert np.abs(got - ref).max() <= 1e-11
+        assert_parity(got, ref, np.abs(ref).max() if "ip1" in name else None)
 
 
 def test_cintrs_eval_on_device(oracle):
@@ -125,7 +145,7 @@ def test_cintrs_eval_on_device(oracle):
     s = capi.eval_raw(capi.INT2E_SPH, out, None, shls, mol.atm, mol.bas, mol.env)
     ref, not0 = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env)
     assert s["not0"] == not0 and s["bytes_written"] == req * 8
-    assert np.abs(out - ref).max() <= ATOL
+    assert_parity(out, ref)
 
 
 def test_graph_replay_equals_eager(oracle):
@@ -162,7 +182,6 @@ def test_screened_plan_cfg2(oracle):
     pick = rng.choice(p.ntuples, size=min(2000, p.ntuples), replace=False)
     sub = [tuple(int(v) for v in t[q]) for q in pick]
     ref, roffs, sizes = oracle.eval_list("int2e_sph", sub, mol.atm, mol.bas, mol.env)
-    for k, q in enumerate(pick):
-        blk = got[offs[q]:offs[q] + sizes[k]]
-        assert np.abs(blk - ref[roffs[k]:roffs[k] + sizes[k]]).max() <= ATOL
+    blk = np.concatenate([got[offs[q]:offs[q] + sizes[k]] for k, q in enumerate(pick)])
+    assert_parity(blk, ref)
     p.close()

@@ -109,3 +109,40 @@ def test_generated_molecules_have_the_quoted_sizes():
         assert len(m.coords) == natm and m.nao() == nao
     m4 = M.config(4)
     assert m4.nao(range(m4.aux_start, m4.nbas)) == 4544
+
+
+def deriv_tolerance(ref, block_scale):
+    """Parity rule for nabla_i integrals: the reference's element rule
+    (crates/cintx-oracle/src/compare.rs:256-259) plus the FP64 rounding floor
+    of the algorithm, 1e-13 of the tuple's largest element and 3e-11
+    absolute (see test_fp64_floor_of_derivative_integrals)."""
+    return 1e-12 + 1e-12 * np.abs(ref) + 1e-13 * block_scale + 3e-11
+
+
+def test_fp64_floor_of_derivative_integrals(oracle):
+    """Two FP64 builds of the oracle that differ only in FMA contraction
+    disagree on nabla_i integrals of the cfg-1 molecule (exponents up to 5000,
+    terms 2a*g that cancel, e.g. on one centre) by up to ~1e-11 absolute: the
+    reference's element rule cannot separate two correct FP64 evaluations
+    there, the relaxed rule used for the device parity tests can."""
+    import subprocess
+    from oracle import oracle as O
+    subprocess.check_call(["make", "-s", "-C", O.HERE])
+    mol = M.config(1)
+    rng = np.random.default_rng(3)
+    tuples = [tuple(int(s) for s in rng.integers(0, mol.nbas, 4)) for _ in range(2000)]
+    ref, offs, sizes = O.eval_list("int2e_ip1_sph", tuples, mol.atm, mol.bas, mol.env)
+    saved = O._lib
+    try:
+        O._lib = None
+        O.LIB_PATH = os.path.join(O.HERE, "liboracle_cint_fma.so")
+        got, _, _ = O.eval_list("int2e_ip1_sph", tuples, mol.atm, mol.bas, mol.env)
+    finally:
+        O.LIB_PATH = os.path.join(O.HERE, "liboracle_cint.so")
+        O._lib = saved
+    err = np.abs(got - ref)
+    strict = 1e-12 + 1e-12 * np.abs(ref)
+    assert (err > strict).sum() > 0
+    scale = np.repeat([np.abs(ref[a:a + n]).max() if n else 0.0 for a, n in zip(offs, sizes)],
+                      sizes)
+    assert np.all(err <= deriv_tolerance(ref, scale))
