@@ -28,7 +28,7 @@ __host__ __device__ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 
 // libcint Cartesian order (cint_bas.c:166-180): lx from l down, then ly from
 // l-lx down; within an lx group the position equals lz.
-__host__ __device__ inline void cart_decode(int l, int c, int& lx, int& ly, int& lz) {
+__host__ __device__ constexpr void cart_decode(int l, int c, int& lx, int& ly, int& lz) {
   int start = 0;
   for (int x = l; x >= 0; --x) {
     int cnt = l - x + 1;
@@ -84,88 +84,106 @@ struct PrimTuple {
 
 // One (axis, root) recurrence chain: 2D VRR then bra and ket HRR; writes
 // g[((axis*DI + i)*DJ + j)*DK + k)*DL + l)*NR + r].
+//
+// Register budget: the VRR is streamed over the ket power m (columns m-1, m,
+// m+1 live), each finished column goes straight through the bra transfer
+// (in place, one extra column) into g at the (i, j) block's first DK*DL
+// slots, and the ket transfer then runs block by block in place.  This keeps
+// about 4*(NMAX+1) + (MMAX+1) values live instead of the whole 2D and 3D
+// intermediates.
 template <int LIc, int LJ, int LK, int LL, typename R>
 __device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, double u,
-                                            double wz, const PrimTuple& p, double rij_ax,
-                                            double rkl_ax) {
+                                            double wz, double a0, double aij, double akl,
+                                            double a1, double dx, double prb, double qrk,
+                                            double unit, double rij_ax, double rkl_ax) {
   constexpr int NR = (LIc + LJ + LK + LL) / 2 + 1;
   constexpr int NMAX = LIc + LJ, MMAX = LK + LL;
   constexpr bool IBASE = LIc > LJ, KBASE = LK > LL;
   constexpr int DI = LIc + 1, DJ = LJ + 1, DK = LK + 1, DL = LL + 1;
   constexpr int A = IBASE ? LIc : LJ, B = IBASE ? LJ : LIc;
   constexpr int C = KBASE ? LK : LL, D = KBASE ? LL : LK;
+  constexpr int BLK = DK * DL;  // positions per (i, j) block
 
-  const double u2 = p.a0 * u;
-  const double tmp4 = 0.5 / (u2 * (p.aij + p.akl) + p.a1);
+  const double u2 = a0 * u;
+  const double tmp4 = 0.5 / (u2 * (aij + akl) + a1);
   const double tmp5 = u2 * tmp4;
   const R b00 = (R)tmp5;
-  const R b10 = (R)(tmp5 + tmp4 * p.akl);
-  const R b01 = (R)(tmp5 + tmp4 * p.aij);
-  const R c00 = (R)(p.prb[axis] - 2.0 * tmp5 * p.akl * p.dx[axis]);
-  const R c0p = (R)(p.qrk[axis] + 2.0 * tmp5 * p.aij * p.dx[axis]);
+  const R b10 = (R)(tmp5 + tmp4 * akl);
+  const R b01 = (R)(tmp5 + tmp4 * aij);
+  const R c00 = (R)(prb - 2.0 * tmp5 * akl * dx);
+  const R c0p = (R)(qrk + 2.0 * tmp5 * aij * dx);
+  const R drb = (R)rij_ax;
+  const R drk = (R)rkl_ax;
+  R* __restrict__ ga = g + (size_t)axis * DI * DJ * BLK * NR + r;
 
-  R g2[NMAX + 1][MMAX + 1];
-  g2[0][0] = (axis == 2) ? (R)wz : (R)1;
+  // bra transfer of one finished VRR column m: h1[i][j][m] -> block(i,j)+m
+  auto bra = [&](const R* col, int m) {
+    R t[NMAX + 1];
+#pragma unroll
+    for (int a = 0; a <= NMAX; ++a) t[a] = col[a];
+#pragma unroll
+    for (int b = 0; b <= B; ++b) {
+      if (b > 0) {
+#pragma unroll
+        for (int a = 0; a <= A + B - b; ++a) t[a] = t[a + 1] + drb * t[a];
+      }
+#pragma unroll
+      for (int a = 0; a <= A; ++a) {
+        const int i = IBASE ? a : b;
+        const int j = IBASE ? b : a;
+        ga[((i * DJ + j) * BLK + m) * NR] = t[a];
+      }
+    }
+  };
+
+  R cm1[NMAX + 1], c0[NMAX + 1], c1[NMAX + 1];
+  c0[0] = (R)(unit + (1.0 - unit) * wz);  // 1 for x and y, the weight for z
 #pragma unroll
   for (int n = 0; n < NMAX; ++n) {
-    R v = c00 * g2[n][0];
-    if (n > 0) v += (R)n * b10 * g2[n - 1][0];
-    g2[n + 1][0] = v;
+    R v = c00 * c0[n];
+    if (n > 0) v += (R)n * b10 * c0[n - 1];
+    c0[n + 1] = v;
   }
+  bra(c0, 0);
 #pragma unroll
   for (int m = 0; m < MMAX; ++m) {
 #pragma unroll
     for (int n = 0; n <= NMAX; ++n) {
-      R v = c0p * g2[n][m];
-      if (m > 0) v += (R)m * b01 * g2[n][m - 1];
-      if (n > 0) v += (R)n * b00 * g2[n - 1][m];
-      g2[n][m + 1] = v;
+      R v = c0p * c0[n];
+      if (m > 0) v += (R)m * b01 * cm1[n];
+      if (n > 0) v += (R)n * b00 * c0[n - 1];
+      c1[n] = v;
+    }
+    bra(c1, m + 1);
+#pragma unroll
+    for (int n = 0; n <= NMAX; ++n) {
+      cm1[n] = c0[n];
+      c0[n] = c1[n];
     }
   }
-  // bra HRR: H(a, b+1) = H(a+1, b) + dr H(a, b) on the base/other powers.
-  const R drb = (R)rij_ax;
-  R h1[DI][DJ][MMAX + 1];
+  // ket transfer, block by block in place
+  if constexpr (D > 0 || !KBASE) {
 #pragma unroll
-  for (int m = 0; m <= MMAX; ++m) {
-    R t[A + B + 1][B + 1];
+    for (int ij = 0; ij < DI * DJ; ++ij) {
+      R* blk = ga + (size_t)ij * BLK * NR;
+      R t[MMAX + 1];
 #pragma unroll
-    for (int a = 0; a <= A + B; ++a) t[a][0] = g2[a][m];
+      for (int c = 0; c <= MMAX; ++c) t[c] = blk[c * NR];
 #pragma unroll
-    for (int b = 0; b < B; ++b)
+      for (int d = 0; d <= D; ++d) {
+        if (d > 0) {
 #pragma unroll
-      for (int a = 0; a < A + B - b; ++a) t[a][b + 1] = t[a + 1][b] + drb * t[a][b];
+          for (int c = 0; c <= C + D - d; ++c) t[c] = t[c + 1] + drk * t[c];
+        }
 #pragma unroll
-    for (int a = 0; a <= A; ++a)
-#pragma unroll
-      for (int b = 0; b <= B; ++b) {
-        if (IBASE)
-          h1[a][b][m] = t[a][b];
-        else
-          h1[b][a][m] = t[a][b];
-      }
-  }
-  const R drk = (R)rkl_ax;
-  R* __restrict__ ga = g + (size_t)axis * DI * DJ * DK * DL * NR + r;
-#pragma unroll
-  for (int i = 0; i < DI; ++i)
-#pragma unroll
-    for (int j = 0; j < DJ; ++j) {
-      R t[C + D + 1][D + 1];
-#pragma unroll
-      for (int c = 0; c <= C + D; ++c) t[c][0] = h1[i][j][c];
-#pragma unroll
-      for (int d = 0; d < D; ++d)
-#pragma unroll
-        for (int c = 0; c < C + D - d; ++c) t[c][d + 1] = t[c + 1][d] + drk * t[c][d];
-#pragma unroll
-      for (int c = 0; c <= C; ++c)
-#pragma unroll
-        for (int d = 0; d <= D; ++d) {
+        for (int c = 0; c <= C; ++c) {
           const int k = KBASE ? c : d;
           const int l = KBASE ? d : c;
-          ga[(((i * DJ + j) * DK + k) * DL + l) * NR] = t[c][d];
+          blk[(k * DL + l) * NR] = t[c];
         }
+      }
     }
+  }
 }
 
 // Apply the real-spherical transform along one axis of a [l][k][j][i] block
@@ -206,7 +224,7 @@ struct ClassShape {
   static constexpr int NF = ncart(LI) * ncart(LJ) * ncart(LK) * ncart(LL);
   static constexpr int NCOMP = DERIV ? 3 : 1;
   static constexpr int NE = NF * NCOMP;
-  static constexpr int T = NE <= 8 ? 1 : (NE <= 48 ? 4 : (NE <= 192 ? 16 : 64));
+  static constexpr int T = NE <= 8 ? 1 : (NE <= 48 ? 4 : (NE <= 192 ? 16 : (NE <= 768 ? 64 : 256)));
   static constexpr int NFT = (NF + T - 1) / T;
   template <typename R>
   static constexpr int lds_per_tuple() {
@@ -239,9 +257,15 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
   constexpr bool IBASE = LIc > LJ, KBASE = LK > LL;
   constexpr int GS = 3 * G4 * NR;
 
-  __shared__ R sg[Q][GS];
-  __shared__ double suw[Q][2 * NR];
+  // One lane per tuple keeps g and the roots in registers (every index is a
+  // compile-time constant once the loops are unrolled); wider tuples share
+  // them through LDS.
+  constexpr bool REG = (T == 1);
+  __shared__ R sg[Q][REG ? 1 : GS];
+  __shared__ double suw[Q][REG ? 1 : 2 * NR];
   __shared__ double sbuf[Q][2][NF];
+  R greg[REG ? GS : 1];
+  double uwreg[REG ? 2 * NR : 1];
 
   const int slot = threadIdx.x / T;
   const int t = threadIdx.x - slot * T;
@@ -289,8 +313,8 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
   for (int kk = 0; kk < NFT * NCOMP; ++kk) acc[kk] = 0.0;
   unsigned int nsurv = 0;
 
-  R* __restrict__ g = sg[slot];
-  const double* __restrict__ uw = suw[slot];
+  R* __restrict__ g = REG ? greg : sg[slot];
+  double* __restrict__ uw = REG ? uwreg : suw[slot];
 
   for (int kp = 0; kp < a.npkl; ++kp) {
     const int kidx = koff + kp;
@@ -329,20 +353,30 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
       if (ok && t == 0) ++nsurv;
 
       // roots
-      double* uwm = suw[slot];
+#pragma unroll(T == 1 ? NR : 1)
       for (int r = t; r < NR; r += T) {
         double u, w;
         rys_eval(a.rys, NR, r, x, u, w);
-        uwm[r] = u;
-        uwm[NR + r] = w * fac1;
+        uw[r] = u;
+        uw[NR + r] = w * fac1;
       }
       if constexpr (T > 1) __syncthreads();
       // recurrence chains
+#pragma unroll(T == 1 ? 3 * NR : 1)
       for (int c = t; c < 3 * NR; c += T) {
         const int axis = c / NR;
         const int r = c - axis * NR;
-        build_chain<LIc, LJ, LK, LL, R>(g, axis, r, uw[r], uw[NR + r], p, rij[axis],
-                                        rkl[axis]);
+        // the axis selects compile-time members, so no private array is
+        // indexed at run time (that would put it in scratch memory)
+        if (axis == 0)
+          build_chain<LIc, LJ, LK, LL, R>(g, 0, r, uw[r], uw[NR + r], p.a0, p.aij, p.akl, p.a1,
+                                          p.dx[0], p.prb[0], p.qrk[0], 1.0, rij[0], rkl[0]);
+        else if (axis == 1)
+          build_chain<LIc, LJ, LK, LL, R>(g, 1, r, uw[r], uw[NR + r], p.a0, p.aij, p.akl, p.a1,
+                                          p.dx[1], p.prb[1], p.qrk[1], 1.0, rij[1], rkl[1]);
+        else
+          build_chain<LIc, LJ, LK, LL, R>(g, 2, r, uw[r], uw[NR + r], p.a0, p.aij, p.akl, p.a1,
+                                          p.dx[2], p.prb[2], p.qrk[2], 0.0, rij[2], rkl[2]);
       }
       if constexpr (T > 1) __syncthreads();
       // Cartesian elements

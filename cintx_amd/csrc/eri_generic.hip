@@ -1,0 +1,247 @@
+// Runtime-class fallback kernel: any (li, lj, lk, ll) with l <= 4 (+1 for
+// nabla_i), used for the classes that have no compile-time instantiation
+// (high angular momentum: (ff|..), (gg|gg), the cfg-5 workload).  One shell
+// tuple per 256-lane workgroup, grid-stride over tuples; the g tensor, the
+// Cartesian accumulators and the cart->sph buffers live in a per-workgroup
+// global scratch slice (L2/MALL resident), so any class fits.  Arithmetic is
+// the same as eri_class_kernel (libcint g2e.c:272-690, 4425-4620).
+#include "eri_device.h"
+
+namespace cx {
+
+struct GenShape {
+  int l[4];
+  int lic;  // l[0] + deriv
+  int deriv, nr;
+  int di, dj, dk, dl, g4;
+  int nf, ncomp;
+  int ibase, kbase;
+};
+
+__device__ void generic_chain(double* __restrict__ g, const GenShape& S, int axis, int r,
+                              double u, double wz, double a0, double aij, double akl, double a1,
+                              double dx, double prb, double qrk, double drb, double drk) {
+  const int NR = S.nr;
+  const int NMAX = S.lic + S.l[1], MMAX = S.l[2] + S.l[3];
+  const int A = S.ibase ? S.lic : S.l[1], B = S.ibase ? S.l[1] : S.lic;
+  const int C = S.kbase ? S.l[2] : S.l[3], D = S.kbase ? S.l[3] : S.l[2];
+  const int DJ = S.dj, DL = S.dl, BLK = S.dk * S.dl;
+  const double u2 = a0 * u;
+  const double tmp4 = 0.5 / (u2 * (aij + akl) + a1);
+  const double tmp5 = u2 * tmp4;
+  const double b00 = tmp5, b10 = tmp5 + tmp4 * akl, b01 = tmp5 + tmp4 * aij;
+  const double c00 = prb - 2.0 * tmp5 * akl * dx;
+  const double c0p = qrk + 2.0 * tmp5 * aij * dx;
+  double* ga = g + (size_t)axis * S.g4 * NR + r;
+  double cm1[12], c0[12], c1[12], t[12];
+  c0[0] = (axis == 2) ? wz : 1.0;
+  for (int n = 0; n < NMAX; ++n) c0[n + 1] = c00 * c0[n] + (n > 0 ? n * b10 * c0[n - 1] : 0.0);
+  for (int m = 0; m <= MMAX; ++m) {
+    if (m > 0) {
+      for (int n = 0; n <= NMAX; ++n) {
+        double v = c0p * c0[n];
+        if (m > 1) v += (m - 1) * b01 * cm1[n];
+        if (n > 0) v += n * b00 * c0[n - 1];
+        c1[n] = v;
+      }
+      for (int n = 0; n <= NMAX; ++n) {
+        cm1[n] = c0[n];
+        c0[n] = c1[n];
+      }
+    }
+    // bra transfer of column m into block(i, j) slot m
+    for (int a = 0; a <= NMAX; ++a) t[a] = c0[a];
+    for (int b = 0; b <= B; ++b) {
+      if (b > 0)
+        for (int a = 0; a <= A + B - b; ++a) t[a] = t[a + 1] + drb * t[a];
+      for (int a = 0; a <= A; ++a) {
+        const int i = S.ibase ? a : b, j = S.ibase ? b : a;
+        ga[((i * DJ + j) * BLK + m) * NR] = t[a];
+      }
+    }
+  }
+  for (int ij = 0; ij < S.di * S.dj; ++ij) {
+    double* blk = ga + (size_t)ij * BLK * NR;
+    for (int c = 0; c <= MMAX; ++c) t[c] = blk[c * NR];
+    for (int d = 0; d <= D; ++d) {
+      if (d > 0)
+        for (int c = 0; c <= C + D - d; ++c) t[c] = t[c + 1] + drk * t[c];
+      for (int c = 0; c <= C; ++c) {
+        const int k = S.kbase ? c : d, l = S.kbase ? d : c;
+        blk[(k * DL + l) * NR] = t[c];
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
+                                                          int64_t per_block) {
+  const int t = threadIdx.x;
+  const int T = blockDim.x;
+  const int NR = S.nr;
+  double* g = a.scratch + (size_t)blockIdx.x * per_block;
+  double* acc = g + (size_t)3 * S.g4 * NR;
+  double* b0 = acc + (size_t)S.nf * S.ncomp;
+  double* b1 = b0 + S.nf;
+  __shared__ double uw[2 * kMaxRoots];
+  __shared__ unsigned long long cnt[2];
+  const int nfi = ncart(S.l[0]), nfj = ncart(S.l[1]), nfk = ncart(S.l[2]), nfl = ncart(S.l[3]);
+  const int SI = S.dj * S.dk * S.dl;
+  for (int64_t q = blockIdx.x; q < a.nq; q += gridDim.x) {
+    const int2 pq = a.quartets[q];
+    const int boff = a.pair_off[pq.x], koff = a.pair_off[pq.y];
+    const double* gb = a.pair_geo + 8 * (size_t)pq.x;
+    const double* gk = a.pair_geo + 8 * (size_t)pq.y;
+    double ri[3], rj[3], rk[3], rl[3], rij[3], rkl[3];
+    for (int d = 0; d < 3; ++d) {
+      ri[d] = gb[d];
+      rj[d] = gb[3 + d];
+      rk[d] = gk[d];
+      rl[d] = gk[3 + d];
+      rij[d] = S.ibase ? ri[d] - rj[d] : rj[d] - ri[d];
+      rkl[d] = S.kbase ? rk[d] - rl[d] : rl[d] - rk[d];
+    }
+    for (int e = t; e < S.nf * S.ncomp; e += T) acc[e] = 0.0;
+    if (t == 0) cnt[0] = 0;
+    __syncthreads();
+    const double logd_ij = gb[6], logd_kl = gk[6];
+    for (int kp = 0; kp < a.npkl; ++kp) {
+      const int kidx = koff + kp;
+      const double akl = a.aij[kidx];
+      const double Qv[3] = {a.px[kidx], a.py[kidx], a.pz[kidx]};
+      const double ccekl = a.three_center ? 0.0 : a.cce[kidx] - a.lkl_ceil * logd_kl;
+      if (ccekl > a.expcutoff) continue;  // uniform across the workgroup
+      const double eijcut = a.expcutoff - ccekl;
+      for (int ip = 0; ip < a.npij; ++ip) {
+        const int iidx = boff + ip;
+        const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
+        if (cceij > eijcut) continue;
+        const double aij = a.aij[iidx];
+        const double Pv[3] = {a.px[iidx], a.py[iidx], a.pz[iidx]};
+        double dx[3], prb[3], qrk[3], rr = 0;
+        for (int d = 0; d < 3; ++d) {
+          dx[d] = Pv[d] - Qv[d];
+          rr += dx[d] * dx[d];
+          prb[d] = Pv[d] - (S.ibase ? ri[d] : rj[d]);
+          qrk[d] = Qv[d] - (S.kbase ? rk[d] : rl[d]);
+        }
+        const double a1 = aij * akl, a0 = a1 / (aij + akl);
+        const double fac = a.common * a.kfac[iidx] * a.kfac[kidx];
+        const double fac1 = sqrt(a0 / (a1 * a1 * a1)) * fac;
+        const double x = a0 * rr;
+        if (t < NR) {
+          double u, w;
+          rys_eval(a.rys, NR, t, x, u, w);
+          uw[t] = u;
+          uw[NR + t] = w * fac1;
+        }
+        if (t == 0) ++cnt[0];
+        __syncthreads();
+        for (int c = t; c < 3 * NR; c += T) {
+          const int axis = c / NR, r = c - axis * NR;
+          generic_chain(g, S, axis, r, uw[r], uw[NR + r], a0, aij, akl, a1, dx[axis], prb[axis],
+                        qrk[axis], rij[axis], rkl[axis]);
+        }
+        __syncthreads();
+        const double ai2 = -2.0 * a.ai[iidx];
+        for (int e = t; e < S.nf; e += T) {
+          int fi = e % nfi, e1 = e / nfi;
+          int fj = e1 % nfj, e2 = e1 / nfj;
+          int fk = e2 % nfk, fl = e2 / nfk;
+          int ix, iy, iz, jx, jy, jz, kx, ky, kz, lx, ly, lz;
+          cart_decode(S.l[0], fi, ix, iy, iz);
+          cart_decode(S.l[1], fj, jx, jy, jz);
+          cart_decode(S.l[2], fk, kx, ky, kz);
+          cart_decode(S.l[3], fl, lx, ly, lz);
+          const double* gx = g + (size_t)(((ix * S.dj + jx) * S.dk + kx) * S.dl + lx) * NR;
+          const double* gy =
+              g + (size_t)(S.g4 + ((iy * S.dj + jy) * S.dk + ky) * S.dl + ly) * NR;
+          const double* gz =
+              g + (size_t)(2 * S.g4 + ((iz * S.dj + jz) * S.dk + kz) * S.dl + lz) * NR;
+          if (!S.deriv) {
+            double s = 0;
+            for (int r = 0; r < NR; ++r) s += gx[r] * gy[r] * gz[r];
+            acc[e] += s;
+          } else {
+            double sx = 0, sy = 0, sz = 0;
+            for (int r = 0; r < NR; ++r) {
+              double fx = ai2 * gx[SI * NR + r], fy = ai2 * gy[SI * NR + r],
+                     fz = ai2 * gz[SI * NR + r];
+              if (ix > 0) fx += ix * gx[r - SI * NR];
+              if (iy > 0) fy += iy * gy[r - SI * NR];
+              if (iz > 0) fz += iz * gz[r - SI * NR];
+              sx += fx * gy[r] * gz[r];
+              sy += gx[r] * fy * gz[r];
+              sz += gx[r] * gy[r] * fz;
+            }
+            acc[e] += sx;
+            acc[S.nf + e] += sy;
+            acc[2 * S.nf + e] += sz;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (a.counters && t == 0 && cnt[0]) {
+      atomicAdd(a.counters, cnt[0]);
+      atomicAdd(a.counters + 1, 1ull);
+    }
+    const int ns0 = nsph(S.l[0]), ns1 = nsph(S.l[1]), ns2 = nsph(S.l[2]), ns3 = nsph(S.l[3]);
+    const int NS = ns0 * ns1 * ns2 * ns3;
+    for (int comp = 0; comp < S.ncomp; ++comp) {
+      for (int e = t; e < S.nf; e += T) b0[e] = acc[comp * S.nf + e];
+      __syncthreads();
+      int dims[4] = {nfi, nfj, nfk, nfl};
+      double* src = b0;
+      double* dst = b1;
+      for (int ax = 0; ax < 4; ++ax) {
+        if (S.l[ax] < 2) continue;
+        c2s_pass(dst, src, dims, ax, S.l[ax], a.c2s, t, T);
+        dims[ax] = nsph(S.l[ax]);
+        double* tmp = src;
+        src = dst;
+        dst = tmp;
+        __syncthreads();
+      }
+      double* o = a.out + a.out_off[q] + comp * a.st_comp;
+      for (int e = t; e < NS; e += T) {
+        int mi = e % ns0, e1 = e / ns0;
+        int mj = e1 % ns1, e2 = e1 / ns1;
+        int mk = e2 % ns2, ml = e2 / ns2;
+        o[mi * a.st[0] + mj * a.st[1] + mk * a.st[2] + ml * a.st[3]] = src[e];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+int64_t generic_scratch_per_block(const int* l, int deriv) {
+  int lic = l[0] + deriv;
+  int nr = (lic + l[1] + l[2] + l[3]) / 2 + 1;
+  int64_t g4 = (int64_t)(lic + 1) * (l[1] + 1) * (l[2] + 1) * (l[3] + 1);
+  int64_t nf = (int64_t)ncart(l[0]) * ncart(l[1]) * ncart(l[2]) * ncart(l[3]);
+  int64_t n = 3 * g4 * nr + nf * (deriv ? 3 : 1) + 2 * nf;
+  return (n + 31) / 32 * 32;
+}
+
+hipError_t launch_generic(const EriArgs& a, const int* l, int deriv, int grid, hipStream_t s) {
+  GenShape S;
+  for (int c = 0; c < 4; ++c) S.l[c] = l[c];
+  S.deriv = deriv;
+  S.lic = l[0] + deriv;
+  S.nr = (S.lic + l[1] + l[2] + l[3]) / 2 + 1;
+  S.di = S.lic + 1;
+  S.dj = l[1] + 1;
+  S.dk = l[2] + 1;
+  S.dl = l[3] + 1;
+  S.g4 = S.di * S.dj * S.dk * S.dl;
+  S.nf = ncart(l[0]) * ncart(l[1]) * ncart(l[2]) * ncart(l[3]);
+  S.ncomp = deriv ? 3 : 1;
+  S.ibase = S.lic > l[1];
+  S.kbase = l[2] > l[3];
+  int64_t per = generic_scratch_per_block(l, deriv);
+  hipLaunchKernelGGL(eri_generic_kernel, dim3(grid), dim3(256), 0, s, a, S, per);
+  return hipGetLastError();
+}
+
+}  // namespace cx

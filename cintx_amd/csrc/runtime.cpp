@@ -29,6 +29,9 @@ namespace cx {
 
 hipError_t launch_rys_eval(const RysTable& tab, int n, const double* x, int64_t nx, double* u,
                            double* w, hipStream_t s);
+hipError_t launch_generic(const EriArgs& a, const int* l, int deriv, int grid, hipStream_t s);
+int64_t generic_scratch_per_block(const int* l, int deriv);
+static constexpr int kGenericGrid = 1024;
 
 // ------------------------------------------------------------------
 // Errors: fail-closed, typed, thread-local last error.
@@ -351,6 +354,7 @@ struct PairSet {
 struct Bin {
   int l[4];
   int prec;
+  bool generic = false;  // no compile-time instantiation: runtime-class kernel
   int npij, npkl;
   std::vector<int2> q;
   int64_t out_base = 0;  // element offset of the bin's first block
@@ -389,6 +393,7 @@ struct cintx_plan {
   char* blob = nullptr;
   size_t blob_bytes = 0;
   double* d_out = nullptr;
+  double* d_scratch = nullptr;  // generic-kernel workspace (shared by its bins)
   unsigned long long* d_counters = nullptr;  // 2 per bin
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
@@ -534,7 +539,14 @@ static void plan_upload(cintx_plan* P) {
     o_off[i] = take(P->bins[i].q.size() * 8);
   }
   size_t o_cnt = take(P->bins.size() * 2 * 8 + 8);
+  int64_t scratch = 0;
+  for (auto& b : P->bins)
+    if (b.generic) {
+      int64_t grid = std::min<int64_t>((int64_t)b.q.size(), kGenericGrid);
+      scratch = std::max(scratch, grid * generic_scratch_per_block(b.l, kFam[P->family].deriv));
+    }
   size_t o_out = take((size_t)P->total * 8);
+  size_t o_scr = take((size_t)scratch * 8);
   size_t host_bytes = o_out;  // everything before the output is uploaded
   P->blob_bytes = cur;
   HIPCHK(hipMalloc(&P->blob, P->blob_bytes));
@@ -573,6 +585,7 @@ static void plan_upload(cintx_plan* P) {
   P->d_geo = (double*)(P->blob + o_geo);
   P->d_counters = (unsigned long long*)(P->blob + o_cnt);
   P->d_out = (double*)(P->blob + o_out);
+  P->d_scratch = scratch ? (double*)(P->blob + o_scr) : nullptr;
   HIPCHK(hipEventCreate(&P->ev0));
   HIPCHK(hipEventCreate(&P->ev1));
 }
@@ -582,12 +595,7 @@ static void plan_check_kernels(cintx_plan* P) {
   const FamilyInfo& f = kFam[P->family];
   for (auto& b : P->bins) {
     const KernelInfo& k = C.reg->k[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
-    if (!k.fn) {
-      char buf[160];
-      snprintf(buf, sizeof buf, "%s class (%d%d|%d%d) precision %d has no kernel in this build",
-               f.name, b.l[0], b.l[1], b.l[2], b.l[3], b.prec);
-      fail(CINTX_STATUS_UNSUPPORTED_API, buf);
-    }
+    b.generic = (k.fn == nullptr);
   }
 }
 
@@ -599,6 +607,7 @@ static void plan_launch_all(cintx_plan* P, double* out, bool count) {
     Bin& b = P->bins[i];
     const KernelInfo& k = C.reg->k[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
     EriArgs a;
+    // (generic bins run the runtime-class kernel below)
     std::memset(&a, 0, sizeof a);
     a.aij = P->d_aij;
     a.px = P->d_px;
@@ -633,7 +642,14 @@ static void plan_launch_all(cintx_plan* P, double* out, bool count) {
       a.quartets = b.d_q + s;
       a.out_off = b.d_off + s;
       a.nq = std::min(chunk, (int64_t)b.q.size() - s);
-      hipError_t e = k.fn(a, C.stream);
+      hipError_t e;
+      if (b.generic) {
+        a.scratch = P->d_scratch;
+        int grid = (int)std::min<int64_t>(a.nq, kGenericGrid);
+        e = launch_generic(a, b.l, f.deriv, grid, C.stream);
+      } else {
+        e = k.fn(a, C.stream);
+      }
       if (e != hipSuccess)
         fail(CINTX_STATUS_EXECUTION_FAILED, std::string("kernel launch: ") + hipGetErrorString(e));
     }

@@ -185,3 +185,30 @@ def test_screened_plan_cfg2(oracle):
     blk = np.concatenate([got[offs[q]:offs[q] + sizes[k]] for k, q in enumerate(pick)])
     assert_parity(blk, ref)
     p.close()
+
+
+def test_high_l_generic_kernel_cfg5(oracle):
+    # (ff|..) and (gg|gg) classes run the runtime-class kernel
+    mol = M.config(5)
+    rng = np.random.default_rng(8)
+    ls = mol.bas[:, 1]
+    hi = [s for s in range(mol.nbas) if ls[s] >= 3]
+    tuples = [tuple(int(s) for s in rng.integers(0, mol.nbas, 4)) for _ in range(150)]
+    tuples += [(int(hi[i]), int(hi[j]), int(hi[k]), int(hi[l])) for i, j, k, l in
+               rng.integers(0, len(hi), (40, 4))]
+    check_plan(oracle, "int2e_sph", tuples, mol)
+
+
+def test_high_l_ip1_cfg5(oracle):
+    mol = M.config(5)
+    rng = np.random.default_rng(9)
+    tuples = [tuple(int(s) for s in rng.integers(0, mol.nbas, 4)) for _ in range(120)]
+    check_plan(oracle, "int2e_ip1_sph", tuples, mol)
+
+
+def test_general_contraction_all_classes(oracle):
+    mol = M.general_contraction_h2()
+    rng = np.random.default_rng(10)
+    tuples = [tuple(int(s) for s in rng.integers(0, mol.nbas, 4)) for _ in range(200)]
+    check_plan(oracle, "int2e_sph", tuples, mol)
+    check_plan(oracle, "int3c2e_sph", [t[:3] for t in tuples[:100]], mol)
