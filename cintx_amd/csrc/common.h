@@ -29,9 +29,9 @@ __host__ __device__ constexpr int nsph(int l) { return 2 * l + 1; }
 __host__ __device__ constexpr double rys_largex(int n) { return 35.0 + 5.0 * n; }
 
 // Rys table: for n in 1..kMaxRoots, x in [0, largex(n)) split into unit
-// intervals; per interval, per root r: kRysNc Chebyshev coefficients of u_r
-// followed (after all n roots) by those of w_r.  For x >= largex(n) the
-// asymptotic rule u = R/(x-R), w = W/sqrt(x) is used.
+// intervals; per interval, per root r: kRysNc Chebyshev coefficients of
+// s_r = t_r^2 = u_r/(1+u_r) followed (after all n roots) by those of w_r.
+// For x >= largex(n) the asymptotic rule s = R/x, w = W/sqrt(x) is used.
 struct RysTable {
   const double* coef;   // device
   const double* large;  // device: per n at n(n-1): R[0..n-1], W[0..n-1]
@@ -48,6 +48,7 @@ struct EriArgs {
   const double* kfac;   // exp(-a_i a_j/a_ij |R_ij|^2) * c_i * c_j
   const double* ai;     // exponent of the first shell (nabla_i)
   const double* cce;    // screening key without the l-dependent term
+  const double* hinv;   // 0.5 / a_ij
   // per pair
   const int* pair_off;  // first prim pair of the pair
   const double* pair_geo;  // 8 per pair: ri[3], rj[3], log(|Rij|+1), unused

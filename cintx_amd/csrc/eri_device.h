@@ -43,14 +43,16 @@ __host__ __device__ constexpr void cart_decode(int l, int c, int& lx, int& ly, i
   lx = ly = lz = 0;
 }
 
-// Rys roots/weights for n roots, root r, argument x (rys_roots.c:57-123
-// regimes: Chebyshev table below 35+5n, asymptotic Hermite rule above).
-__device__ __forceinline__ void rys_eval(const RysTable& tab, int n, int r, double x, double& u,
-                                         double& w) {
+// Rys rule for n roots, root r, argument x: s = t^2 = u/(1+u), q = 1 - s and
+// the weight (rys_roots.c:57-123 regimes: Chebyshev table below 35+5n,
+// asymptotic Hermite rule above).  The table holds, per root and interval,
+// the fit of the smaller of s and q; a fitted q is stored negated.
+__device__ __forceinline__ void rys_eval(const RysTable& tab, int n, int r, double x, double& s,
+                                         double& q, double& w) {
   if (!(x < rys_largex(n))) {
     const double* L = tab.large + n * (n - 1);
-    double R = L[r];
-    u = R / (x - R);
+    s = L[r] / x;
+    q = 1.0 - s;
     w = L[n + r] / sqrt(x);
     return;
   }
@@ -69,13 +71,18 @@ __device__ __forceinline__ void rys_eval(const RysTable& tab, int n, int r, doub
     bw2 = bw1;
     bw1 = bw0;
   }
-  u = fma(t, bu1, cu[0] - bu2);
+  const double v = fma(t, bu1, cu[0] - bu2);
+  const double small = fabs(v);
+  s = v < 0 ? 1.0 - small : small;
+  q = v < 0 ? small : 1.0 - small;
   w = fma(t, bw1, cw[0] - bw2);
 }
 
 // Per-tuple scalars of one primitive tuple.
 struct PrimTuple {
   double aij, akl, a0, a1;
+  double inv_sum;        // 1/(a_ij + a_kl)
+  double hij, hkl;       // 0.5/a_ij, 0.5/a_kl
   double dx[3];   // P - Q
   double prb[3];  // P - R(bra base)
   double qrk[3];  // Q - R(ket base)
@@ -92,10 +99,11 @@ struct PrimTuple {
 // about 4*(NMAX+1) + (MMAX+1) values live instead of the whole 2D and 3D
 // intermediates.
 template <int LIc, int LJ, int LK, int LL, typename R>
-__device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, double u,
-                                            double wz, double a0, double aij, double akl,
-                                            double a1, double dx, double prb, double qrk,
-                                            double unit, double rij_ax, double rkl_ax) {
+__device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, double s,
+                                            double q, double wz, double inv_sum, double hij, double hkl,
+                                            double aij, double akl, double dx, double prb,
+                                            double qrk, double unit, double rij_ax,
+                                            double rkl_ax) {
   constexpr int NR = (LIc + LJ + LK + LL) / 2 + 1;
   constexpr int NMAX = LIc + LJ, MMAX = LK + LL;
   constexpr bool IBASE = LIc > LJ, KBASE = LK > LL;
@@ -104,14 +112,15 @@ __device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, 
   constexpr int C = KBASE ? LK : LL, D = KBASE ? LL : LK;
   constexpr int BLK = DK * DL;  // positions per (i, j) block
 
-  const double u2 = a0 * u;
-  const double tmp4 = 0.5 / (u2 * (aij + akl) + a1);
-  const double tmp5 = u2 * tmp4;
-  const R b00 = (R)tmp5;
-  const R b10 = (R)(tmp5 + tmp4 * akl);
-  const R b01 = (R)(tmp5 + tmp4 * aij);
-  const R c00 = (R)(prb - 2.0 * tmp5 * akl * dx);
-  const R c0p = (R)(qrk + 2.0 * tmp5 * aij * dx);
+  // libcint's per-root coefficients (g2e.c:4523-4543) written in s = u/(1+u):
+  // tmp4 = .5/(a1 (1+u)) = .5 (1-s)/a1 and tmp5 = u2 tmp4 = .5 s/(aij+akl).
+  const double sa = s * inv_sum;
+  const double oms = q;  // 1 - s with full relative accuracy
+  const R b00 = (R)(0.5 * sa);
+  const R b10 = (R)(0.5 * sa + oms * hij);
+  const R b01 = (R)(0.5 * sa + oms * hkl);
+  const R c00 = (R)(prb - sa * akl * dx);
+  const R c0p = (R)(qrk + sa * aij * dx);
   const R drb = (R)rij_ax;
   const R drk = (R)rkl_ax;
   R* __restrict__ ga = g + (size_t)axis * DI * DJ * BLK * NR + r;
@@ -228,7 +237,7 @@ struct ClassShape {
   static constexpr int NFT = (NF + T - 1) / T;
   template <typename R>
   static constexpr int lds_per_tuple() {
-    return (int)(3 * G4 * NR * sizeof(R) + 2 * NR * sizeof(double) + 2 * NF * sizeof(double));
+    return (int)(3 * G4 * NR * sizeof(R) + 3 * NR * sizeof(double) + 2 * NF * sizeof(double));
   }
   template <typename R>
   static constexpr int Q() {
@@ -262,10 +271,10 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
   // them through LDS.
   constexpr bool REG = (T == 1);
   __shared__ R sg[Q][REG ? 1 : GS];
-  __shared__ double suw[Q][REG ? 1 : 2 * NR];
+  __shared__ double suw[Q][REG ? 1 : 3 * NR];
   __shared__ double sbuf[Q][2][NF];
   R greg[REG ? GS : 1];
-  double uwreg[REG ? 2 * NR : 1];
+  double uwreg[REG ? 3 * NR : 1];
 
   const int slot = threadIdx.x / T;
   const int t = threadIdx.x - slot * T;
@@ -321,6 +330,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
     const double akl = a.aij[kidx];
     const double Qv[3] = {a.px[kidx], a.py[kidx], a.pz[kidx]};
     const double kkl = a.kfac[kidx];
+    const double hkl = a.hinv[kidx];
     const double ccekl = a.three_center ? 0.0 : a.cce[kidx] - a.lkl_ceil * logd_kl;
     const bool klok = active && ccekl <= a.expcutoff;
     const double eijcut = a.expcutoff - ccekl;
@@ -345,20 +355,25 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
         p.prb[d] = Pv[d] - (IBASE ? ri[d] : rj[d]);
         p.qrk[d] = Qv[d] - (KBASE ? rk[d] : rl[d]);
       }
+      p.inv_sum = 1.0 / (p.aij + akl);
+      p.hij = a.hinv[iidx];
+      p.hkl = hkl;
       p.a1 = p.aij * akl;
-      p.a0 = p.a1 / (p.aij + akl);
+      p.a0 = p.a1 * p.inv_sum;
+      // sqrt(a0/a1^3) = 1/(a1 sqrt(aij+akl)) = 4 hij hkl sqrt(inv_sum)
       const double fac = ok ? a.common * a.kfac[iidx] * kkl : 0.0;
-      const double fac1 = sqrt(p.a0 / (p.a1 * p.a1 * p.a1)) * fac;
+      const double fac1 = 4.0 * p.hij * hkl * sqrt(p.inv_sum) * fac;
       const double x = p.a0 * rr;
       if (ok && t == 0) ++nsurv;
 
       // roots
 #pragma unroll(T == 1 ? NR : 1)
       for (int r = t; r < NR; r += T) {
-        double u, w;
-        rys_eval(a.rys, NR, r, x, u, w);
-        uw[r] = u;
+        double sr, qr, w;
+        rys_eval(a.rys, NR, r, x, sr, qr, w);
+        uw[r] = sr;
         uw[NR + r] = w * fac1;
+        uw[2 * NR + r] = qr;
       }
       if constexpr (T > 1) __syncthreads();
       // recurrence chains
@@ -369,14 +384,20 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
         // the axis selects compile-time members, so no private array is
         // indexed at run time (that would put it in scratch memory)
         if (axis == 0)
-          build_chain<LIc, LJ, LK, LL, R>(g, 0, r, uw[r], uw[NR + r], p.a0, p.aij, p.akl, p.a1,
-                                          p.dx[0], p.prb[0], p.qrk[0], 1.0, rij[0], rkl[0]);
+          build_chain<LIc, LJ, LK, LL, R>(g, 0, r, uw[r], uw[2 * NR + r], uw[NR + r], p.inv_sum,
+                                          p.hij, p.hkl,
+                                          p.aij, p.akl, p.dx[0], p.prb[0], p.qrk[0], 1.0,
+                                          rij[0], rkl[0]);
         else if (axis == 1)
-          build_chain<LIc, LJ, LK, LL, R>(g, 1, r, uw[r], uw[NR + r], p.a0, p.aij, p.akl, p.a1,
-                                          p.dx[1], p.prb[1], p.qrk[1], 1.0, rij[1], rkl[1]);
+          build_chain<LIc, LJ, LK, LL, R>(g, 1, r, uw[r], uw[2 * NR + r], uw[NR + r], p.inv_sum,
+                                          p.hij, p.hkl,
+                                          p.aij, p.akl, p.dx[1], p.prb[1], p.qrk[1], 1.0,
+                                          rij[1], rkl[1]);
         else
-          build_chain<LIc, LJ, LK, LL, R>(g, 2, r, uw[r], uw[NR + r], p.a0, p.aij, p.akl, p.a1,
-                                          p.dx[2], p.prb[2], p.qrk[2], 0.0, rij[2], rkl[2]);
+          build_chain<LIc, LJ, LK, LL, R>(g, 2, r, uw[r], uw[2 * NR + r], uw[NR + r], p.inv_sum,
+                                          p.hij, p.hkl,
+                                          p.aij, p.akl, p.dx[2], p.prb[2], p.qrk[2], 0.0,
+                                          rij[2], rkl[2]);
       }
       if constexpr (T > 1) __syncthreads();
       // Cartesian elements

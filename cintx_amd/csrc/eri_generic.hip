@@ -19,19 +19,18 @@ struct GenShape {
 };
 
 __device__ void generic_chain(double* __restrict__ g, const GenShape& S, int axis, int r,
-                              double u, double wz, double a0, double aij, double akl, double a1,
-                              double dx, double prb, double qrk, double drb, double drk) {
+                              double sr, double qr, double wz, double inv_sum, double hij, double hkl,
+                              double aij, double akl, double dx, double prb, double qrk,
+                              double drb, double drk) {
   const int NR = S.nr;
   const int NMAX = S.lic + S.l[1], MMAX = S.l[2] + S.l[3];
   const int A = S.ibase ? S.lic : S.l[1], B = S.ibase ? S.l[1] : S.lic;
   const int C = S.kbase ? S.l[2] : S.l[3], D = S.kbase ? S.l[3] : S.l[2];
   const int DJ = S.dj, DL = S.dl, BLK = S.dk * S.dl;
-  const double u2 = a0 * u;
-  const double tmp4 = 0.5 / (u2 * (aij + akl) + a1);
-  const double tmp5 = u2 * tmp4;
-  const double b00 = tmp5, b10 = tmp5 + tmp4 * akl, b01 = tmp5 + tmp4 * aij;
-  const double c00 = prb - 2.0 * tmp5 * akl * dx;
-  const double c0p = qrk + 2.0 * tmp5 * aij * dx;
+  const double sa = sr * inv_sum, oms = qr;
+  const double b00 = 0.5 * sa, b10 = 0.5 * sa + oms * hij, b01 = 0.5 * sa + oms * hkl;
+  const double c00 = prb - sa * akl * dx;
+  const double c0p = qrk + sa * aij * dx;
   double* ga = g + (size_t)axis * S.g4 * NR + r;
   double cm1[12], c0[12], c1[12], t[12];
   c0[0] = (axis == 2) ? wz : 1.0;
@@ -83,7 +82,7 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
   double* acc = g + (size_t)3 * S.g4 * NR;
   double* b0 = acc + (size_t)S.nf * S.ncomp;
   double* b1 = b0 + S.nf;
-  __shared__ double uw[2 * kMaxRoots];
+  __shared__ double uw[3 * kMaxRoots];
   __shared__ unsigned long long cnt[2];
   const int nfi = ncart(S.l[0]), nfj = ncart(S.l[1]), nfk = ncart(S.l[2]), nfl = ncart(S.l[3]);
   const int SI = S.dj * S.dk * S.dl;
@@ -125,22 +124,26 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
           prb[d] = Pv[d] - (S.ibase ? ri[d] : rj[d]);
           qrk[d] = Qv[d] - (S.kbase ? rk[d] : rl[d]);
         }
-        const double a1 = aij * akl, a0 = a1 / (aij + akl);
+        const double inv_sum = 1.0 / (aij + akl);
+        const double hij = a.hinv[iidx], hkl = a.hinv[kidx];
+        const double a0 = aij * akl * inv_sum;
         const double fac = a.common * a.kfac[iidx] * a.kfac[kidx];
-        const double fac1 = sqrt(a0 / (a1 * a1 * a1)) * fac;
+        const double fac1 = 4.0 * hij * hkl * sqrt(inv_sum) * fac;
         const double x = a0 * rr;
         if (t < NR) {
-          double u, w;
-          rys_eval(a.rys, NR, t, x, u, w);
-          uw[t] = u;
+          double sr, qr, w;
+          rys_eval(a.rys, NR, t, x, sr, qr, w);
+          uw[t] = sr;
           uw[NR + t] = w * fac1;
+          uw[2 * NR + t] = qr;
         }
         if (t == 0) ++cnt[0];
         __syncthreads();
         for (int c = t; c < 3 * NR; c += T) {
           const int axis = c / NR, r = c - axis * NR;
-          generic_chain(g, S, axis, r, uw[r], uw[NR + r], a0, aij, akl, a1, dx[axis], prb[axis],
-                        qrk[axis], rij[axis], rkl[axis]);
+          generic_chain(g, S, axis, r, uw[r], uw[2 * NR + r], uw[NR + r], inv_sum, hij, hkl, aij,
+                        akl, dx[axis],
+                        prb[axis], qrk[axis], rij[axis], rkl[axis]);
         }
         __syncthreads();
         const double ai2 = -2.0 * a.ai[iidx];

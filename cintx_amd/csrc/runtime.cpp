@@ -280,7 +280,7 @@ static HostBasis ingest(const int* atm, int natm, const int* bas, int nbas, cons
 // l_ij*log(|R_ij|+1) term, which the kernel subtracts per class.
 // ------------------------------------------------------------------
 struct PairSet {
-  std::vector<double> aij, px, py, pz, kfac, ai, cce;
+  std::vector<double> aij, px, py, pz, kfac, ai, cce, hinv;
   std::vector<int> off, np;
   std::vector<double> geo;
   std::unordered_map<uint64_t, int> index;
@@ -306,6 +306,7 @@ struct PairSet {
         double eij = rr * a_i * a_j * inv;
         double wj = a_j * inv;
         aij.push_back(a_i + a_j);
+        hinv.push_back(0.5 * inv);
         px.push_back(A.r[0] + wj * (Bs.r[0] - A.r[0]));
         py.push_back(A.r[1] + wj * (Bs.r[1] - A.r[1]));
         pz.push_back(A.r[2] + wj * (Bs.r[2] - A.r[2]));
@@ -332,6 +333,7 @@ struct PairSet {
     np.push_back(K.nprim);
     for (int kp = 0; kp < K.nprim; ++kp) {
       aij.push_back(B.exps[K.off + kp]);
+      hinv.push_back(0.5 / B.exps[K.off + kp]);
       px.push_back(K.r[0]);
       py.push_back(K.r[1]);
       pz.push_back(K.r[2]);
@@ -402,7 +404,7 @@ struct cintx_plan {
   double* graph_out = nullptr;
   // device pointers of the pair SoA
   double *d_aij = nullptr, *d_px = nullptr, *d_py = nullptr, *d_pz = nullptr, *d_kfac = nullptr,
-         *d_ai = nullptr, *d_cce = nullptr, *d_geo = nullptr;
+         *d_ai = nullptr, *d_cce = nullptr, *d_geo = nullptr, *d_hinv = nullptr;
   int* d_poff = nullptr;
 };
 
@@ -531,7 +533,7 @@ static void plan_upload(cintx_plan* P) {
     return o;
   };
   size_t o_aij = take(npp * 8), o_px = take(npp * 8), o_py = take(npp * 8), o_pz = take(npp * 8),
-         o_k = take(npp * 8), o_ai = take(npp * 8), o_cce = take(npp * 8);
+         o_k = take(npp * 8), o_ai = take(npp * 8), o_cce = take(npp * 8), o_h = take(npp * 8);
   size_t o_poff = take(npair * 4), o_geo = take(npair * 64);
   std::vector<size_t> o_q(P->bins.size()), o_off(P->bins.size());
   for (size_t i = 0; i < P->bins.size(); ++i) {
@@ -561,6 +563,7 @@ static void plan_upload(cintx_plan* P) {
   put(o_k, S.kfac.data(), npp * 8);
   put(o_ai, S.ai.data(), npp * 8);
   put(o_cce, S.cce.data(), npp * 8);
+  put(o_h, S.hinv.data(), npp * 8);
   put(o_poff, S.off.data(), npair * 4);
   put(o_geo, S.geo.data(), npair * 64);
   for (size_t i = 0; i < P->bins.size(); ++i) {
@@ -581,6 +584,7 @@ static void plan_upload(cintx_plan* P) {
   P->d_kfac = (double*)(P->blob + o_k);
   P->d_ai = (double*)(P->blob + o_ai);
   P->d_cce = (double*)(P->blob + o_cce);
+  P->d_hinv = (double*)(P->blob + o_h);
   P->d_poff = (int*)(P->blob + o_poff);
   P->d_geo = (double*)(P->blob + o_geo);
   P->d_counters = (unsigned long long*)(P->blob + o_cnt);
@@ -616,6 +620,7 @@ static void plan_launch_all(cintx_plan* P, double* out, bool count) {
     a.kfac = P->d_kfac;
     a.ai = P->d_ai;
     a.cce = P->d_cce;
+    a.hinv = P->d_hinv;
     a.pair_off = P->d_poff;
     a.pair_geo = P->d_geo;
     a.npij = b.npij;
@@ -1394,7 +1399,8 @@ int cintx_rys_eval_host(int n, const double* x, int64_t nx, double* u, double* w
       for (int r = 0; r < n; ++r) {
         if (!(xi < rys_largex(n))) {
           const double* L = large.data() + n * (n - 1);
-          u[i * n + r] = L[r] / (xi - L[r]);
+          double sr = L[r] / xi;
+          u[i * n + r] = sr / (1.0 - sr);  // the asymptotic s is small
           w[i * n + r] = L[n + r] / std::sqrt(xi);
           continue;
         }
@@ -1411,7 +1417,10 @@ int cintx_rys_eval_host(int n, const double* x, int64_t nx, double* u, double* w
           bw2 = bw1;
           bw1 = bw0;
         }
-        u[i * n + r] = std::fma(t, bu1, cu[0] - bu2);
+        double v = std::fma(t, bu1, cu[0] - bu2);
+        double small = std::fabs(v);
+        double sr = v < 0 ? 1.0 - small : small, qr = v < 0 ? small : 1.0 - small;
+        u[i * n + r] = sr / qr;
         w[i * n + r] = std::fma(t, bw1, cw[0] - bw2);
       }
     }

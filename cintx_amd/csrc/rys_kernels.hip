@@ -11,9 +11,9 @@ __global__ void rys_eval_kernel(RysTable tab, int n, const double* __restrict__ 
   if (i >= nx) return;
   double xi = x[i];
   for (int r = 0; r < n; ++r) {
-    double ur, wr;
-    rys_eval(tab, n, r, xi, ur, wr);
-    u[i * n + r] = ur;
+    double sr, qr, wr;
+    rys_eval(tab, n, r, xi, sr, qr, wr);
+    u[i * n + r] = sr / qr;  // libcint's u = t^2/(1-t^2)
     w[i * n + r] = wr;
   }
 }

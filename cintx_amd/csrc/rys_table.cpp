@@ -8,9 +8,11 @@
 // Gauss-Legendre nodes and running the Stieltjes procedure (W. Gautschi,
 // SIAM J. Sci. Stat. Comput. 3 (1982) 289) followed by the Golub-Welsch
 // eigen-solve (implicit QL).  The device evaluates degree-11 Chebyshev fits
-// of u_r(x) and w_r(x) on unit x-intervals below 35+5n (the LARGEX bound of
-// rys_roots.c:67) and the asymptotic Hermite rule u = R/(x-R), w = W/sqrt(x)
-// above it.
+// of s_r(x) = t_r^2 = u_r/(1+u_r) (or of q_r = 1 - s_r where s_r > 1/2, stored
+// negated) and w_r(x) on unit x-intervals below 35+5n (the LARGEX bound of
+// rys_roots.c:67) and the asymptotic Hermite rule s = R/x, w = W/sqrt(x)
+// above it.  Keeping s and q rather than u makes every per-root coefficient
+// of the recurrences division-free without losing accuracy near s = 1.
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -198,17 +200,23 @@ void rys_table_build(std::vector<double>& coef, std::vector<double>& large,
         long double s[kMaxRoots + 2], w[kMaxRoots + 2];
         rys_rule_ld(n, x, s, w);
         for (int r = 0; r < n; ++r) {
-          su[k][r] = s[r] / (1.0L - s[r]);
+          su[k][r] = s[r];  // t^2; u = s/(1-s) is formed only where needed
           sw[k][r] = w[r];
         }
       }
       double* blk = coef.data() + off[n] + (size_t)iv * 2 * n * kRysNc;
       for (int r = 0; r < n; ++r) {
+        // Fit whichever of s and q = 1-s is the smaller on this interval, so
+        // both come out with full relative accuracy (q matters near s = 1);
+        // a fitted q is stored negated, which the evaluator detects by sign.
+        long double smid = 0;
+        for (int k = 0; k < kRysNc; ++k) smid += su[k][r];
+        const bool fit_q = smid / kRysNc > 0.5L;
         for (int j = 0; j < kRysNc; ++j) {
           long double cu = 0, cw = 0;
           for (int k = 0; k < kRysNc; ++k) {
             long double tj = cosl(pi * j * (k + 0.5L) / kRysNc);
-            cu += su[k][r] * tj;
+            cu += (fit_q ? -(1.0L - su[k][r]) : su[k][r]) * tj;
             cw += sw[k][r] * tj;
           }
           long double sc = (j == 0 ? 1.0L : 2.0L) / kRysNc;
