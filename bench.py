@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sample", type=int, default=4000)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the class kernels directly instead of replaying a hipGraph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -67,12 +69,12 @@ def main():
 
     def timed(plan, label):
         for _ in range(args.warmup):
-            plan.execute(graph=True)
+            plan.execute(graph=not args.no_graph)
             plan.sync()
         ms = []
         t_start = time.perf_counter()
         for _ in range(args.steps):
-            plan.execute(graph=True)
+            plan.execute(graph=not args.no_graph)
             plan.sync()
             ms.append(plan.last_ms())
         wall = (time.perf_counter() - t_start) * 1e3 / args.steps

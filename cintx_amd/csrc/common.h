@@ -51,12 +51,12 @@ struct EriArgs {
   const double* hinv;   // 0.5 / a_ij
   // per pair
   const int* pair_off;  // first prim pair of the pair
+  const int* pair_np;   // primitive pairs of the pair
   const double* pair_geo;  // 8 per pair: ri[3], rj[3], log(|Rij|+1), unused
   // work list
   const int2* quartets;    // (bra pair, ket pair)
   const int64_t* out_off;  // output offset (elements) of each quartet block
   int64_t nq;
-  int npij, npkl;          // primitive pairs per bra / ket pair (uniform)
   double common;           // pi^3 2/sqrt(pi) prod common_fac_sp
   double expcutoff;
   int lij_ceil, lkl_ceil;  // li_ceil+lj_ceil, lk_ceil+ll_ceil for the screen

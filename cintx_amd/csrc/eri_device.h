@@ -325,19 +325,40 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
   R* __restrict__ g = REG ? greg : sg[slot];
   double* __restrict__ uw = REG ? uwreg : suw[slot];
 
-  for (int kp = 0; kp < a.npkl; ++kp) {
-    const int kidx = koff + kp;
+  // Primitive counts are per tuple; a workgroup of several tuples loops to
+  // its largest counts (tuples are sorted by (npij, npkl) so that is rarely
+  // more) and masks the excess iterations.
+  const int np_ij = a.pair_np[pq.x], np_kl = a.pair_np[pq.y];
+  int mij = np_ij, mkl = np_kl;
+  if constexpr (T > 1 || Q > 1) {
+    if constexpr (T > 1) {
+      __shared__ int smax[2];
+      if (threadIdx.x == 0) smax[0] = smax[1] = 0;
+      __syncthreads();
+      if (t == 0 && active) {
+        atomicMax(&smax[0], np_ij);
+        atomicMax(&smax[1], np_kl);
+      }
+      __syncthreads();
+      mij = smax[0];
+      mkl = smax[1];
+    }
+  }
+  for (int kp = 0; kp < mkl; ++kp) {
+    const bool kin = kp < np_kl;
+    const int kidx = koff + (kin ? kp : 0);
     const double akl = a.aij[kidx];
     const double Qv[3] = {a.px[kidx], a.py[kidx], a.pz[kidx]};
     const double kkl = a.kfac[kidx];
     const double hkl = a.hinv[kidx];
     const double ccekl = a.three_center ? 0.0 : a.cce[kidx] - a.lkl_ceil * logd_kl;
-    const bool klok = active && ccekl <= a.expcutoff;
+    const bool klok = active && kin && ccekl <= a.expcutoff;
     const double eijcut = a.expcutoff - ccekl;
-    for (int ip = 0; ip < a.npij; ++ip) {
-      const int iidx = boff + ip;
+    for (int ip = 0; ip < mij; ++ip) {
+      const bool iin = ip < np_ij;
+      const int iidx = boff + (iin ? ip : 0);
       const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
-      const bool ok = klok && cceij <= eijcut;
+      const bool ok = klok && iin && cceij <= eijcut;
       if constexpr (T > 1) {
         if (!__syncthreads_or(ok)) continue;
       } else {

@@ -104,14 +104,15 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
     if (t == 0) cnt[0] = 0;
     __syncthreads();
     const double logd_ij = gb[6], logd_kl = gk[6];
-    for (int kp = 0; kp < a.npkl; ++kp) {
+    const int np_ij = a.pair_np[pq.x], np_kl = a.pair_np[pq.y];
+    for (int kp = 0; kp < np_kl; ++kp) {
       const int kidx = koff + kp;
       const double akl = a.aij[kidx];
       const double Qv[3] = {a.px[kidx], a.py[kidx], a.pz[kidx]};
       const double ccekl = a.three_center ? 0.0 : a.cce[kidx] - a.lkl_ceil * logd_kl;
       if (ccekl > a.expcutoff) continue;  // uniform across the workgroup
       const double eijcut = a.expcutoff - ccekl;
-      for (int ip = 0; ip < a.npij; ++ip) {
+      for (int ip = 0; ip < np_ij; ++ip) {
         const int iidx = boff + ip;
         const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
         if (cceij > eijcut) continue;

@@ -357,7 +357,6 @@ struct Bin {
   int l[4];
   int prec;
   bool generic = false;  // no compile-time instantiation: runtime-class kernel
-  int npij, npkl;
   std::vector<int2> q;
   int64_t out_base = 0;  // element offset of the bin's first block
   int64_t block = 0;     // elements per block (NS * ncomp)
@@ -406,6 +405,7 @@ struct cintx_plan {
   double *d_aij = nullptr, *d_px = nullptr, *d_py = nullptr, *d_pz = nullptr, *d_kfac = nullptr,
          *d_ai = nullptr, *d_cce = nullptr, *d_geo = nullptr, *d_hinv = nullptr;
   int* d_poff = nullptr;
+  int* d_pnp = nullptr;
 };
 
 namespace cx {
@@ -467,19 +467,15 @@ static void plan_add_tuples(cintx_plan* P, const int* tuples, int64_t nt, int pr
     tr.nat_off = P->total;
     tr.nat_size = nat_size(P->family, B, tr.shl);
     P->total += tr.nat_size;
-    int npij = B.nprim[cs[0]] * B.nprim[cs[1]];
-    int npkl = f.three ? B.nprim[cs[2]] : B.nprim[cs[2]] * B.nprim[cs[3]];
     uint64_t key = 0;
     for (int c = 0; c < 4; ++c) key = key * 8 + cl[c];
-    key = ((key * 2 + prec) * 128 + npij) * 128 + npkl;
+    key = key * 2 + prec;
     int bi;
     auto it = bin_of.find(key);
     if (it == bin_of.end()) {
       Bin b;
       for (int c = 0; c < 4; ++c) b.l[c] = cl[c];
       b.prec = prec;
-      b.npij = npij;
-      b.npkl = npkl;
       b.block = (int64_t)f.ncomp * nsph(cl[0]) * nsph(cl[1]) * nsph(cl[2]) * nsph(cl[3]);
       bi = (int)P->bins.size();
       P->bins.push_back(b);
@@ -515,8 +511,36 @@ static void plan_add_tuples(cintx_plan* P, const int* tuples, int64_t nt, int pr
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Lay out and upload pairs, bins and offsets in one device blob.
+// Order every bin's tuples by (bra, ket) primitive-pair counts (counting
+// sort) so that the tuples sharing a workgroup have equal loop trip counts.
+static void plan_sort_bins(cintx_plan* P) {
+  const PairSet& S = P->pairs;
+  std::vector<std::vector<int64_t>> remap(P->bins.size());
+  for (size_t bi = 0; bi < P->bins.size(); ++bi) {
+    Bin& b = P->bins[bi];
+    const int64_t n = (int64_t)b.q.size();
+    const int K = 65 * 65;
+    std::vector<int64_t> count(K + 1, 0);
+    auto key = [&](const int2& v) {
+      return std::min(S.np[v.x], 64) * 65 + std::min(S.np[v.y], 64);
+    };
+    for (const int2& v : b.q) ++count[key(v) + 1];
+    for (int k = 0; k < K; ++k) count[k + 1] += count[k];
+    std::vector<int2> sorted((size_t)n);
+    remap[bi].resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+      int64_t d = count[key(b.q[i])]++;
+      sorted[(size_t)d] = b.q[i];
+      remap[bi][(size_t)i] = d;
+    }
+    b.q.swap(sorted);
+  }
+  for (VQ& v : P->vqs) v.idx = remap[v.bin][(size_t)v.idx];
+}
+
 static void plan_upload(cintx_plan* P) {
   DevCtx& C = ctx();
+  plan_sort_bins(P);
   int64_t base = 0;
   for (auto& b : P->bins) {
     b.out_base = base;
@@ -534,7 +558,7 @@ static void plan_upload(cintx_plan* P) {
   };
   size_t o_aij = take(npp * 8), o_px = take(npp * 8), o_py = take(npp * 8), o_pz = take(npp * 8),
          o_k = take(npp * 8), o_ai = take(npp * 8), o_cce = take(npp * 8), o_h = take(npp * 8);
-  size_t o_poff = take(npair * 4), o_geo = take(npair * 64);
+  size_t o_poff = take(npair * 4), o_pnp = take(npair * 4), o_geo = take(npair * 64);
   std::vector<size_t> o_q(P->bins.size()), o_off(P->bins.size());
   for (size_t i = 0; i < P->bins.size(); ++i) {
     o_q[i] = take(P->bins[i].q.size() * sizeof(int2));
@@ -565,6 +589,7 @@ static void plan_upload(cintx_plan* P) {
   put(o_cce, S.cce.data(), npp * 8);
   put(o_h, S.hinv.data(), npp * 8);
   put(o_poff, S.off.data(), npair * 4);
+  put(o_pnp, S.np.data(), npair * 4);
   put(o_geo, S.geo.data(), npair * 64);
   for (size_t i = 0; i < P->bins.size(); ++i) {
     Bin& b = P->bins[i];
@@ -586,6 +611,7 @@ static void plan_upload(cintx_plan* P) {
   P->d_cce = (double*)(P->blob + o_cce);
   P->d_hinv = (double*)(P->blob + o_h);
   P->d_poff = (int*)(P->blob + o_poff);
+  P->d_pnp = (int*)(P->blob + o_pnp);
   P->d_geo = (double*)(P->blob + o_geo);
   P->d_counters = (unsigned long long*)(P->blob + o_cnt);
   P->d_out = (double*)(P->blob + o_out);
@@ -622,9 +648,8 @@ static void plan_launch_all(cintx_plan* P, double* out, bool count) {
     a.cce = P->d_cce;
     a.hinv = P->d_hinv;
     a.pair_off = P->d_poff;
+    a.pair_np = P->d_pnp;
     a.pair_geo = P->d_geo;
-    a.npij = b.npij;
-    a.npkl = b.npkl;
     a.common = common_base * common_fac_sp(b.l[0]) * common_fac_sp(b.l[1]) *
                common_fac_sp(b.l[2]) * (f.three ? 1.0 : common_fac_sp(b.l[3]));
     a.expcutoff = P->expcutoff;
