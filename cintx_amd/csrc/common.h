@@ -55,7 +55,8 @@ struct EriArgs {
   const double* pair_geo;  // 8 per pair: ri[3], rj[3], log(|Rij|+1), unused
   // work list
   const int2* quartets;    // (bra pair, ket pair)
-  const int64_t* out_off;  // output offset (elements) of each quartet block
+  int64_t out_base;        // element offset of this launch's first block
+  int64_t block;           // elements per block (components included)
   int64_t nq;
   double common;           // pi^3 2/sqrt(pi) prod common_fac_sp
   double expcutoff;

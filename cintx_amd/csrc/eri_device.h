@@ -503,7 +503,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
       if constexpr (T > 1) __syncthreads();
     }
     if (active) {
-      double* o = a.out + a.out_off[q] + comp * a.st_comp;
+      double* o = a.out + a.out_base + q * a.block + comp * a.st_comp;
       for (int e = t; e < NS; e += T) {
         int mi = e % NSI, e1 = e / NSI;
         int mj = e1 % NSJ, e2 = e1 / NSJ;

@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
         dst = tmp;
         __syncthreads();
       }
-      double* o = a.out + a.out_off[q] + comp * a.st_comp;
+      double* o = a.out + a.out_base + q * a.block + comp * a.st_comp;
       for (int e = t; e < NS; e += T) {
         int mi = e % ns0, e1 = e / ns0;
         int mj = e1 % ns1, e2 = e1 / ns1;

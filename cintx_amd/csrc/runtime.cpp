@@ -284,11 +284,26 @@ struct PairSet {
   std::vector<int> off, np;
   std::vector<double> geo;
   std::unordered_map<uint64_t, int> index;
+  std::vector<int> dense;  // (va, vb) -> id when nv^2 is small enough
+  int nv = 0;
+
+  void init(const HostBasis& B) {
+    nv = (int)B.v.size();
+    if ((int64_t)nv * (nv + 1) <= ((int64_t)1 << 26)) dense.assign((size_t)nv * (nv + 1), -1);
+  }
+  int* slot(int va, int vb) {  // vb == nv: pseudo pair of va
+    if (dense.empty()) return nullptr;
+    return &dense[(size_t)va * (nv + 1) + vb];
+  }
 
   int add(const HostBasis& B, int va, int vb) {
+    int* sl = slot(va, vb);
+    if (sl && *sl >= 0) return *sl;
     uint64_t key = ((uint64_t)(uint32_t)va << 32) | (uint32_t)vb;
-    auto it = index.find(key);
-    if (it != index.end()) return it->second;
+    if (!sl) {
+      auto it = index.find(key);
+      if (it != index.end()) return it->second;
+    }
     const VShell& A = B.v[va];
     const VShell& Bs = B.v[vb];
     int id = (int)off.size();
@@ -318,15 +333,22 @@ struct PairSet {
     for (int d = 0; d < 3; ++d) geo.push_back(Bs.r[d]);
     geo.push_back(std::log(std::sqrt(rr) + 1.0));
     geo.push_back(0.0);
-    index.emplace(key, id);
+    if (sl)
+      *sl = id;
+    else
+      index.emplace(key, id);
     return id;
   }
 
   // int3c2e ket: shell k alone (the l slot of g3c2e.c:68-69 carries k, al=0).
   int add_pseudo(const HostBasis& B, int vk) {
+    int* sl = slot(vk, nv);
+    if (sl && *sl >= 0) return *sl;
     uint64_t key = ((uint64_t)0xffffffffu << 32) | (uint32_t)vk;
-    auto it = index.find(key);
-    if (it != index.end()) return it->second;
+    if (!sl) {
+      auto it = index.find(key);
+      if (it != index.end()) return it->second;
+    }
     const VShell& K = B.v[vk];
     int id = (int)off.size();
     off.push_back((int)aij.size());
@@ -345,7 +367,10 @@ struct PairSet {
     for (int d = 0; d < 3; ++d) geo.push_back(K.r[d]);
     geo.push_back(0.0);
     geo.push_back(0.0);
-    index.emplace(key, id);
+    if (sl)
+      *sl = id;
+    else
+      index.emplace(key, id);
     return id;
   }
 };
@@ -360,9 +385,8 @@ struct Bin {
   std::vector<int2> q;
   int64_t out_base = 0;  // element offset of the bin's first block
   int64_t block = 0;     // elements per block (NS * ncomp)
-  // device pointers (inside the plan blob)
+  // device pointer (inside the plan blob)
   int2* d_q = nullptr;
-  int64_t* d_off = nullptr;
 };
 
 struct VQ {  // one virtual (segmented) tuple of a caller tuple
@@ -559,11 +583,8 @@ static void plan_upload(cintx_plan* P) {
   size_t o_aij = take(npp * 8), o_px = take(npp * 8), o_py = take(npp * 8), o_pz = take(npp * 8),
          o_k = take(npp * 8), o_ai = take(npp * 8), o_cce = take(npp * 8), o_h = take(npp * 8);
   size_t o_poff = take(npair * 4), o_pnp = take(npair * 4), o_geo = take(npair * 64);
-  std::vector<size_t> o_q(P->bins.size()), o_off(P->bins.size());
-  for (size_t i = 0; i < P->bins.size(); ++i) {
-    o_q[i] = take(P->bins[i].q.size() * sizeof(int2));
-    o_off[i] = take(P->bins[i].q.size() * 8);
-  }
+  std::vector<size_t> o_q(P->bins.size());
+  for (size_t i = 0; i < P->bins.size(); ++i) o_q[i] = take(P->bins[i].q.size() * sizeof(int2));
   size_t o_cnt = take(P->bins.size() * 2 * 8 + 8);
   int64_t scratch = 0;
   for (auto& b : P->bins)
@@ -594,11 +615,7 @@ static void plan_upload(cintx_plan* P) {
   for (size_t i = 0; i < P->bins.size(); ++i) {
     Bin& b = P->bins[i];
     put(o_q[i], b.q.data(), b.q.size() * sizeof(int2));
-    std::vector<int64_t> offs(b.q.size());
-    for (size_t k = 0; k < b.q.size(); ++k) offs[k] = b.out_base + (int64_t)k * b.block;
-    put(o_off[i], offs.data(), offs.size() * 8);
     b.d_q = (int2*)(P->blob + o_q[i]);
-    b.d_off = (int64_t*)(P->blob + o_off[i]);
   }
   HIPCHK(hipMemcpyAsync(P->blob, h.data(), host_bytes, hipMemcpyHostToDevice, C.stream));
   HIPCHK(hipStreamSynchronize(C.stream));
@@ -670,7 +687,8 @@ static void plan_launch_all(cintx_plan* P, double* out, bool count) {
     const int64_t chunk = (int64_t)1 << 28;
     for (int64_t s = 0; s < (int64_t)b.q.size(); s += chunk) {
       a.quartets = b.d_q + s;
-      a.out_off = b.d_off + s;
+      a.out_base = b.out_base + s * b.block;
+      a.block = b.block;
       a.nq = std::min(chunk, (int64_t)b.q.size() - s);
       hipError_t e;
       if (b.generic) {
@@ -693,6 +711,7 @@ static cintx_plan* plan_new(int family, const int* atm, int natm, const int* bas
   P->family = family;
   try {
     P->basis = ingest(atm, natm, bas, nbas, env, env_len);
+    P->pairs.init(P->basis);
     P->expcutoff = fam_expcutoff(family, env);
   } catch (...) {
     delete P;
