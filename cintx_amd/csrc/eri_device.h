@@ -235,9 +235,17 @@ struct ClassShape {
   static constexpr int NE = NF * NCOMP;
   static constexpr int T = NE <= 8 ? 1 : (NE <= 48 ? 4 : (NE <= 192 ? 16 : (NE <= 768 ? 64 : 256)));
   static constexpr int NFT = (NF + T - 1) / T;
+  // primitive tuples in flight per step (wide tuples): enough to keep the
+  // lanes of the root and recurrence phases busy
+  static constexpr int NSC = 16;  // per-primitive scalars shared in LDS
+  // (LDS per primitive slot is sized for FP64 so both precisions share P)
+  static constexpr int PLDS = (60 * 1024 - 2 * NF * 8) / (3 * G4 * NR * 8 + (3 * NR + NSC) * 8);
+  static constexpr int P = T == 1 ? 1 : cmax(1, cmin(cmin(8, T / (3 * NR)), PLDS));
   template <typename R>
   static constexpr int lds_per_tuple() {
-    return (int)(3 * G4 * NR * sizeof(R) + 3 * NR * sizeof(double) + 2 * NF * sizeof(double));
+    return T == 1 ? (int)(2 * NF * sizeof(double))
+                  : (int)(P * (3 * G4 * NR * sizeof(R) + (3 * NR + NSC) * sizeof(double)) +
+                          2 * NF * sizeof(double));
   }
   template <typename R>
   static constexpr int Q() {
@@ -270,8 +278,6 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
   // compile-time constant once the loops are unrolled); wider tuples share
   // them through LDS.
   constexpr bool REG = (T == 1);
-  __shared__ R sg[Q][REG ? 1 : GS];
-  __shared__ double suw[Q][REG ? 1 : 3 * NR];
   __shared__ double sbuf[Q][2][NF];
   R greg[REG ? GS : 1];
   double uwreg[REG ? 3 * NR : 1];
@@ -322,9 +328,6 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
   for (int kk = 0; kk < NFT * NCOMP; ++kk) acc[kk] = 0.0;
   unsigned int nsurv = 0;
 
-  R* __restrict__ g = REG ? greg : sg[slot];
-  double* __restrict__ uw = REG ? uwreg : suw[slot];
-
   // Primitive counts are per tuple; a workgroup of several tuples loops to
   // its largest counts (tuples are sorted by (npij, npkl) so that is rarely
   // more) and masks the excess iterations.
@@ -344,6 +347,10 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
       mkl = smax[1];
     }
   }
+  if constexpr (REG) {
+  R* __restrict__ g = greg;
+  double* __restrict__ uw = uwreg;
+
   for (int kp = 0; kp < mkl; ++kp) {
     const bool kin = kp < np_kl;
     const int kidx = koff + (kin ? kp : 0);
@@ -467,6 +474,142 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
             acc[2 * NFT + kk] += (double)sz;
           }
         }
+      }
+    }
+  }
+  } else {
+    // Wide tuples: P primitive tuples per step.  Phase 1 (lanes < P*NR):
+    // screening, per-primitive scalars and one Rys root each; phase 2 (lanes
+    // < P*3*NR): one recurrence chain each; phase 3: every lane accumulates
+    // its elements over the P*NR quadrature points.
+    constexpr int P = S::P;
+    constexpr int NSC = S::NSC;
+    __shared__ R sgb[Q][P][GS];
+    __shared__ double suwb[Q][P][3 * NR];
+    __shared__ double sps[Q][P][NSC];
+    const int npq = mij * mkl;
+    for (int base = 0; base < npq; base += P) {
+      bool any = false;
+      if (t < P * NR) {
+        const int pp = t / NR, r = t - pp * NR;
+        const int k = base + pp;
+        const int kp = k / mij, ip = k - kp * mij;
+        const bool kin = k < npq && kp < np_kl && ip < np_ij;
+        const int kidx = koff + (kin ? kp : 0), iidx = boff + (kin ? ip : 0);
+        const double ccekl = a.three_center ? 0.0 : a.cce[kidx] - a.lkl_ceil * logd_kl;
+        const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
+        const bool ok = active && kin && ccekl <= a.expcutoff && cceij <= a.expcutoff - ccekl;
+        const double aij = a.aij[iidx], akl = a.aij[kidx];
+        const double Pv[3] = {a.px[iidx], a.py[iidx], a.pz[iidx]};
+        const double Qv[3] = {a.px[kidx], a.py[kidx], a.pz[kidx]};
+        double dx[3], rr = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          dx[d] = Pv[d] - Qv[d];
+          rr += dx[d] * dx[d];
+        }
+        const double inv_sum = 1.0 / (aij + akl);
+        const double hij = a.hinv[iidx], hkl = a.hinv[kidx];
+        const double fac = ok ? a.common * a.kfac[iidx] * a.kfac[kidx] : 0.0;
+        const double fac1 = 4.0 * hij * hkl * sqrt(inv_sum) * fac;
+        const double x = aij * akl * inv_sum * rr;
+        double sr, qr, w;
+        rys_eval(a.rys, NR, r, x, sr, qr, w);
+        suwb[slot][pp][r] = sr;
+        suwb[slot][pp][NR + r] = w * fac1;
+        suwb[slot][pp][2 * NR + r] = qr;
+        if (r == 0) {
+          double* sc = sps[slot][pp];
+          sc[0] = inv_sum;
+          sc[1] = hij;
+          sc[2] = hkl;
+          sc[3] = aij;
+          sc[4] = akl;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            sc[5 + d] = dx[d];
+            sc[8 + d] = Pv[d] - (IBASE ? ri[d] : rj[d]);
+            sc[11 + d] = Qv[d] - (KBASE ? rk[d] : rl[d]);
+          }
+          sc[14] = a.ai[iidx];
+          if (ok) ++nsurv;
+        }
+        any = ok;
+      }
+      if (!__syncthreads_or(any)) continue;
+      if (t < P * 3 * NR) {
+        const int pp = t / (3 * NR), c = t - pp * 3 * NR;
+        const int axis = c / NR, r = c - axis * NR;
+        const double* sc = sps[slot][pp];
+        const double* uwp = suwb[slot][pp];
+        R* gp = sgb[slot][pp];
+        if (axis == 0)
+          build_chain<LIc, LJ, LK, LL, R>(gp, 0, r, uwp[r], uwp[2 * NR + r], uwp[NR + r], sc[0],
+                                          sc[1], sc[2], sc[3], sc[4], sc[5], sc[8], sc[11], 1.0,
+                                          rij[0], rkl[0]);
+        else if (axis == 1)
+          build_chain<LIc, LJ, LK, LL, R>(gp, 1, r, uwp[r], uwp[2 * NR + r], uwp[NR + r], sc[0],
+                                          sc[1], sc[2], sc[3], sc[4], sc[6], sc[9], sc[12], 1.0,
+                                          rij[1], rkl[1]);
+        else
+          build_chain<LIc, LJ, LK, LL, R>(gp, 2, r, uwp[r], uwp[2 * NR + r], uwp[NR + r], sc[0],
+                                          sc[1], sc[2], sc[3], sc[4], sc[7], sc[10], sc[13], 0.0,
+                                          rij[2], rkl[2]);
+      }
+      __syncthreads();
+      if constexpr (!DERIV) {
+#pragma unroll
+        for (int kk = 0; kk < NFT; ++kk) {
+          if (kk * T + t < NF) {
+            R sum = 0;
+#pragma unroll 1
+            for (int pp = 0; pp < P; ++pp) {
+              const R* gpp = sgb[slot][pp];
+              const R* gx = gpp + (pos[kk] & 1023) * NR;
+              const R* gy = gpp + (G4 + ((pos[kk] >> 10) & 1023)) * NR;
+              const R* gz = gpp + (2 * G4 + (pos[kk] >> 20)) * NR;
+#pragma unroll
+              for (int r = 0; r < NR; ++r) sum += gx[r] * gy[r] * gz[r];
+            }
+            acc[kk] += (double)sum;
+          }
+        }
+      } else {
+        constexpr int SI = DJ * DK * DL;
+#pragma unroll
+        for (int kk = 0; kk < NFT; ++kk) {
+          const int e = kk * T + t;
+          if (e < NF) {
+            const int px = pos[kk] & 1023, py = (pos[kk] >> 10) & 1023, pz = pos[kk] >> 20;
+            int ix, iy, iz;
+            cart_decode(LI, e % NFI, ix, iy, iz);
+            R sx = 0, sy = 0, sz = 0;
+#pragma unroll 1
+            for (int pp = 0; pp < P; ++pp) {
+              const R* gpp = sgb[slot][pp];
+              const R ai2 = (R)(-2.0 * sps[slot][pp][14]);
+              const R* gx = gpp + px * NR;
+              const R* gy = gpp + (G4 + py) * NR;
+              const R* gz = gpp + (2 * G4 + pz) * NR;
+#pragma unroll
+              for (int r = 0; r < NR; ++r) {
+                R fx = ai2 * gx[SI * NR + r];
+                R fy = ai2 * gy[SI * NR + r];
+                R fz = ai2 * gz[SI * NR + r];
+                if (ix > 0) fx += (R)ix * gx[r - SI * NR];
+                if (iy > 0) fy += (R)iy * gy[r - SI * NR];
+                if (iz > 0) fz += (R)iz * gz[r - SI * NR];
+                sx += fx * gy[r] * gz[r];
+                sy += gx[r] * fy * gz[r];
+                sz += gx[r] * gy[r] * fz;
+              }
+            }
+            acc[kk] += (double)sx;
+            acc[NFT + kk] += (double)sy;
+            acc[2 * NFT + kk] += (double)sz;
+          }
+        }
+        __syncthreads();  // phase 1 of the next step rewrites a_i in sps
       }
     }
   }
