@@ -31,6 +31,15 @@ hipError_t launch_rys_eval(const RysTable& tab, int n, const double* x, int64_t 
                            double* w, hipStream_t s);
 hipError_t launch_generic(const EriArgs& a, const int* l, int deriv, int grid, hipStream_t s);
 int64_t generic_scratch_per_block(const int* l, int deriv);
+hipError_t mlp_train_device(const RysTable& tab, float* prm, int64_t nsamples, int epochs,
+                            int batch, float lr, uint64_t seed, double* final_loss,
+                            hipStream_t st);
+hipError_t mlp_grad_device(const float* prm, const float* z, const int* nr, const float* tgt,
+                           int batch, float* grad, float* loss, hipStream_t st);
+hipError_t mlp_eval_device(const float* prm, const RysTable& tab, int n, const double* x,
+                           int64_t nx, int steps, int mode, double* u, double* w, int* status,
+                           hipStream_t st);
+int mlp_nparam();
 static constexpr int kGenericGrid = 1024;
 
 // ------------------------------------------------------------------
@@ -885,6 +894,16 @@ static int family_of_api(int api) {
   }
 }
 
+// Scoped device allocation.
+template <typename T>
+struct DevArr {
+  T* p = nullptr;
+  explicit DevArr(size_t n) {
+    HIPCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+  }
+  ~DevArr() { (void)hipFree(p); }
+};
+
 }  // namespace cx
 
 using namespace cx;
@@ -1495,6 +1514,76 @@ int cintx_rys_eval_device(int n, const double* x, int64_t nx, double* u, double*
     HIPCHK(hipStreamSynchronize(C.stream));
     HIPCHK(hipMemcpy(u, du, (size_t)nx * n * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(w, dw, (size_t)nx * n * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+// ------------------------------------------------------------------
+// Learned Rys initializer (rys_mlp.hip)
+// ------------------------------------------------------------------
+int cintx_rys_mlp_nparam(void) { return mlp_nparam(); }
+
+int cintx_rys_mlp_train(float* params, int64_t nsamples, int epochs, int batch, float lr,
+                        uint64_t seed, double* final_loss) {
+  return guarded([&] {
+    if (!params) fail(CINTX_STATUS_NULL_POINTER, "params is null");
+    if (nsamples <= 0 || epochs <= 0 || batch <= 0)
+      fail(CINTX_STATUS_INVALID_INPUT, "nsamples, epochs and batch must be positive");
+    DevCtx& C = ctx();
+    const int np = mlp_nparam();
+    DevArr<float> d(np);
+    HIPCHK(hipMemcpy(d.p, params, np * 4, hipMemcpyHostToDevice));
+    HIPCHK(mlp_train_device(C.rys, d.p, nsamples, epochs, batch, lr, seed, final_loss, C.stream));
+    HIPCHK(hipMemcpy(params, d.p, np * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int cintx_rys_mlp_grad(const float* params, const float* z, const int* nroots,
+                       const float* targets, int batch, float* grad, float* loss) {
+  return guarded([&] {
+    if (!params || !z || !nroots || !targets || !grad || !loss)
+      fail(CINTX_STATUS_NULL_POINTER, "null argument");
+    DevCtx& C = ctx();
+    const int np = mlp_nparam();
+    DevArr<float> dp(np), dz((size_t)batch * 2), dt((size_t)batch * 26), dg(np), dl(batch);
+    DevArr<int> dn(batch);
+    HIPCHK(hipMemcpy(dp.p, params, np * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dz.p, z, (size_t)batch * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dn.p, nroots, (size_t)batch * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dt.p, targets, (size_t)batch * 26 * 4, hipMemcpyHostToDevice));
+    HIPCHK(mlp_grad_device(dp.p, dz.p, dn.p, dt.p, batch, dg.p, dl.p, C.stream));
+    HIPCHK(hipMemcpy(grad, dg.p, np * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(loss, dl.p, (size_t)batch * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int cintx_rys_mlp_eval(const float* params, int n, const double* x, int64_t nx, int newton_steps,
+                       int init_mode, double* u, double* w, int* status, double* ms) {
+  return guarded([&] {
+    if (n < 1 || n > 13) fail(CINTX_STATUS_INVALID_INPUT, "nroots must be in 1..13");
+    if (!params || !x || !u || !w || !status) fail(CINTX_STATUS_NULL_POINTER, "null argument");
+    DevCtx& C = ctx();
+    const int np = mlp_nparam();
+    DevArr<float> dp(np);
+    DevArr<double> dx(nx), du((size_t)nx * n), dw((size_t)nx * n);
+    DevArr<int> ds(nx);
+    HIPCHK(hipMemcpy(dp.p, params, np * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dx.p, x, (size_t)nx * 8, hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, C.stream));
+    HIPCHK(mlp_eval_device(dp.p, C.rys, n, dx.p, nx, newton_steps, init_mode, du.p, dw.p, ds.p,
+                           C.stream));
+    HIPCHK(hipEventRecord(e1, C.stream));
+    HIPCHK(hipStreamSynchronize(C.stream));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipMemcpy(u, du.p, (size_t)nx * n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(w, dw.p, (size_t)nx * n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(status, ds.p, (size_t)nx * 4, hipMemcpyDeviceToHost));
   });
 }
 

@@ -178,6 +178,25 @@ int cintx_plan_work(cintx_plan *plan, double *prim_tuples, double *root_evals, d
 int cintx_rys_eval_host(int n, const double *x, int64_t nx, double *u, double *w);
 int cintx_rys_eval_device(int n, const double *x, int64_t nx, double *u, double *w);
 
+/* Learned Rys initializer (no counterpart in the reference): a 2x64 tanh
+ * MLP mapping (log x, nroots) to initial roots/weights, trained on the device
+ * by this library's forward/backward/Adam kernels on rules produced by the
+ * table solver, polished by FP64 Newton steps on the moment equations, with
+ * the table rule as fallback.  params: cintx_rys_mlp_nparam() floats. */
+int cintx_rys_mlp_nparam(void);
+int cintx_rys_mlp_train(float *params, int64_t nsamples, int epochs, int batch, float lr,
+                        uint64_t seed, double *final_loss);
+/* batch-mean loss gradient for given inputs z (batch x 2), nroots and
+ * targets (batch x 26: ln u_r then ln w_r); loss per sample */
+int cintx_rys_mlp_grad(const float *params, const float *z, const int *nroots,
+                       const float *targets, int batch, float *grad, float *loss);
+/* init_mode 0: network, 1: asymptotic rule (the initializer without the
+ * network).  status[i] = 1 when Newton settled below 1e-14 relative, 0 when
+ * the table rule was substituted.  ms: device time of the evaluation. */
+int cintx_rys_mlp_eval(const float *params, int n, const double *x, int64_t nx,
+                       int newton_steps, int init_mode, double *u, double *w, int *status,
+                       double *ms);
+
 /* Device memory helpers for callers without a runtime of their own. */
 int cintx_device_alloc(double **ptr, int64_t nelem);
 int cintx_device_free(double *ptr);

@@ -1,0 +1,69 @@
+"""Learned Rys initializer: gradient checks (CPU back-propagation against
+finite differences, device kernels against the CPU back-propagation),
+training on the device, and what the network gains over Newton started from
+the same rule without it (the asymptotic Hermite rule)."""
+import numpy as np
+import pytest
+
+from cintx_amd import rys_mlp as R
+
+
+def batch(seed, B=96):
+    rng = np.random.default_rng(seed)
+    x = np.exp(rng.uniform(np.log(1e-6), np.log(1e3), B))
+    n = rng.integers(1, 14, B)
+    z = R.features(x, n)
+    tgt = rng.normal(size=(B, R.OUT)) * R.mask(n)
+    return z, n, tgt
+
+
+def test_backprop_matches_finite_differences():
+    p = R.init_params(3).astype(np.float64)
+    z, n, tgt = batch(1)
+    g = R.grad_np(p, z, n, tgt)
+    rng = np.random.default_rng(0)
+    idx = np.concatenate([rng.choice(np.arange(s, e), 6, replace=False) for s, e in
+                          [(R.OW1, R.OB1), (R.OB1, R.OW2), (R.OW2, R.OB2), (R.OB2, R.OW3),
+                           (R.OW3, R.OB3), (R.OB3, R.NPARAM)]])
+    h = 1e-6
+    for k in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (R.loss_np(pp, z, n, tgt).mean() - R.loss_np(pm, z, n, tgt).mean()) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-7 + 1e-6 * abs(g[k]), (k, fd, g[k])
+
+
+@pytest.mark.gpu
+def test_device_gradient_matches_cpu_backprop():
+    p = R.init_params(5)
+    z, n, tgt = batch(2, B=1000)
+    g_dev, loss_dev = R.device_grad(p, z, n, tgt)
+    g_ref = R.grad_np(p.astype(np.float64), z.astype(np.float32).astype(np.float64), n,
+                      tgt.astype(np.float32).astype(np.float64))
+    loss_ref = R.loss_np(p.astype(np.float64), z.astype(np.float32).astype(np.float64), n,
+                         tgt.astype(np.float32).astype(np.float64))
+    assert np.allclose(loss_dev, loss_ref, rtol=2e-5, atol=1e-6)
+    scale = np.abs(g_ref).max()
+    assert np.max(np.abs(g_dev - g_ref)) <= 2e-5 * scale
+
+
+@pytest.mark.gpu
+def test_trained_network_gains_over_newton_without_it(oracle):
+    p, loss = R.train(nsamples=1 << 18, epochs=6, batch=4096, lr=3e-3)
+    assert loss < 0.05
+    rng = np.random.default_rng(4)
+    x = np.exp(rng.uniform(np.log(1e-3), np.log(60.0), 400))
+    report = {}
+    for n in (1, 2, 3, 4, 5):
+        u, w, st, _ = R.evaluate(p, n, x, newton_steps=2, init_mode=0)
+        u0, w0, st0, _ = R.evaluate(p, n, x, newton_steps=2, init_mode=1)
+        report[n] = (st.mean(), st0.mean())
+        # whatever the path, the returned rule is the oracle's to 1e-12
+        for k in range(0, len(x), 37):
+            uo, wo = oracle.rys_roots(n, x[k])
+            assert np.max(np.abs(u[k] - uo) / uo) < 1e-12
+            assert np.max(np.abs(w[k] - wo) / wo) < 1e-12
+    print("settled fraction (network, asymptotic init):", report)
+    # the network's initial rule lets two Newton steps settle more often
+    assert sum(v[0] for v in report.values()) > sum(v[1] for v in report.values())
