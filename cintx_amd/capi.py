@@ -48,7 +48,7 @@ EXPORTS = [
     "cintx_plan_create", "cintx_plan_create_screened", "cintx_plan_destroy",
     "cintx_plan_sizes", "cintx_plan_info", "cintx_plan_tuples", "cintx_plan_execute", "cintx_plan_sync",
     "cintx_plan_copy_output", "cintx_plan_copy_tuples", "cintx_plan_execute_graph", "cintx_plan_last_time_ms",
-    "cintx_plan_work", "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
+    "cintx_plan_work", "cintx_plan_execute_timed", "cintx_plan_bin_stats", "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
 ]
 
 
@@ -113,6 +113,8 @@ def lib():
         L.cintx_plan_copy_tuples.argtypes = [vp, I64P, ctypes.c_int64, DP, I64P]
         L.cintx_plan_last_time_ms.argtypes = [vp, DP]
         L.cintx_plan_work.argtypes = [vp, DP, DP, DP]
+        L.cintx_plan_execute_timed.argtypes = [vp, DP]
+        L.cintx_plan_bin_stats.argtypes = [vp, ctypes.c_int64, IP, I64P, DP, DP, DP]
         L.cintx_device_count.argtypes = [IP]
         _lib = L
     return _lib
@@ -269,6 +271,24 @@ class Plan:
         v = ctypes.c_double()
         _check(lib().cintx_plan_last_time_ms(self.h, ctypes.byref(v)))
         return v.value
+
+    def execute_timed(self):
+        """run every bin with HIP events around it (synchronous)"""
+        _check(lib().cintx_plan_execute_timed(self.h, None))
+
+    def bin_stats(self):
+        n = self.nbins
+        cls = np.zeros((n, 6), dtype=np.int32)
+        nt = np.zeros(n, dtype=np.int64)
+        ms = np.zeros(n)
+        pt = np.zeros(n)
+        fl = np.zeros(n)
+        _check(lib().cintx_plan_bin_stats(self.h, n, _ip(cls), nt.ctypes.data_as(I64P), _dp(ms),
+                                          _dp(pt), _dp(fl)))
+        return [{"class": tuple(int(v) for v in c[:4]), "precision": int(c[4]),
+                 "generic": bool(c[5]), "tuples": int(t), "ms": float(m),
+                 "prim_tuples": float(p), "flops": float(f)}
+                for c, t, m, p, f in zip(cls, nt, ms, pt, fl)]
 
     def output(self):
         out = np.zeros(self.nelem)

@@ -390,6 +390,9 @@ struct PairSet {
 struct Bin {
   int l[4];
   int prec;
+  double last_ms = 0;           // last timed execute
+  double prim_tuples = 0;       // from the last work count
+  double flops = 0;
   bool generic = false;  // no compile-time instantiation: runtime-class kernel
   std::vector<int2> q;
   int64_t out_base = 0;  // element offset of the bin's first block
@@ -655,7 +658,8 @@ static void plan_check_kernels(cintx_plan* P) {
   }
 }
 
-static void plan_launch_all(cintx_plan* P, double* out, bool count) {
+static void plan_launch_all(cintx_plan* P, double* out, bool count,
+                            std::vector<hipEvent_t>* evs = nullptr) {
   DevCtx& C = ctx();
   const FamilyInfo& f = kFam[P->family];
   double common_base = (M_PI * M_PI * M_PI) * 2.0 / std::sqrt(M_PI);
@@ -692,6 +696,7 @@ static void plan_launch_all(cintx_plan* P, double* out, bool count) {
     a.rys = C.rys;
     a.c2s = C.d_c2s;
     a.counters = count ? P->d_counters + 2 * i : nullptr;
+    if (evs) HIPCHK(hipEventRecord((*evs)[2 * i], C.stream));
     // grid dimension limit: launch in chunks of at most 2^28 tuples
     const int64_t chunk = (int64_t)1 << 28;
     for (int64_t s = 0; s < (int64_t)b.q.size(); s += chunk) {
@@ -710,6 +715,7 @@ static void plan_launch_all(cintx_plan* P, double* out, bool count) {
       if (e != hipSuccess)
         fail(CINTX_STATUS_EXECUTION_FAILED, std::string("kernel launch: ") + hipGetErrorString(e));
     }
+    if (evs) HIPCHK(hipEventRecord((*evs)[2 * i + 1], C.stream));
   }
 }
 
@@ -1295,6 +1301,52 @@ int cintx_plan_info(const cintx_plan* plan, int64_t* ntuples_fp32, int64_t* nbin
   });
 }
 
+int cintx_plan_execute_timed(cintx_plan* plan, double* out_dev) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    DevCtx& C = ctx();
+    std::vector<hipEvent_t> evs(2 * plan->bins.size());
+    for (auto& e : evs) HIPCHK(hipEventCreate(&e));
+    struct G {
+      std::vector<hipEvent_t>& v;
+      ~G() {
+        for (auto e : v) (void)hipEventDestroy(e);
+      }
+    } g{evs};
+    HIPCHK(hipEventRecord(plan->ev0, C.stream));
+    plan_launch_all(plan, out_dev ? out_dev : plan->d_out, false, &evs);
+    HIPCHK(hipEventRecord(plan->ev1, C.stream));
+    HIPCHK(hipStreamSynchronize(C.stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, plan->ev0, plan->ev1));
+    plan->last_ms = ms;
+    for (size_t i = 0; i < plan->bins.size(); ++i) {
+      HIPCHK(hipEventElapsedTime(&ms, evs[2 * i], evs[2 * i + 1]));
+      plan->bins[i].last_ms = ms;
+    }
+  });
+}
+
+int cintx_plan_bin_stats(const cintx_plan* plan, int64_t nmax, int* cls, int64_t* ntuples,
+                         double* ms, double* prim_tuples, double* flops) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    int64_t n = std::min<int64_t>(nmax, (int64_t)plan->bins.size());
+    for (int64_t i = 0; i < n; ++i) {
+      const Bin& b = plan->bins[i];
+      if (cls) {
+        for (int c = 0; c < 4; ++c) cls[6 * i + c] = b.l[c];
+        cls[6 * i + 4] = b.prec;
+        cls[6 * i + 5] = b.generic ? 1 : 0;
+      }
+      if (ntuples) ntuples[i] = (int64_t)b.q.size();
+      if (ms) ms[i] = b.last_ms;
+      if (prim_tuples) prim_tuples[i] = b.prim_tuples;
+      if (flops) flops[i] = b.flops;
+    }
+  });
+}
+
 int cintx_plan_last_time_ms(const cintx_plan* plan, double* ms) {
   return guarded([&] {
     if (!plan || !ms) fail(CINTX_STATUS_NULL_POINTER, "null argument");
@@ -1320,6 +1372,8 @@ int cintx_plan_work(cintx_plan* plan, double* prim_tuples, double* root_evals, d
       np += (double)cnt[2 * i];
       nr += (double)cnt[2 * i] * r;
       fl += (double)cnt[2 * i] * per + (double)cnt[2 * i + 1] * pt;
+      plan->bins[i].prim_tuples = (double)cnt[2 * i];
+      plan->bins[i].flops = (double)cnt[2 * i] * per + (double)cnt[2 * i + 1] * pt;
     }
     if (prim_tuples) *prim_tuples = np;
     if (root_evals) *root_evals = nr;

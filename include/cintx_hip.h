@@ -170,6 +170,14 @@ int cintx_plan_execute_graph(cintx_plan *plan, double *out_dev);
  * per-kernel-launch records, and the algorithmic work of the plan
  * (surviving primitive tuples, root evaluations, FLOPs). */
 int cintx_plan_last_time_ms(const cintx_plan *plan, double *ms);
+/* Execute with HIP events around every class bin (no graph), synchronously;
+ * then cintx_plan_bin_stats reports per bin: class (li,lj,lk,ll, precision,
+ * runtime-class kernel flag; 6 ints), tuples, milliseconds of the last timed
+ * execute, and the surviving primitive tuples / algorithmic FLOPs of the last
+ * cintx_plan_work. */
+int cintx_plan_execute_timed(cintx_plan *plan, double *out_dev);
+int cintx_plan_bin_stats(const cintx_plan *plan, int64_t nmax, int *cls, int64_t *ntuples,
+                         double *ms, double *prim_tuples, double *flops);
 int cintx_plan_work(cintx_plan *plan, double *prim_tuples, double *root_evals, double *flops);
 
 /* The n-point Rys rule (roots u = t^2/(1-t^2), weights) that the kernels use,
