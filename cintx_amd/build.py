@@ -115,7 +115,7 @@ def build(jobs=None, verbose=True):
     gen = generate()
     srcs = [os.path.join(CSRC, "runtime.cpp"), os.path.join(CSRC, "rys_table.cpp"),
             os.path.join(CSRC, "rys_kernels.hip"), os.path.join(CSRC, "eri_generic.hip"),
-            os.path.join(CSRC, "rys_mlp.hip")] + gen
+            os.path.join(CSRC, "rys_mlp.hip"), os.path.join(CSRC, "jk_digest.hip")] + gen
     extra = {os.path.join(CSRC, "rys_table.cpp"): ["-fopenmp"]}
     jobs = jobs or min(16, os.cpu_count() or 4)
     objs = []

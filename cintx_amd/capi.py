@@ -48,7 +48,7 @@ EXPORTS = [
     "cintx_plan_create", "cintx_plan_create_screened", "cintx_plan_destroy",
     "cintx_plan_sizes", "cintx_plan_info", "cintx_plan_tuples", "cintx_plan_execute", "cintx_plan_sync",
     "cintx_plan_copy_output", "cintx_plan_copy_tuples", "cintx_plan_execute_graph", "cintx_plan_last_time_ms",
-    "cintx_plan_work", "cintx_plan_execute_timed", "cintx_plan_bin_stats", "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
+    "cintx_plan_work", "cintx_plan_execute_timed", "cintx_plan_jk", "cintx_plan_bin_stats", "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
 ]
 
 
@@ -114,6 +114,7 @@ def lib():
         L.cintx_plan_last_time_ms.argtypes = [vp, DP]
         L.cintx_plan_work.argtypes = [vp, DP, DP, DP]
         L.cintx_plan_execute_timed.argtypes = [vp, DP]
+        L.cintx_plan_jk.argtypes = [vp, DP, DP, DP]
         L.cintx_plan_bin_stats.argtypes = [vp, ctypes.c_int64, IP, I64P, DP, DP, DP]
         L.cintx_device_count.argtypes = [IP]
         _lib = L
@@ -289,6 +290,14 @@ class Plan:
                  "generic": bool(c[5]), "tuples": int(t), "ms": float(m),
                  "prim_tuples": float(p), "flops": float(f)}
                 for c, t, m, p, f in zip(cls, nt, ms, pt, fl)]
+
+    def jk(self, D):
+        """J and K against the symmetric density D (unique-quartet plans)."""
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        J = np.zeros_like(D)
+        K = np.zeros_like(D)
+        _check(lib().cintx_plan_jk(self.h, _dp(D), _dp(J), _dp(K)))
+        return J, K
 
     def output(self):
         out = np.zeros(self.nelem)

@@ -40,6 +40,9 @@ hipError_t mlp_eval_device(const float* prm, const RysTable& tab, int n, const d
                            int64_t nx, int steps, int mode, double* u, double* w, int* status,
                            hipStream_t st);
 int mlp_nparam();
+hipError_t launch_jk_digest(const double* eri, int64_t out_base, int64_t block, const int2* q,
+                            int64_t nq, const int2* pair_sh, const int* ao_off, const int* shl_l,
+                            const double* D, double* J, double* K, int nao, hipStream_t s);
 static constexpr int kGenericGrid = 1024;
 
 // ------------------------------------------------------------------
@@ -291,6 +294,7 @@ static HostBasis ingest(const int* atm, int natm, const int* bas, int nbas, cons
 struct PairSet {
   std::vector<double> aij, px, py, pz, kfac, ai, cce, hinv;
   std::vector<int> off, np;
+  std::vector<int2> shells;  // original shells (a, b) of each pair (b = -1: pseudo)
   std::vector<double> geo;
   std::unordered_map<uint64_t, int> index;
   std::vector<int> dense;  // (va, vb) -> id when nv^2 is small enough
@@ -318,6 +322,7 @@ struct PairSet {
     int id = (int)off.size();
     off.push_back((int)aij.size());
     np.push_back(A.nprim * Bs.nprim);
+    shells.push_back(make_int2(A.orig, Bs.orig));
     double rr = 0;
     for (int d = 0; d < 3; ++d) rr += (A.r[d] - Bs.r[d]) * (A.r[d] - Bs.r[d]);
     double aij_last = B.exps[A.off + A.nprim - 1] + B.exps[Bs.off + Bs.nprim - 1];
@@ -362,6 +367,7 @@ struct PairSet {
     int id = (int)off.size();
     off.push_back((int)aij.size());
     np.push_back(K.nprim);
+    shells.push_back(make_int2(K.orig, -1));
     for (int kp = 0; kp < K.nprim; ++kp) {
       aij.push_back(B.exps[K.off + kp]);
       hinv.push_back(0.5 / B.exps[K.off + kp]);
@@ -1638,6 +1644,44 @@ int cintx_rys_mlp_eval(const float* params, int n, const double* x, int64_t nx, 
     HIPCHK(hipMemcpy(u, du.p, (size_t)nx * n * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(w, dw.p, (size_t)nx * n * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(status, ds.p, (size_t)nx * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+// J/K build from an int2e_sph plan over unique (8-fold) quartets of
+// segmented shells; D, J, K are nao x nao row-major on the host.
+int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    if (!D || !J || !K) fail(CINTX_STATUS_NULL_POINTER, "null argument");
+    if (plan->family != 0) fail(CINTX_STATUS_UNSUPPORTED_API, "J/K needs an int2e_sph plan");
+    const HostBasis& B = plan->basis;
+    for (int sh = 0; sh < B.nbas; ++sh)
+      if (B.nctr[sh] != 1) fail(CINTX_STATUS_UNSUPPORTED_API, "J/K needs segmented shells");
+    DevCtx& C = ctx();
+    std::vector<int> ao_off(B.nbas), shl_l(B.l);
+    int nao = 0;
+    for (int sh = 0; sh < B.nbas; ++sh) {
+      ao_off[sh] = nao;
+      nao += nsph(B.l[sh]);
+    }
+    const size_t nn = (size_t)nao * nao;
+    DevArr<double> dD(nn), dJ(nn), dK(nn);
+    DevArr<int> dao(B.nbas), dl(B.nbas);
+    DevArr<int2> dsh(plan->pairs.shells.size());
+    HIPCHK(hipMemcpy(dD.p, D, nn * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(dJ.p, 0, nn * 8));
+    HIPCHK(hipMemset(dK.p, 0, nn * 8));
+    HIPCHK(hipMemcpy(dao.p, ao_off.data(), B.nbas * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dl.p, shl_l.data(), B.nbas * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dsh.p, plan->pairs.shells.data(), plan->pairs.shells.size() * sizeof(int2),
+                     hipMemcpyHostToDevice));
+    plan_launch_all(plan, plan->d_out, false);
+    for (const Bin& b : plan->bins)
+      HIPCHK(launch_jk_digest(plan->d_out, b.out_base, b.block, b.d_q, (int64_t)b.q.size(), dsh.p,
+                              dao.p, dl.p, dD.p, dJ.p, dK.p, nao, C.stream));
+    HIPCHK(hipStreamSynchronize(C.stream));
+    HIPCHK(hipMemcpy(J, dJ.p, nn * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(K, dK.p, nn * 8, hipMemcpyDeviceToHost));
   });
 }
 

@@ -186,6 +186,12 @@ int cintx_plan_work(cintx_plan *plan, double *prim_tuples, double *root_evals, d
 int cintx_rys_eval_host(int n, const double *x, int64_t nx, double *u, double *w);
 int cintx_rys_eval_device(int n, const double *x, int64_t nx, double *u, double *w);
 
+/* Coulomb and exchange matrices from an int2e_sph plan whose tuples are
+ * unique 8-fold shell quartets (each unordered quartet once) of segmented
+ * shells: J_pq = sum_rs (pq|rs) D_rs, K_pr = sum_qs (pq|rs) D_qs for a
+ * symmetric density D (nao x nao, row-major, host memory). */
+int cintx_plan_jk(cintx_plan *plan, const double *D, double *J, double *K);
+
 /* Learned Rys initializer (no counterpart in the reference): a 2x64 tanh
  * MLP mapping (log x, nroots) to initial roots/weights, trained on the device
  * by this library's forward/backward/Adam kernels on rules produced by the

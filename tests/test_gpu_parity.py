@@ -212,3 +212,25 @@ def test_general_contraction_all_classes(oracle):
     tuples = [tuple(int(s) for s in rng.integers(0, mol.nbas, 4)) for _ in range(200)]
     check_plan(oracle, "int2e_sph", tuples, mol)
     check_plan(oracle, "int3c2e_sph", [t[:3] for t in tuples[:100]], mol)
+
+
+def test_jk_build_cfg1(oracle):
+    # J/K from the screened unique-quartet plan (screen off) against an
+    # einsum over the oracle's full cfg-1 tensor
+    mol = M.config(1)
+    eri = oracle.full_tensor_int2e(mol.atm, mol.bas, mol.env)
+    nao = eri.shape[0]
+    rng = np.random.default_rng(11)
+    D = rng.normal(size=(nao, nao)) / nao
+    D = D + D.T
+    Jr = np.einsum("pqrs,rs->pq", eri, D)
+    Kr = np.einsum("pqrs,qs->pr", eri, D)
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, screened=(1e-300, 1))
+    n = mol.nbas
+    assert p.ntuples == (n * (n + 1) // 2) * (n * (n + 1) // 2 + 1) // 2
+    J, K = p.jk(D)
+    say(f"  J err {np.abs(J - Jr).max():.3e}  K err {np.abs(K - Kr).max():.3e}  "
+        f"|J| {np.abs(Jr).max():.3e}")
+    assert np.abs(J - Jr).max() <= 1e-11 * max(1.0, np.abs(Jr).max())
+    assert np.abs(K - Kr).max() <= 1e-11 * max(1.0, np.abs(Kr).max())
+    p.close()
