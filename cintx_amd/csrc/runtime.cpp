@@ -40,6 +40,7 @@ hipError_t mlp_eval_device(const float* prm, const RysTable& tab, int n, const d
                            int64_t nx, int steps, int mode, double* u, double* w, int* status,
                            hipStream_t st);
 int mlp_nparam();
+hipError_t launch_copy_probe(const double* src, double* dst, int64_t n, hipStream_t s);
 hipError_t launch_jk_digest(const double* eri, int64_t out_base, int64_t block, const int2* q,
                             int64_t nq, const int2* pair_sh, const int* ao_off, const int* shl_l,
                             const double* D, double* J, double* K, int nao, hipStream_t s);
@@ -1682,6 +1683,28 @@ int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
     HIPCHK(hipStreamSynchronize(C.stream));
     HIPCHK(hipMemcpy(J, dJ.p, nn * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(K, dK.p, nn * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+int cintx_copy_probe(int64_t nelem, int reps, double* ms) {
+  return guarded([&] {
+    if (nelem <= 0 || reps <= 0) fail(CINTX_STATUS_INVALID_INPUT, "nelem and reps must be positive");
+    DevCtx& C = ctx();
+    DevArr<double> a((size_t)nelem), b((size_t)nelem);
+    HIPCHK(hipMemsetAsync(a.p, 0, (size_t)nelem * 8, C.stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(launch_copy_probe(a.p, b.p, nelem, C.stream));  // warm-up
+    HIPCHK(hipEventRecord(e0, C.stream));
+    for (int r = 0; r < reps; ++r) HIPCHK(launch_copy_probe(a.p, b.p, nelem, C.stream));
+    HIPCHK(hipEventRecord(e1, C.stream));
+    HIPCHK(hipStreamSynchronize(C.stream));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   });
 }
 

@@ -28,3 +28,21 @@ hipError_t launch_rys_eval(const RysTable& tab, int n, const double* x, int64_t 
 }
 
 }  // namespace cx
+
+namespace cx {
+
+// Plain streaming copy (one read, one write per element): the calibration
+// reference for hardware byte counters (FETCH_SIZE / WRITE_SIZE).
+__global__ void copy_probe_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                  int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy_probe(const double* src, double* dst, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(2048), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+}  // namespace cx

@@ -211,6 +211,10 @@ int cintx_rys_mlp_eval(const float *params, int n, const double *x, int64_t nx,
                        int newton_steps, int init_mode, double *u, double *w, int *status,
                        double *ms);
 
+/* A plain device copy of nelem doubles, reps times (average ms per copy):
+ * the reference kernel for calibrating hardware byte counters. */
+int cintx_copy_probe(int64_t nelem, int reps, double *ms);
+
 /* Device memory helpers for callers without a runtime of their own. */
 int cintx_device_alloc(double **ptr, int64_t nelem);
 int cintx_device_free(double *ptr);
