@@ -78,6 +78,17 @@ __device__ __forceinline__ void rys_eval(const RysTable& tab, int n, int r, doub
   w = fma(t, bw1, cw[0] - bw2);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; remap
+// so that XCD x runs a contiguous range of block indices (tuples that are
+// adjacent in a bin share bra pairs, whose primitive data then stays in that
+// XCD's L2).  A bijection on [0, gridDim.x).
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int x = b & 7, local = b >> 3;
+  const int q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + local;
+}
+
 // Per-tuple scalars of one primitive tuple.
 struct PrimTuple {
   double aij, akl, a0, a1;
@@ -284,7 +295,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
 
   const int slot = threadIdx.x / T;
   const int t = threadIdx.x - slot * T;
-  const int64_t q = (int64_t)blockIdx.x * Q + slot;
+  const int64_t q = (int64_t)xcd_block() * Q + slot;
   const bool active = q < a.nq;
   const int2 pq = a.quartets[active ? q : 0];
   const int boff = a.pair_off[pq.x];
