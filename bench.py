@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sample", type=int, default=4000)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip training/evaluating the learned Rys initializer")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the class kernels directly instead of replaying a hipGraph")
     args = ap.parse_args()
@@ -113,6 +115,11 @@ def main():
     fp64_rate = p64.ntuples / step_s
     out_bytes = p64.nelem * 8
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_roots_report()
+        log(f"learned roots: {learned['summary']}")
+
     cpu = None
     if not args.no_cpu:
         t64, _ = p64.tuples_cached()
@@ -160,8 +167,40 @@ def main():
                        "tflops": round(b["flops"] / max(b["ms"], 1e-9) / 1e9, 3),
                        "tuples": b["tuples"]} for b in bins],
         "cpu_baseline": cpu,
+        "learned_roots": learned,
     }
     print(json.dumps(res), flush=True)
+
+
+def learned_roots_report(nsamples=1 << 22, epochs=2):
+    """Train the MLP initializer on the device (2^22 samples, x log-uniform in
+    [1e-6, 1e3], n = 1..13) and compare, per n, MLP + 2 Newton steps (table
+    fallback when Newton does not settle below 1e-14) with the table rule."""
+    from cintx_amd import rys_mlp
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    params, loss = rys_mlp.train(nsamples=nsamples, epochs=epochs)
+    train_s = time.perf_counter() - t0
+    rng = np.random.default_rng(3)
+    x = np.exp(rng.uniform(np.log(1e-6), np.log(1e3), 1 << 16))
+    per_n = {}
+    for n in range(1, 10):
+        u, w, st, ms_mlp = rys_mlp.evaluate(params, n, x, newton_steps=2, init_mode=0)
+        _, _, st0, _ = rys_mlp.evaluate(params, n, x, newton_steps=2, init_mode=1)
+        _, _, _, ms_tab = rys_mlp.evaluate(params, n, x, newton_steps=0, init_mode=2)
+        err = 0.0
+        for k in range(0, len(x), 2048):
+            uo, wo = O.rys_roots(n, x[k])
+            err = max(err, float(np.max(np.abs(u[k] - uo) / uo)),
+                      float(np.max(np.abs(w[k] - wo) / wo)))
+        per_n[n] = {"settled": float(st.mean()), "settled_without_network": float(st0.mean()),
+                    "fallback_rate": float(1 - st.mean()), "max_rel_err_vs_oracle": err,
+                    "ms_mlp_newton": ms_mlp, "ms_table": ms_tab}
+    summ = ", ".join(f"n={n}: {v['settled']:.2f}/{v['settled_without_network']:.2f}"
+                     for n, v in per_n.items())
+    return {"train_samples": nsamples, "epochs": epochs, "final_loss": loss,
+            "train_s": train_s, "eval_points": len(x), "per_nroots": per_n,
+            "summary": "settled fraction with/without network: " + summ}
 
 
 def sample_errors(mol, p64, pmx, n):

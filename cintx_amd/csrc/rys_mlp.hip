@@ -297,8 +297,11 @@ __global__ void mlp_eval_kernel(const float* __restrict__ prm, RysTable tab, int
   if (i >= nx) return;
   const double xi = x[i];
   double s[kMaxRoots], ww[kMaxRoots];
-  mlp_initial(sp, tab, n, xi, mode, s, ww);
-  bool ok = newton_moments(n, xi, s, ww, steps);
+  bool ok = false;
+  if (mode != 2) {  // mode 2: the table rule alone (timing reference)
+    mlp_initial(sp, tab, n, xi, mode, s, ww);
+    ok = newton_moments(n, xi, s, ww, steps);
+  }
   if (!ok) {  // plain solver
     for (int r = 0; r < n; ++r) {
       double q;

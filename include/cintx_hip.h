@@ -205,7 +205,7 @@ int cintx_rys_mlp_train(float *params, int64_t nsamples, int epochs, int batch, 
 int cintx_rys_mlp_grad(const float *params, const float *z, const int *nroots,
                        const float *targets, int batch, float *grad, float *loss);
 /* init_mode 0: network, 1: asymptotic rule (the initializer without the
- * network).  status[i] = 1 when Newton settled below 1e-14 relative, 0 when
+ * network), 2: the table rule alone (no Newton; timing reference).  status[i] = 1 when Newton settled below 1e-14 relative, 0 when
  * the table rule was substituted.  ms: device time of the evaluation. */
 int cintx_rys_mlp_eval(const float *params, int n, const double *x, int64_t nx,
                        int newton_steps, int init_mode, double *u, double *w, int *status,
