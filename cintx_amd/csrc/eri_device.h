@@ -501,8 +501,9 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
     const int npq = mij * mkl;
     for (int base = 0; base < npq; base += P) {
       bool any = false;
-      if (t < P * NR) {
-        const int pp = t / NR, r = t - pp * NR;
+      // strided: P*NR and P*3*NR can exceed T when T < 3*NR (P is then 1)
+      for (int ur = t; ur < P * NR; ur += T) {
+        const int pp = ur / NR, r = ur - pp * NR;
         const int k = base + pp;
         const int kp = k / mij, ip = k - kp * mij;
         const bool kin = k < npq && kp < np_kl && ip < np_ij;
@@ -545,11 +546,11 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
           sc[14] = a.ai[iidx];
           if (ok) ++nsurv;
         }
-        any = ok;
+        any = any || ok;
       }
       if (!__syncthreads_or(any)) continue;
-      if (t < P * 3 * NR) {
-        const int pp = t / (3 * NR), c = t - pp * 3 * NR;
+      for (int uc = t; uc < P * 3 * NR; uc += T) {
+        const int pp = uc / (3 * NR), c = uc - pp * 3 * NR;
         const int axis = c / NR, r = c - axis * NR;
         const double* sc = sps[slot][pp];
         const double* uwp = suwb[slot][pp];
