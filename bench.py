@@ -45,6 +45,8 @@ def main():
     ap.add_argument("--sample", type=int, default=4000)
     ap.add_argument("--no-learned", action="store_true",
                     help="skip training/evaluating the learned Rys initializer")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run every class with its default launch shape")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the class kernels directly instead of replaying a hipGraph")
     args = ap.parse_args()
@@ -68,6 +70,16 @@ def main():
                     mixed_threshold=args.mixed_threshold)
     log(f"mixed plan: {pmx.ntuples} quartets, {pmx.ntuples_fp32} in FP32, {pmx.nbins} bins "
         f"({time.time() - t0:.1f}s)")
+
+    tuned = {}
+    if not args.no_autotune:
+        for plan, label in ((p64, "fp64"), (pmx, "mixed")):
+            t0 = time.time()
+            plan.autotune(reps=2)
+            tuned[label] = ["{}{}{}{}{}:T{}P{}".format(*b["class"], "df"[b["precision"]],
+                                                         v["lanes"], v["prims"])
+                            for b, v in zip(plan.bin_stats(), plan.bin_variants())]
+            log(f"autotune {label}: {time.time() - t0:.1f}s")
 
     def timed(plan, label):
         for _ in range(args.warmup):
@@ -167,6 +179,7 @@ def main():
                        "tflops": round(b["flops"] / max(b["ms"], 1e-9) / 1e9, 3),
                        "tuples": b["tuples"]} for b in bins],
         "cpu_baseline": cpu,
+        "launch_shapes": tuned,
         "learned_roots": learned,
     }
     print(json.dumps(res), flush=True)

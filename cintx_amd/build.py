@@ -60,6 +60,48 @@ def class_sets():
     return uniq
 
 
+def _p2(v):
+    p = 1
+    while p < v:
+        p *= 2
+    return p
+
+
+def variants(deriv, prec, ls):
+    """Launch-shape candidates (lanes per tuple T, primitive tuples per step P)
+    of a class kernel; the first is the untuned default.  Mirrors the sizes of
+    ClassShape in csrc/eri_device.h."""
+    li, lj, lk, ll = ls
+    nc = [(l + 1) * (l + 2) // 2 for l in ls]
+    nr = (li + deriv + lj + lk + ll) // 2 + 1
+    g4 = (li + deriv + 1) * (lj + 1) * (lk + 1) * (ll + 1)
+    gs = 3 * g4 * nr
+    nf = nc[0] * nc[1] * nc[2] * nc[3]
+    ne = nf * (3 if deriv else 1)
+    if ne <= 8:
+        return [(1, 1)]
+    nb = nc[0] if deriv else nc[0] * nc[1]
+    nrow = nf // nb
+    narrow = 4 if ne <= 48 else 16 if ne <= 192 else 64 if ne <= 768 else 256
+    wide = min(256, max(4, _p2(max(nrow, 3 * nr))))
+    groups = wide // nrow if wide >= nrow else 1
+    plds = max(1, (60 * 1024) // (gs * 8 + (3 * nr + 16) * 8))
+    p0 = max(1, min(8, wide // (3 * nr)))
+    pw = min(plds, max(p0, min(groups, 8)))
+    cands = [(wide, pw), (wide, min(plds, 2 * pw, 8)), (wide, 1), (narrow, 1)]
+    # one lane per tuple, everything in registers: the default for small
+    # classes, a candidate up to 81 elements
+    if ne <= 27:
+        cands.insert(0, (1, 1))
+    elif ne <= 81:
+        cands.insert(3, (1, 1))
+    out = []
+    for c in cands:
+        if c not in out:
+            out.append(c)
+    return out[:4]
+
+
 def generate(nparts=24):
     os.makedirs(GEN, exist_ok=True)
     classes = class_sets()
@@ -70,8 +112,9 @@ def generate(nparts=24):
                  f"void register_part_{p}(Registry& reg) {{"]
         for deriv, prec, (a, b, c, d) in cls:
             real = "double" if prec == 0 else "float"
-            lines.append(f"  reg.k[{deriv}][{prec}][{a}][{b}][{c}][{d}] = "
-                         f"class_info<{a}, {b}, {c}, {d}, {deriv}, {real}>();")
+            for v, (tt, pp) in enumerate(variants(deriv, prec, (a, b, c, d))):
+                lines.append(f"  reg.k[{deriv}][{prec}][{a}][{b}][{c}][{d}][{v}] = "
+                             f"class_info<{a}, {b}, {c}, {d}, {deriv}, {real}, {tt}, {pp}>();")
         lines += ["}", "}  // namespace cx", ""]
         path = os.path.join(GEN, f"inst_{p:02d}.hip")
         text = "\n".join(lines)
@@ -139,7 +182,8 @@ def build(jobs=None, verbose=True):
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
     if verbose:
-        print(f"built {LIB} ({len(class_sets())} class kernels)")
+        nk = sum(len(variants(d, p, l)) for d, p, l in class_sets())
+        print(f"built {LIB} ({len(class_sets())} classes, {nk} class-kernel variants)")
     return LIB
 
 

@@ -48,7 +48,9 @@ EXPORTS = [
     "cintx_plan_create", "cintx_plan_create_screened", "cintx_plan_destroy",
     "cintx_plan_sizes", "cintx_plan_info", "cintx_plan_tuples", "cintx_plan_execute", "cintx_plan_sync",
     "cintx_plan_copy_output", "cintx_plan_copy_tuples", "cintx_plan_execute_graph", "cintx_plan_last_time_ms",
-    "cintx_plan_work", "cintx_plan_execute_timed", "cintx_plan_jk", "cintx_plan_bin_stats", "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
+    "cintx_plan_work", "cintx_plan_execute_timed", "cintx_plan_jk", "cintx_plan_bin_stats",
+    "cintx_plan_autotune", "cintx_plan_set_variant", "cintx_plan_bin_variants",
+    "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
 ]
 
 
@@ -116,6 +118,9 @@ def lib():
         L.cintx_plan_execute_timed.argtypes = [vp, DP]
         L.cintx_plan_jk.argtypes = [vp, DP, DP, DP]
         L.cintx_plan_bin_stats.argtypes = [vp, ctypes.c_int64, IP, I64P, DP, DP, DP]
+        L.cintx_plan_autotune.argtypes = [vp, ctypes.c_int]
+        L.cintx_plan_set_variant.argtypes = [vp, ctypes.c_int]
+        L.cintx_plan_bin_variants.argtypes = [vp, ctypes.c_int64, IP, IP, IP, IP, IP]
         L.cintx_device_count.argtypes = [IP]
         _lib = L
     return _lib
@@ -290,6 +295,20 @@ class Plan:
                  "generic": bool(c[5]), "tuples": int(t), "ms": float(m),
                  "prim_tuples": float(p), "flops": float(f)}
                 for c, t, m, p, f in zip(cls, nt, ms, pt, fl)]
+
+    def autotune(self, reps=3):
+        """time every compiled launch-shape variant on each bin, keep the fastest"""
+        _check(lib().cintx_plan_autotune(self.h, int(reps)))
+
+    def set_variant(self, v):
+        _check(lib().cintx_plan_set_variant(self.h, int(v)))
+
+    def bin_variants(self):
+        n = self.nbins
+        cols = [np.zeros(n, dtype=np.int32) for _ in range(5)]
+        _check(lib().cintx_plan_bin_variants(self.h, n, *[_ip(c) for c in cols]))
+        return [{"variant": int(a), "nvariants": int(b), "lanes": int(c), "prims": int(d),
+                 "tuples_per_block": int(e)} for a, b, c, d, e in zip(*cols)]
 
     def jk(self, D):
         """J and K against the symmetric density D (unique-quartet plans)."""

@@ -101,7 +101,8 @@ struct PrimTuple {
 };
 
 // One (axis, root) recurrence chain: 2D VRR then bra and ket HRR; writes
-// g[((axis*DI + i)*DJ + j)*DK + k)*DL + l)*NR + r].
+// g[((axis*DI + i)*DJ + j)*DK + k)*DL + l)*RS + r] (root stride RS, NR by
+// default; 1 when a lane builds one root at a time in registers).
 //
 // Register budget: the VRR is streamed over the ket power m (columns m-1, m,
 // m+1 live), each finished column goes straight through the bra transfer
@@ -109,7 +110,7 @@ struct PrimTuple {
 // slots, and the ket transfer then runs block by block in place.  This keeps
 // about 4*(NMAX+1) + (MMAX+1) values live instead of the whole 2D and 3D
 // intermediates.
-template <int LIc, int LJ, int LK, int LL, typename R>
+template <int LIc, int LJ, int LK, int LL, typename R, int RS = -1>
 __device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, double s,
                                             double q, double wz, double inv_sum, double hij, double hkl,
                                             double aij, double akl, double dx, double prb,
@@ -122,6 +123,7 @@ __device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, 
   constexpr int A = IBASE ? LIc : LJ, B = IBASE ? LJ : LIc;
   constexpr int C = KBASE ? LK : LL, D = KBASE ? LL : LK;
   constexpr int BLK = DK * DL;  // positions per (i, j) block
+  constexpr int NRS = RS < 0 ? NR : RS;
 
   // libcint's per-root coefficients (g2e.c:4523-4543) written in s = u/(1+u):
   // tmp4 = .5/(a1 (1+u)) = .5 (1-s)/a1 and tmp5 = u2 tmp4 = .5 s/(aij+akl).
@@ -134,7 +136,7 @@ __device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, 
   const R c0p = (R)(qrk + sa * aij * dx);
   const R drb = (R)rij_ax;
   const R drk = (R)rkl_ax;
-  R* __restrict__ ga = g + (size_t)axis * DI * DJ * BLK * NR + r;
+  R* __restrict__ ga = g + (size_t)axis * DI * DJ * BLK * NRS + r;
 
   // bra transfer of one finished VRR column m: h1[i][j][m] -> block(i,j)+m
   auto bra = [&](const R* col, int m) {
@@ -151,7 +153,7 @@ __device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, 
       for (int a = 0; a <= A; ++a) {
         const int i = IBASE ? a : b;
         const int j = IBASE ? b : a;
-        ga[((i * DJ + j) * BLK + m) * NR] = t[a];
+        ga[((i * DJ + j) * BLK + m) * NRS] = t[a];
       }
     }
   };
@@ -185,10 +187,10 @@ __device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, 
   if constexpr (D > 0 || !KBASE) {
 #pragma unroll
     for (int ij = 0; ij < DI * DJ; ++ij) {
-      R* blk = ga + (size_t)ij * BLK * NR;
+      R* blk = ga + (size_t)ij * BLK * NRS;
       R t[MMAX + 1];
 #pragma unroll
-      for (int c = 0; c <= MMAX; ++c) t[c] = blk[c * NR];
+      for (int c = 0; c <= MMAX; ++c) t[c] = blk[c * NRS];
 #pragma unroll
       for (int d = 0; d <= D; ++d) {
         if (d > 0) {
@@ -199,7 +201,7 @@ __device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, 
         for (int c = 0; c <= C; ++c) {
           const int k = KBASE ? c : d;
           const int l = KBASE ? d : c;
-          blk[(k * DL + l) * NR] = t[c];
+          blk[(k * DL + l) * NRS] = t[c];
         }
       }
     }
@@ -234,43 +236,121 @@ __device__ __forceinline__ void c2s_pass(double* __restrict__ dst, const double*
   }
 }
 
-// Lanes per tuple and tuples per workgroup for a class.
-template <int LI, int LJ, int LK, int LL, int DERIV>
+// Synchronisation of the T lanes of one tuple.  A tuple of at most 64 lanes
+// lies inside one wavefront (T divides 64), whose LDS operations complete in
+// issue order, so a wavefront barrier with LDS fences is enough; wider tuples
+// need the workgroup barrier.  Trip counts of the loops that contain these
+// calls are uniform over the workgroup.
+template <int T>
+__device__ __forceinline__ void tuple_sync() {
+  if constexpr (T > 64) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+template <int T>
+__device__ __forceinline__ bool tuple_any(bool v) {
+  if constexpr (T > 64) {
+    return __syncthreads_or(v);
+  } else {
+    tuple_sync<T>();
+    return __any(v);
+  }
+}
+
+__host__ __device__ constexpr int pow2ceil(int v) {
+  int p = 1;
+  while (p < v) p *= 2;
+  return p;
+}
+
+// The same transform on a block held in registers (one lane per tuple):
+// in[OUTER][ncart(L)][INNER] -> out[OUTER][nsph(L)][INNER], indices
+// compile-time once unrolled; the coefficient reads are wave-uniform.
+template <int L, int INNER, int OUTER>
+__device__ __forceinline__ void c2s_reg(const double* in, double* out,
+                                        const double* __restrict__ c2s) {
+  constexpr int NC = ncart(L), NSL = nsph(L);
+  if constexpr (L < 2) {
+#pragma unroll
+    for (int i = 0; i < OUTER * NC * INNER; ++i) out[i] = in[i];
+  } else {
+    const double* tab = c2s + (size_t)L * kC2sM * kC2sC;
+#pragma unroll
+    for (int o = 0; o < OUTER; ++o)
+#pragma unroll
+      for (int m = 0; m < NSL; ++m)
+#pragma unroll
+        for (int i = 0; i < INNER; ++i) {
+          double v = 0;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) v += tab[m * kC2sC + c] * in[(o * NC + c) * INNER + i];
+          out[(o * NSL + m) * INNER + i] = v;
+        }
+  }
+}
+
+// Lanes per tuple (TT), primitive tuples per step (PP) and tuples per
+// workgroup of one class-kernel variant.  The candidate (TT, PP) pairs of a
+// class are generated by the build and chosen per plan bin by timing them
+// (cintx_plan_autotune); the first candidate is the untuned default.
+//
+// Gout blocking of the wide (T > 1) path: a lane owns a row of NB elements
+// that share their (k, l) components (all (i, j) components; nabla_i: all i
+// components of one (j, k, l)), so per root it loads the (LI+1)(LJ+1) g
+// values of each axis once and reuses them across the row instead of loading
+// three values per element.  With more lanes than rows, G lane groups split
+// the P primitive tuples of a step and their sums are added at the end.
+template <int LI, int LJ, int LK, int LL, int DERIV, int TT, int PP>
 struct ClassShape {
   static constexpr int LIc = LI + DERIV;
   static constexpr int NR = (LIc + LJ + LK + LL) / 2 + 1;
   static constexpr int DI = LIc + 1, DJ = LJ + 1, DK = LK + 1, DL = LL + 1;
   static constexpr int G4 = DI * DJ * DK * DL;
+  static constexpr int GS = 3 * G4 * NR;
   static constexpr int NF = ncart(LI) * ncart(LJ) * ncart(LK) * ncart(LL);
   static constexpr int NCOMP = DERIV ? 3 : 1;
   static constexpr int NE = NF * NCOMP;
-  static constexpr int T = NE <= 8 ? 1 : (NE <= 48 ? 4 : (NE <= 192 ? 16 : (NE <= 768 ? 64 : 256)));
-  static constexpr int NFT = (NF + T - 1) / T;
-  // primitive tuples in flight per step (wide tuples): enough to keep the
-  // lanes of the root and recurrence phases busy
+  static constexpr int NB = DERIV ? ncart(LI) : ncart(LI) * ncart(LJ);
+  static constexpr int NROW = NF / NB;
+  static constexpr int T = TT;
+  static constexpr int P = TT == 1 ? 1 : PP;
+  static constexpr int G = T >= NROW ? T / NROW : 1;               // groups per row
+  static constexpr int RPL = T >= NROW ? 1 : (NROW + T - 1) / T;   // rows per lane
+  static constexpr int NFT = (NF + T - 1) / T;                     // T == 1 path
   static constexpr int NSC = 16;  // per-primitive scalars shared in LDS
-  // (LDS per primitive slot is sized for FP64 so both precisions share P)
-  static constexpr int PLDS = (60 * 1024 - 2 * NF * 8) / (3 * G4 * NR * 8 + (3 * NR + NSC) * 8);
-  static constexpr int P = T == 1 ? 1 : cmax(1, cmin(cmin(8, T / (3 * NR)), PLDS));
+  // the Cartesian/spherical staging buffers of the epilogue alias the g slots
+  template <typename R>
+  static constexpr int union_doubles() {  // per tuple: g slots | staging
+    return cmax((P * GS * (int)sizeof(R) + 7) / 8, 2 * NF);
+  }
   template <typename R>
   static constexpr int lds_per_tuple() {
-    return T == 1 ? (int)(2 * NF * sizeof(double))
-                  : (int)(P * (3 * G4 * NR * sizeof(R) + (3 * NR + NSC) * sizeof(double)) +
-                          2 * NF * sizeof(double));
+    return T == 1 ? 0 : (union_doubles<R>() + P * (3 * NR + NSC)) * 8;
   }
+  // at most 256 lanes per workgroup and about 32 KB of LDS (finer occupancy
+  // steps on the 160 KB of a CU), one tuple at least
   template <typename R>
   static constexpr int Q() {
     int q = 256 / T;
-    while (q > 1 && q * lds_per_tuple<R>() > 60 * 1024) q /= 2;
+    if (q < 1) q = 1;
+    while (q > 1 && q * lds_per_tuple<R>() > 32 * 1024) q /= 2;
     return q;
   }
+  static_assert(NF % NB == 0, "row blocking");
+  static_assert(T == 1 || T >= 4, "lanes per tuple");
 };
 
-template <int LI, int LJ, int LK, int LL, int DERIV, typename R>
-__global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
-                                   ClassShape<LI, LJ, LK, LL, DERIV>::template Q<R>()))
+template <int LI, int LJ, int LK, int LL, int DERIV, typename R, int TT, int PP>
+__global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T *
+                                   ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::template Q<R>()))
     eri_class_kernel(EriArgs a) {
-  using S = ClassShape<LI, LJ, LK, LL, DERIV>;
+  using S = ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>;
+  static_assert(S::template Q<R>() * S::template lds_per_tuple<R>() <= 64 * 1024, "LDS");
   constexpr int T = S::T;
   constexpr int Q = S::template Q<R>();
   constexpr int LIc = S::LIc;
@@ -280,18 +360,22 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
   constexpr int NFI = ncart(LI), NFJ = ncart(LJ), NFK = ncart(LK), NFL = ncart(LL);
   constexpr int NF = S::NF;
   constexpr int NCOMP = S::NCOMP;
-  constexpr int NE = S::NE;
   constexpr int NFT = S::NFT;
   constexpr bool IBASE = LIc > LJ, KBASE = LK > LL;
-  constexpr int GS = 3 * G4 * NR;
+  constexpr int GS = S::GS;
+  constexpr int NB = S::NB, NROW = S::NROW, G = S::G, RPL = S::RPL;
 
   // One lane per tuple keeps g and the roots in registers (every index is a
   // compile-time constant once the loops are unrolled); wider tuples share
   // them through LDS.
   constexpr bool REG = (T == 1);
-  __shared__ double sbuf[Q][2][NF];
-  R greg[REG ? GS : 1];
-  double uwreg[REG ? 3 * NR : 1];
+  constexpr int UD = REG ? 0 : S::template union_doubles<R>();
+  constexpr int LDS_D = REG ? 1 : Q * (UD + S::P * (3 * NR + S::NSC));
+  __shared__ double lds[LDS_D];
+  constexpr int ACC = REG ? NFT * NCOMP : RPL * NB * NCOMP;
+  double acc[ACC];
+#pragma unroll
+  for (int kk = 0; kk < ACC; ++kk) acc[kk] = 0.0;
 
   const int slot = threadIdx.x / T;
   const int t = threadIdx.x - slot * T;
@@ -313,30 +397,6 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
     rij[d] = IBASE ? ri[d] - rj[d] : rj[d] - ri[d];
     rkl[d] = KBASE ? rk[d] - rl[d] : rl[d] - rk[d];
   }
-
-  // packed LDS positions of this lane's elements: x | y << 10 | z << 20
-  int pos[NFT];
-#pragma unroll
-  for (int kk = 0; kk < NFT; ++kk) {
-    int e = t + kk * T;
-    if (e >= NF) e = 0;
-    int fi = e % NFI, e1 = e / NFI;
-    int fj = e1 % NFJ, e2 = e1 / NFJ;
-    int fk = e2 % NFK, fl = e2 / NFK;
-    int ix, iy, iz, jx, jy, jz, kx, ky, kz, lx, ly, lz;
-    cart_decode(LI, fi, ix, iy, iz);
-    cart_decode(LJ, fj, jx, jy, jz);
-    cart_decode(LK, fk, kx, ky, kz);
-    cart_decode(LL, fl, lx, ly, lz);
-    int px = ((ix * DJ + jx) * DK + kx) * DL + lx;
-    int py = ((iy * DJ + jy) * DK + ky) * DL + ly;
-    int pz = ((iz * DJ + jz) * DK + kz) * DL + lz;
-    pos[kk] = px | (py << 10) | (pz << 20);
-  }
-
-  double acc[NFT * NCOMP];
-#pragma unroll
-  for (int kk = 0; kk < NFT * NCOMP; ++kk) acc[kk] = 0.0;
   unsigned int nsurv = 0;
 
   // Primitive counts are per tuple; a workgroup of several tuples loops to
@@ -344,23 +404,33 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
   // more) and masks the excess iterations.
   const int np_ij = a.pair_np[pq.x], np_kl = a.pair_np[pq.y];
   int mij = np_ij, mkl = np_kl;
-  if constexpr (T > 1 || Q > 1) {
-    if constexpr (T > 1) {
-      __shared__ int smax[2];
-      if (threadIdx.x == 0) smax[0] = smax[1] = 0;
-      __syncthreads();
-      if (t == 0 && active) {
-        atomicMax(&smax[0], np_ij);
-        atomicMax(&smax[1], np_kl);
-      }
-      __syncthreads();
-      mij = smax[0];
-      mkl = smax[1];
+  if constexpr (T > 1) {
+    __shared__ int smax[2];
+    if (threadIdx.x == 0) smax[0] = smax[1] = 0;
+    __syncthreads();
+    if (t == 0 && active) {
+      atomicMax(&smax[0], np_ij);
+      atomicMax(&smax[1], np_kl);
     }
+    __syncthreads();
+    mij = smax[0];
+    mkl = smax[1];
   }
+  double* b0 = nullptr;  // wide path: epilogue staging (Cartesian in, spherical out)
+  double* b1 = nullptr;
   if constexpr (REG) {
-  R* __restrict__ g = greg;
-  double* __restrict__ uw = uwreg;
+  // One lane per tuple, everything in registers: per primitive tuple and
+  // root, the three axis chains (3*G4 values) and then every element.
+  (void)lds;
+  constexpr bool F64 = sizeof(R) == sizeof(double);
+  R g[3 * G4];
+  R part[F64 ? 1 : ACC];  // FP32: this primitive tuple's sums over roots
+  auto tgt = [&](int i) -> R& {
+    if constexpr (F64)
+      return acc[i];
+    else
+      return part[i];
+  };
 
   for (int kp = 0; kp < mkl; ++kp) {
     const bool kin = kp < np_kl;
@@ -377,11 +447,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
       const int iidx = boff + (iin ? ip : 0);
       const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
       const bool ok = klok && iin && cceij <= eijcut;
-      if constexpr (T > 1) {
-        if (!__syncthreads_or(ok)) continue;
-      } else {
-        if (!ok) continue;
-      }
+      if (!ok) continue;
       PrimTuple p;
       p.aij = a.aij[iidx];
       p.akl = akl;
@@ -400,108 +466,120 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
       p.a1 = p.aij * akl;
       p.a0 = p.a1 * p.inv_sum;
       // sqrt(a0/a1^3) = 1/(a1 sqrt(aij+akl)) = 4 hij hkl sqrt(inv_sum)
-      const double fac = ok ? a.common * a.kfac[iidx] * kkl : 0.0;
+      const double fac = a.common * a.kfac[iidx] * kkl;
       const double fac1 = 4.0 * p.hij * hkl * sqrt(p.inv_sum) * fac;
       const double x = p.a0 * rr;
-      if (ok && t == 0) ++nsurv;
-
-      // roots
-#pragma unroll(T == 1 ? NR : 1)
-      for (int r = t; r < NR; r += T) {
+      const R ai2 = DERIV ? (R)(-2.0 * a.ai[iidx]) : (R)0;
+      ++nsurv;
+      if constexpr (!F64) {
+#pragma unroll
+        for (int kk = 0; kk < ACC; ++kk) part[kk] = 0;
+      }
+#pragma unroll 1
+      for (int r = 0; r < NR; ++r) {
         double sr, qr, w;
         rys_eval(a.rys, NR, r, x, sr, qr, w);
-        uw[r] = sr;
-        uw[NR + r] = w * fac1;
-        uw[2 * NR + r] = qr;
-      }
-      if constexpr (T > 1) __syncthreads();
-      // recurrence chains
-#pragma unroll(T == 1 ? 3 * NR : 1)
-      for (int c = t; c < 3 * NR; c += T) {
-        const int axis = c / NR;
-        const int r = c - axis * NR;
-        // the axis selects compile-time members, so no private array is
-        // indexed at run time (that would put it in scratch memory)
-        if (axis == 0)
-          build_chain<LIc, LJ, LK, LL, R>(g, 0, r, uw[r], uw[2 * NR + r], uw[NR + r], p.inv_sum,
-                                          p.hij, p.hkl,
-                                          p.aij, p.akl, p.dx[0], p.prb[0], p.qrk[0], 1.0,
-                                          rij[0], rkl[0]);
-        else if (axis == 1)
-          build_chain<LIc, LJ, LK, LL, R>(g, 1, r, uw[r], uw[2 * NR + r], uw[NR + r], p.inv_sum,
-                                          p.hij, p.hkl,
-                                          p.aij, p.akl, p.dx[1], p.prb[1], p.qrk[1], 1.0,
-                                          rij[1], rkl[1]);
-        else
-          build_chain<LIc, LJ, LK, LL, R>(g, 2, r, uw[r], uw[2 * NR + r], uw[NR + r], p.inv_sum,
-                                          p.hij, p.hkl,
-                                          p.aij, p.akl, p.dx[2], p.prb[2], p.qrk[2], 0.0,
-                                          rij[2], rkl[2]);
-      }
-      if constexpr (T > 1) __syncthreads();
-      // Cartesian elements
-      if constexpr (!DERIV) {
+        const double wz = w * fac1;
 #pragma unroll
-        for (int kk = 0; kk < NFT; ++kk) {
-          if (kk * T + t < NF) {
-            const R* gx = g + (pos[kk] & 1023) * NR;
-            const R* gy = g + (G4 + ((pos[kk] >> 10) & 1023)) * NR;
-            const R* gz = g + (2 * G4 + (pos[kk] >> 20)) * NR;
-            R s = 0;
+        for (int axis = 0; axis < 3; ++axis)
+          build_chain<LIc, LJ, LK, LL, R, 1>(g, axis, 0, sr, qr, wz, p.inv_sum, p.hij, p.hkl,
+                                             p.aij, p.akl, p.dx[axis], p.prb[axis], p.qrk[axis],
+                                             axis == 2 ? 0.0 : 1.0, rij[axis], rkl[axis]);
+        // every element; all g indices are compile-time constants
 #pragma unroll
-            for (int r = 0; r < NR; ++r) s += gx[r] * gy[r] * gz[r];
-            acc[kk] += (double)s;
+        for (int e = 0; e < NF; ++e) {
+          int ix, iy, iz, jx, jy, jz, kx, ky, kz, lx, ly, lz;
+          cart_decode(LI, e % NFI, ix, iy, iz);
+          cart_decode(LJ, (e / NFI) % NFJ, jx, jy, jz);
+          cart_decode(LK, (e / (NFI * NFJ)) % NFK, kx, ky, kz);
+          cart_decode(LL, e / (NFI * NFJ * NFK), lx, ly, lz);
+          const int px = ((ix * DJ + jx) * DK + kx) * DL + lx;
+          const int py = ((iy * DJ + jy) * DK + ky) * DL + ly;
+          const int pz = ((iz * DJ + jz) * DK + kz) * DL + lz;
+          const R* gx = g;
+          const R* gy = g + G4;
+          const R* gz = g + 2 * G4;
+          if constexpr (!DERIV) {
+            tgt(e) += gx[px] * gy[py] * gz[pz];
+          } else {
+            constexpr int SI = DJ * DK * DL;  // stride of the i power
+            R fx = ai2 * gx[px + SI];
+            R fy = ai2 * gy[py + SI];
+            R fz = ai2 * gz[pz + SI];
+            if (ix > 0) fx += (R)ix * gx[px - SI];
+            if (iy > 0) fy += (R)iy * gy[py - SI];
+            if (iz > 0) fz += (R)iz * gz[pz - SI];
+            tgt(e) += fx * gy[py] * gz[pz];
+            tgt(NF + e) += gx[px] * fy * gz[pz];
+            tgt(2 * NF + e) += gx[px] * gy[py] * fz;
           }
         }
-      } else {
-        const R ai2 = (R)(-2.0 * a.ai[iidx]);
-        constexpr int SI = DJ * DK * DL;  // stride of the i power
+      }
+      if constexpr (!F64) {
 #pragma unroll
-        for (int kk = 0; kk < NFT; ++kk) {
-          const int e = kk * T + t;
-          if (e < NF) {
-            const int px = pos[kk] & 1023, py = (pos[kk] >> 10) & 1023, pz = pos[kk] >> 20;
-            const int fi = e % NFI;
-            int ix, iy, iz;
-            cart_decode(LI, fi, ix, iy, iz);
-            const R* gx = g + px * NR;
-            const R* gy = g + (G4 + py) * NR;
-            const R* gz = g + (2 * G4 + pz) * NR;
-            R sx = 0, sy = 0, sz = 0;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-              R fx = ai2 * gx[SI * NR + r];
-              R fy = ai2 * gy[SI * NR + r];
-              R fz = ai2 * gz[SI * NR + r];
-              if (ix > 0) fx += (R)ix * gx[r - SI * NR];
-              if (iy > 0) fy += (R)iy * gy[r - SI * NR];
-              if (iz > 0) fz += (R)iz * gz[r - SI * NR];
-              sx += fx * gy[r] * gz[r];
-              sy += gx[r] * fy * gz[r];
-              sz += gx[r] * gy[r] * fz;
-            }
-            acc[kk] += (double)sx;
-            acc[NFT + kk] += (double)sy;
-            acc[2 * NFT + kk] += (double)sz;
-          }
-        }
+        for (int kk = 0; kk < ACC; ++kk) acc[kk] += (double)part[kk];
       }
     }
   }
   } else {
-    // Wide tuples: P primitive tuples per step.  Phase 1 (lanes < P*NR):
-    // screening, per-primitive scalars and one Rys root each; phase 2 (lanes
-    // < P*3*NR): one recurrence chain each; phase 3: every lane accumulates
-    // its elements over the P*NR quadrature points.
+    // Wide tuples: P primitive tuples per step.  Phase 1 (P*NR root slots):
+    // screening, per-primitive scalars and one Rys root each; phase 2
+    // (P*3*NR chain slots): one recurrence chain each; phase 3: every lane
+    // accumulates its rows over its groups' primitive tuples.  Slots beyond
+    // T lanes are strided (T < 3*NR happens for small tuples).
     constexpr int P = S::P;
     constexpr int NSC = S::NSC;
-    __shared__ R sgb[Q][P][GS];
-    __shared__ double suwb[Q][P][3 * NR];
-    __shared__ double sps[Q][P][NSC];
+    constexpr int BLK = DK * DL;
+    constexpr int SI = DJ * DK * DL;
+    constexpr bool F64 = sizeof(R) == sizeof(double);
+    double* const ubase = lds + slot * UD;
+    R* const sg = reinterpret_cast<R*>(ubase);  // [P][GS]
+    double* const suw = lds + Q * UD + slot * P * (3 * NR + NSC);  // [P][3NR]
+    double* const sps = suw + P * 3 * NR;                          // [P][NSC]
+    b0 = ubase;
+    b1 = ubase + NF;
+
+    // this lane's rows: packed base positions x | y << 10 | z << 20
+    const int grp = T >= NROW ? t / NROW : 0;
+    int rowid[RPL];
+    int rb[RPL];
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const int row = T >= NROW ? t - grp * NROW : t + T * k;
+      rowid[k] = (grp < G && row < NROW) ? row : -1;
+      const int rr = rowid[k] < 0 ? 0 : row;
+      int bx, by, bz;
+      if constexpr (!DERIV) {
+        int kx, ky, kz, lx, ly, lz;
+        cart_decode(LK, rr % NFK, kx, ky, kz);
+        cart_decode(LL, rr / NFK, lx, ly, lz);
+        bx = kx * DL + lx;
+        by = ky * DL + ly;
+        bz = kz * DL + lz;
+      } else {
+        int jx, jy, jz, kx, ky, kz, lx, ly, lz;
+        const int fj = rr % NFJ, r1 = rr / NFJ;
+        cart_decode(LJ, fj, jx, jy, jz);
+        cart_decode(LK, r1 % NFK, kx, ky, kz);
+        cart_decode(LL, r1 / NFK, lx, ly, lz);
+        bx = (jx * DK + kx) * DL + lx;
+        by = (jy * DK + ky) * DL + ly;
+        bz = (jz * DK + kz) * DL + lz;
+      }
+      rb[k] = bx | (by << 10) | (bz << 20);
+    }
+    // this step's partial sums (FP32 path) or the accumulators themselves
+    R svf[F64 ? 1 : ACC];
+    auto tgt = [&](int i) -> R& {
+      if constexpr (F64)
+        return acc[i];
+      else
+        return svf[i];
+    };
+
     const int npq = mij * mkl;
     for (int base = 0; base < npq; base += P) {
       bool any = false;
-      // strided: P*NR and P*3*NR can exceed T when T < 3*NR (P is then 1)
       for (int ur = t; ur < P * NR; ur += T) {
         const int pp = ur / NR, r = ur - pp * NR;
         const int k = base + pp;
@@ -511,50 +589,61 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
         const double ccekl = a.three_center ? 0.0 : a.cce[kidx] - a.lkl_ceil * logd_kl;
         const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
         const bool ok = active && kin && ccekl <= a.expcutoff && cceij <= a.expcutoff - ccekl;
-        const double aij = a.aij[iidx], akl = a.aij[kidx];
-        const double Pv[3] = {a.px[iidx], a.py[iidx], a.pz[iidx]};
-        const double Qv[3] = {a.px[kidx], a.py[kidx], a.pz[kidx]};
-        double dx[3], rr = 0;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          dx[d] = Pv[d] - Qv[d];
-          rr += dx[d] * dx[d];
-        }
-        const double inv_sum = 1.0 / (aij + akl);
-        const double hij = a.hinv[iidx], hkl = a.hinv[kidx];
-        const double fac = ok ? a.common * a.kfac[iidx] * a.kfac[kidx] : 0.0;
-        const double fac1 = 4.0 * hij * hkl * sqrt(inv_sum) * fac;
-        const double x = aij * akl * inv_sum * rr;
-        double sr, qr, w;
-        rys_eval(a.rys, NR, r, x, sr, qr, w);
-        suwb[slot][pp][r] = sr;
-        suwb[slot][pp][NR + r] = w * fac1;
-        suwb[slot][pp][2 * NR + r] = qr;
-        if (r == 0) {
-          double* sc = sps[slot][pp];
-          sc[0] = inv_sum;
-          sc[1] = hij;
-          sc[2] = hkl;
-          sc[3] = aij;
-          sc[4] = akl;
+        double* uwp = suw + pp * 3 * NR;
+        double* sc = sps + pp * NSC;
+        if (ok) {
+          const double aij = a.aij[iidx], akl = a.aij[kidx];
+          const double Pv[3] = {a.px[iidx], a.py[iidx], a.pz[iidx]};
+          const double Qv[3] = {a.px[kidx], a.py[kidx], a.pz[kidx]};
+          double dx[3], rr = 0;
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            sc[5 + d] = dx[d];
-            sc[8 + d] = Pv[d] - (IBASE ? ri[d] : rj[d]);
-            sc[11 + d] = Qv[d] - (KBASE ? rk[d] : rl[d]);
+            dx[d] = Pv[d] - Qv[d];
+            rr += dx[d] * dx[d];
           }
-          sc[14] = a.ai[iidx];
-          if (ok) ++nsurv;
+          const double inv_sum = 1.0 / (aij + akl);
+          const double hij = a.hinv[iidx], hkl = a.hinv[kidx];
+          const double fac = a.common * a.kfac[iidx] * a.kfac[kidx];
+          const double fac1 = 4.0 * hij * hkl * sqrt(inv_sum) * fac;
+          const double x = aij * akl * inv_sum * rr;
+          double sr, qr, w;
+          rys_eval(a.rys, NR, r, x, sr, qr, w);
+          uwp[r] = sr;
+          uwp[NR + r] = w * fac1;
+          uwp[2 * NR + r] = qr;
+          if (r == 0) {
+            sc[0] = inv_sum;
+            sc[1] = hij;
+            sc[2] = hkl;
+            sc[3] = aij;
+            sc[4] = akl;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              sc[5 + d] = dx[d];
+              sc[8 + d] = Pv[d] - (IBASE ? ri[d] : rj[d]);
+              sc[11 + d] = Qv[d] - (KBASE ? rk[d] : rl[d]);
+            }
+            sc[14] = a.ai[iidx];
+            ++nsurv;
+          }
         }
+        if (r == 0) sc[15] = ok ? 1.0 : 0.0;
         any = any || ok;
       }
-      if (!__syncthreads_or(any)) continue;
+      if (!tuple_any<T>(any)) continue;
+      // which of the P slots hold a surviving primitive tuple (read before
+      // the next barrier; phase 1 of the next step rewrites the flags)
+      unsigned okmask = 0;
+#pragma unroll
+      for (int pp = 0; pp < P; ++pp)
+        if (sps[pp * NSC + 15] != 0.0) okmask |= 1u << pp;
       for (int uc = t; uc < P * 3 * NR; uc += T) {
         const int pp = uc / (3 * NR), c = uc - pp * 3 * NR;
+        if (!((okmask >> pp) & 1u)) continue;
         const int axis = c / NR, r = c - axis * NR;
-        const double* sc = sps[slot][pp];
-        const double* uwp = suwb[slot][pp];
-        R* gp = sgb[slot][pp];
+        const double* sc = sps + pp * NSC;
+        const double* uwp = suw + pp * 3 * NR;
+        R* gp = sg + pp * GS;
         if (axis == 0)
           build_chain<LIc, LJ, LK, LL, R>(gp, 0, r, uwp[r], uwp[2 * NR + r], uwp[NR + r], sc[0],
                                           sc[1], sc[2], sc[3], sc[4], sc[5], sc[8], sc[11], 1.0,
@@ -568,62 +657,81 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
                                           sc[1], sc[2], sc[3], sc[4], sc[7], sc[10], sc[13], 0.0,
                                           rij[2], rkl[2]);
       }
-      __syncthreads();
-      if constexpr (!DERIV) {
+      // nabla_i: a_i of every slot, read before the barrier for the same reason
+      R ai2v[DERIV ? P : 1];
+      if constexpr (DERIV) {
 #pragma unroll
-        for (int kk = 0; kk < NFT; ++kk) {
-          if (kk * T + t < NF) {
-            R sum = 0;
-#pragma unroll 1
-            for (int pp = 0; pp < P; ++pp) {
-              const R* gpp = sgb[slot][pp];
-              const R* gx = gpp + (pos[kk] & 1023) * NR;
-              const R* gy = gpp + (G4 + ((pos[kk] >> 10) & 1023)) * NR;
-              const R* gz = gpp + (2 * G4 + (pos[kk] >> 20)) * NR;
+        for (int pp = 0; pp < P; ++pp) ai2v[pp] = (R)(-2.0 * sps[pp * NSC + 14]);
+      }
+      tuple_sync<T>();
+      if constexpr (!F64) {
 #pragma unroll
-              for (int r = 0; r < NR; ++r) sum += gx[r] * gy[r] * gz[r];
-            }
-            acc[kk] += (double)sum;
-          }
-        }
-      } else {
-        constexpr int SI = DJ * DK * DL;
+        for (int kk = 0; kk < ACC; ++kk) svf[kk] = 0;
+      }
 #pragma unroll
-        for (int kk = 0; kk < NFT; ++kk) {
-          const int e = kk * T + t;
-          if (e < NF) {
-            const int px = pos[kk] & 1023, py = (pos[kk] >> 10) & 1023, pz = pos[kk] >> 20;
-            int ix, iy, iz;
-            cart_decode(LI, e % NFI, ix, iy, iz);
-            R sx = 0, sy = 0, sz = 0;
-#pragma unroll 1
-            for (int pp = 0; pp < P; ++pp) {
-              const R* gpp = sgb[slot][pp];
-              const R ai2 = (R)(-2.0 * sps[slot][pp][14]);
-              const R* gx = gpp + px * NR;
-              const R* gy = gpp + (G4 + py) * NR;
-              const R* gz = gpp + (2 * G4 + pz) * NR;
+      for (int k = 0; k < RPL; ++k) {
+        if (rowid[k] < 0) continue;
+        const int bx = rb[k] & 1023, by = (rb[k] >> 10) & 1023, bz = rb[k] >> 20;
+        // P is at most 8; unrolled so ai2v[pp] stays in registers
 #pragma unroll
-              for (int r = 0; r < NR; ++r) {
-                R fx = ai2 * gx[SI * NR + r];
-                R fy = ai2 * gy[SI * NR + r];
-                R fz = ai2 * gz[SI * NR + r];
-                if (ix > 0) fx += (R)ix * gx[r - SI * NR];
-                if (iy > 0) fy += (R)iy * gy[r - SI * NR];
-                if (iz > 0) fz += (R)iz * gz[r - SI * NR];
-                sx += fx * gy[r] * gz[r];
-                sy += gx[r] * fy * gz[r];
-                sz += gx[r] * gy[r] * fz;
+        for (int pp = 0; pp < P; ++pp) {
+          if (pp % G != grp % G || grp >= G) continue;
+          if (!((okmask >> pp) & 1u)) continue;
+          const R* gpp = sg + pp * GS;
+          if constexpr (!DERIV) {
+            constexpr int NIJ = DI * DJ;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+              R xv[NIJ], yv[NIJ], zv[NIJ];
+#pragma unroll
+              for (int ij = 0; ij < NIJ; ++ij) {
+                xv[ij] = gpp[(ij * BLK + bx) * NR + r];
+                yv[ij] = gpp[(G4 + ij * BLK + by) * NR + r];
+                zv[ij] = gpp[(2 * G4 + ij * BLK + bz) * NR + r];
+              }
+#pragma unroll
+              for (int ib = 0; ib < NB; ++ib) {
+                int ix, iy, iz, jx, jy, jz;
+                cart_decode(LI, ib % NFI, ix, iy, iz);
+                cart_decode(LJ, ib / NFI, jx, jy, jz);
+                tgt(k * NB + ib) += xv[ix * DJ + jx] * yv[iy * DJ + jy] * zv[iz * DJ + jz];
               }
             }
-            acc[kk] += (double)sx;
-            acc[NFT + kk] += (double)sy;
-            acc[2 * NFT + kk] += (double)sz;
+          } else {
+            const R ai2 = ai2v[pp];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+              R xv[DI], yv[DI], zv[DI];
+#pragma unroll
+              for (int i = 0; i < DI; ++i) {
+                xv[i] = gpp[(i * SI + bx) * NR + r];
+                yv[i] = gpp[(G4 + i * SI + by) * NR + r];
+                zv[i] = gpp[(2 * G4 + i * SI + bz) * NR + r];
+              }
+#pragma unroll
+              for (int ib = 0; ib < NB; ++ib) {
+                int ix, iy, iz;
+                cart_decode(LI, ib, ix, iy, iz);
+                R fx = ai2 * xv[ix + 1];
+                R fy = ai2 * yv[iy + 1];
+                R fz = ai2 * zv[iz + 1];
+                if (ix > 0) fx += (R)ix * xv[ix - 1];
+                if (iy > 0) fy += (R)iy * yv[iy - 1];
+                if (iz > 0) fz += (R)iz * zv[iz - 1];
+                tgt((k * 3 + 0) * NB + ib) += fx * yv[iy] * zv[iz];
+                tgt((k * 3 + 1) * NB + ib) += xv[ix] * fy * zv[iz];
+                tgt((k * 3 + 2) * NB + ib) += xv[ix] * yv[iy] * fz;
+              }
+            }
           }
         }
-        __syncthreads();  // phase 1 of the next step rewrites a_i in sps
+      }
+      if constexpr (!F64) {
+#pragma unroll
+        for (int kk = 0; kk < ACC; ++kk) acc[kk] += (double)svf[kk];
       }
     }
+    (void)rowid;
   }
   if (a.counters && t == 0 && nsurv) {
     atomicAdd(a.counters, (unsigned long long)nsurv);
@@ -631,18 +739,53 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
   }
 
   // Cartesian -> spherical, then the strided write.
-  if constexpr (T > 1) __syncthreads();
-  double* b0 = sbuf[slot][0];
-  double* b1 = sbuf[slot][1];
   constexpr int NSI = nsph(LI), NSJ = nsph(LJ), NSK = nsph(LK), NSL = nsph(LL);
   constexpr int NS = NSI * NSJ * NSK * NSL;
-  for (int comp = 0; comp < NCOMP; ++comp) {
+  if constexpr (REG) {
+    // in registers, axis by axis
 #pragma unroll
-    for (int kk = 0; kk < NFT; ++kk) {
-      const int e = kk * T + t;
-      if (e < NF) b0[e] = acc[comp * NFT + kk];
+    for (int comp = 0; comp < NCOMP; ++comp) {
+      double t1[NSI * NFJ * NFK * NFL], t2[NSI * NSJ * NFK * NFL], t3[NSI * NSJ * NSK * NFL];
+      double t4[NS];
+      c2s_reg<LI, 1, NFJ * NFK * NFL>(acc + comp * NF, t1, a.c2s);
+      c2s_reg<LJ, NSI, NFK * NFL>(t1, t2, a.c2s);
+      c2s_reg<LK, NSI * NSJ, NFL>(t2, t3, a.c2s);
+      c2s_reg<LL, NSI * NSJ * NSK, 1>(t3, t4, a.c2s);
+      if (active) {
+        double* o = a.out + a.out_base + q * a.block + comp * a.st_comp;
+#pragma unroll
+        for (int e = 0; e < NS; ++e) {
+          const int mi = e % NSI, mj = (e / NSI) % NSJ, mk = (e / (NSI * NSJ)) % NSK,
+                    ml = e / (NSI * NSJ * NSK);
+          o[mi * a.st[0] + mj * a.st[1] + mk * a.st[2] + ml * a.st[3]] = t4[e];
+        }
+      }
     }
-    if constexpr (T > 1) __syncthreads();
+  } else {
+  tuple_sync<T>();
+  for (int comp = 0; comp < NCOMP; ++comp) {
+    // lane groups add their partial rows in turn
+    {
+      const int grp = T >= NROW ? t / NROW : 0;
+#pragma unroll 1
+      for (int gg = 0; gg < G; ++gg) {
+        if (grp == gg) {
+#pragma unroll
+          for (int k = 0; k < RPL; ++k) {
+            const int row = T >= NROW ? t - grp * NROW : t + T * k;
+            if (row < NROW) {
+#pragma unroll
+              for (int ib = 0; ib < NB; ++ib) {
+                const double v = acc[(k * NCOMP + comp) * NB + ib];
+                double* d = b0 + row * NB + ib;
+                *d = gg == 0 ? v : *d + v;
+              }
+            }
+          }
+        }
+        tuple_sync<T>();
+      }
+    }
     int dims[4] = {NFI, NFJ, NFK, NFL};
     const int ls[4] = {LI, LJ, LK, LL};
     double* src = b0;
@@ -655,7 +798,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
       double* tmp = src;
       src = dst;
       dst = tmp;
-      if constexpr (T > 1) __syncthreads();
+      tuple_sync<T>();
     }
     if (active) {
       double* o = a.out + a.out_base + q * a.block + comp * a.st_comp;
@@ -666,7 +809,8 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
         o[mi * a.st[0] + mj * a.st[1] + mk * a.st[2] + ml * a.st[3]] = src[e];
       }
     }
-    if constexpr (T > 1) __syncthreads();
+    tuple_sync<T>();
+  }
   }
 }
 
@@ -674,20 +818,21 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV>::T *
 
 namespace cx {
 
-template <int LI, int LJ, int LK, int LL, int DERIV, typename R>
+template <int LI, int LJ, int LK, int LL, int DERIV, typename R, int TT, int PP>
 hipError_t launch_class(const EriArgs& a, hipStream_t s) {
-  using S = ClassShape<LI, LJ, LK, LL, DERIV>;
+  using S = ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>;
   constexpr int Q = S::template Q<R>();
   const int64_t nblk = (a.nq + Q - 1) / Q;
-  hipLaunchKernelGGL((eri_class_kernel<LI, LJ, LK, LL, DERIV, R>), dim3((unsigned)nblk),
+  hipLaunchKernelGGL((eri_class_kernel<LI, LJ, LK, LL, DERIV, R, TT, PP>), dim3((unsigned)nblk),
                      dim3(S::T * Q), 0, s, a);
   return hipGetLastError();
 }
 
-template <int LI, int LJ, int LK, int LL, int DERIV, typename R>
+template <int LI, int LJ, int LK, int LL, int DERIV, typename R, int TT, int PP>
 KernelInfo class_info() {
-  using S = ClassShape<LI, LJ, LK, LL, DERIV>;
-  return KernelInfo{&launch_class<LI, LJ, LK, LL, DERIV, R>, S::template Q<R>(), S::T};
+  using S = ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>;
+  return KernelInfo{&launch_class<LI, LJ, LK, LL, DERIV, R, TT, PP>, S::template Q<R>(), S::T,
+                    S::P};
 }
 
 }  // namespace cx

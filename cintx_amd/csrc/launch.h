@@ -1,5 +1,6 @@
-// Class-kernel registry: one launcher per (derivative, precision, LI, LJ, LK,
-// LL) instantiated in the generated translation units.
+// Class-kernel registry: up to kVariants launchers per (derivative,
+// precision, LI, LJ, LK, LL), instantiated in the generated translation units;
+// variant 0 is the default, the others are autotuning candidates.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -9,14 +10,17 @@ namespace cx {
 
 typedef hipError_t (*LaunchFn)(const EriArgs&, hipStream_t);
 
+constexpr int kVariants = 4;
+
 struct KernelInfo {
   LaunchFn fn;
   int tuples_per_block;  // Q
   int lanes_per_tuple;   // T
+  int prims_per_step;    // P
 };
 
 struct Registry {
-  KernelInfo k[2][2][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1];
+  KernelInfo k[2][2][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1][kVariants];
 };
 
 // Filled by the generated instantiation units.

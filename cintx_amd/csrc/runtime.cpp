@@ -182,6 +182,8 @@ struct DevCtx {
   double* d_large = nullptr;
   double* d_c2s = nullptr;
   Registry* reg = nullptr;
+  // autotuned class-kernel variant per (deriv, prec, class); -1: untuned
+  signed char tuned[2][2][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1];
 };
 
 static std::mutex g_mu;  // guards context creation and the per-call arena
@@ -213,6 +215,7 @@ static DevCtx& ctx() {
   c->reg = new Registry();
   std::memset(c->reg, 0, sizeof(Registry));
   register_class_kernels(*c->reg);
+  std::memset(c->tuned, -1, sizeof c->tuned);
   g_ctx = c;
   return *c;
 }
@@ -401,6 +404,7 @@ struct Bin {
   double prim_tuples = 0;       // from the last work count
   double flops = 0;
   bool generic = false;  // no compile-time instantiation: runtime-class kernel
+  int variant = 0;       // class-kernel variant (launch shape) in use
   std::vector<int2> q;
   int64_t out_base = 0;  // element offset of the bin's first block
   int64_t block = 0;     // elements per block (NS * ncomp)
@@ -656,72 +660,94 @@ static void plan_upload(cintx_plan* P) {
   HIPCHK(hipEventCreate(&P->ev1));
 }
 
+static const KernelInfo* bin_kernels(const cintx_plan* P, const Bin& b) {
+  const FamilyInfo& f = kFam[P->family];
+  return ctx().reg->k[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
+}
+
+static signed char& tuned_slot(const cintx_plan* P, const Bin& b) {
+  const FamilyInfo& f = kFam[P->family];
+  return ctx().tuned[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
+}
+
+static void plan_drop_graph(cintx_plan* P) {
+  if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  P->graph_exec = nullptr;
+  P->graph = nullptr;
+}
+
 static void plan_check_kernels(cintx_plan* P) {
+  for (auto& b : P->bins) {
+    const KernelInfo* ks = bin_kernels(P, b);
+    b.generic = (ks[0].fn == nullptr);
+    const int v = tuned_slot(P, b);
+    b.variant = (v > 0 && v < kVariants && ks[v].fn) ? v : 0;
+  }
+}
+
+static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count) {
   DevCtx& C = ctx();
   const FamilyInfo& f = kFam[P->family];
-  for (auto& b : P->bins) {
-    const KernelInfo& k = C.reg->k[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
-    b.generic = (k.fn == nullptr);
+  const double common_base = (M_PI * M_PI * M_PI) * 2.0 / std::sqrt(M_PI);
+  Bin& b = P->bins[i];
+  const KernelInfo& k = bin_kernels(P, b)[b.variant];
+  EriArgs a;
+  // (generic bins run the runtime-class kernel below)
+  std::memset(&a, 0, sizeof a);
+  a.aij = P->d_aij;
+  a.px = P->d_px;
+  a.py = P->d_py;
+  a.pz = P->d_pz;
+  a.kfac = P->d_kfac;
+  a.ai = P->d_ai;
+  a.cce = P->d_cce;
+  a.hinv = P->d_hinv;
+  a.pair_off = P->d_poff;
+  a.pair_np = P->d_pnp;
+  a.pair_geo = P->d_geo;
+  a.common = common_base * common_fac_sp(b.l[0]) * common_fac_sp(b.l[1]) *
+             common_fac_sp(b.l[2]) * (f.three ? 1.0 : common_fac_sp(b.l[3]));
+  a.expcutoff = P->expcutoff;
+  a.lij_ceil = b.l[0] + b.l[1] + f.deriv;
+  a.lkl_ceil = b.l[2] + b.l[3];
+  a.three_center = f.three ? 1 : 0;
+  int ns0 = nsph(b.l[0]), ns1 = nsph(b.l[1]), ns2 = nsph(b.l[2]);
+  a.st[0] = 1;
+  a.st[1] = ns0;
+  a.st[2] = (int64_t)ns0 * ns1;
+  a.st[3] = (int64_t)ns0 * ns1 * ns2;
+  a.st_comp = b.block / f.ncomp;
+  a.out = out;
+  a.rys = C.rys;
+  a.c2s = C.d_c2s;
+  a.counters = count ? P->d_counters + 2 * i : nullptr;
+  // grid dimension limit: launch in chunks of at most 2^28 tuples
+  const int64_t chunk = (int64_t)1 << 28;
+  for (int64_t s = 0; s < (int64_t)b.q.size(); s += chunk) {
+    a.quartets = b.d_q + s;
+    a.out_base = b.out_base + s * b.block;
+    a.block = b.block;
+    a.nq = std::min(chunk, (int64_t)b.q.size() - s);
+    hipError_t e;
+    if (b.generic) {
+      a.scratch = P->d_scratch;
+      int grid = (int)std::min<int64_t>(a.nq, kGenericGrid);
+      e = launch_generic(a, b.l, f.deriv, grid, C.stream);
+    } else {
+      e = k.fn(a, C.stream);
+    }
+    if (e != hipSuccess)
+      fail(CINTX_STATUS_EXECUTION_FAILED, std::string("kernel launch: ") + hipGetErrorString(e));
   }
 }
 
 static void plan_launch_all(cintx_plan* P, double* out, bool count,
                             std::vector<hipEvent_t>* evs = nullptr) {
   DevCtx& C = ctx();
-  const FamilyInfo& f = kFam[P->family];
-  double common_base = (M_PI * M_PI * M_PI) * 2.0 / std::sqrt(M_PI);
   for (size_t i = 0; i < P->bins.size(); ++i) {
-    Bin& b = P->bins[i];
-    const KernelInfo& k = C.reg->k[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
-    EriArgs a;
-    // (generic bins run the runtime-class kernel below)
-    std::memset(&a, 0, sizeof a);
-    a.aij = P->d_aij;
-    a.px = P->d_px;
-    a.py = P->d_py;
-    a.pz = P->d_pz;
-    a.kfac = P->d_kfac;
-    a.ai = P->d_ai;
-    a.cce = P->d_cce;
-    a.hinv = P->d_hinv;
-    a.pair_off = P->d_poff;
-    a.pair_np = P->d_pnp;
-    a.pair_geo = P->d_geo;
-    a.common = common_base * common_fac_sp(b.l[0]) * common_fac_sp(b.l[1]) *
-               common_fac_sp(b.l[2]) * (f.three ? 1.0 : common_fac_sp(b.l[3]));
-    a.expcutoff = P->expcutoff;
-    a.lij_ceil = b.l[0] + b.l[1] + f.deriv;
-    a.lkl_ceil = b.l[2] + b.l[3];
-    a.three_center = f.three ? 1 : 0;
-    int ns0 = nsph(b.l[0]), ns1 = nsph(b.l[1]), ns2 = nsph(b.l[2]);
-    a.st[0] = 1;
-    a.st[1] = ns0;
-    a.st[2] = (int64_t)ns0 * ns1;
-    a.st[3] = (int64_t)ns0 * ns1 * ns2;
-    a.st_comp = b.block / f.ncomp;
-    a.out = out;
-    a.rys = C.rys;
-    a.c2s = C.d_c2s;
-    a.counters = count ? P->d_counters + 2 * i : nullptr;
     if (evs) HIPCHK(hipEventRecord((*evs)[2 * i], C.stream));
-    // grid dimension limit: launch in chunks of at most 2^28 tuples
-    const int64_t chunk = (int64_t)1 << 28;
-    for (int64_t s = 0; s < (int64_t)b.q.size(); s += chunk) {
-      a.quartets = b.d_q + s;
-      a.out_base = b.out_base + s * b.block;
-      a.block = b.block;
-      a.nq = std::min(chunk, (int64_t)b.q.size() - s);
-      hipError_t e;
-      if (b.generic) {
-        a.scratch = P->d_scratch;
-        int grid = (int)std::min<int64_t>(a.nq, kGenericGrid);
-        e = launch_generic(a, b.l, f.deriv, grid, C.stream);
-      } else {
-        e = k.fn(a, C.stream);
-      }
-      if (e != hipSuccess)
-        fail(CINTX_STATUS_EXECUTION_FAILED, std::string("kernel launch: ") + hipGetErrorString(e));
-    }
+    plan_launch_bin(P, i, out, count);
     if (evs) HIPCHK(hipEventRecord((*evs)[2 * i + 1], C.stream));
   }
 }
@@ -744,8 +770,7 @@ static cintx_plan* plan_new(int family, const int* atm, int natm, const int* bas
 
 static void plan_free(cintx_plan* P) {
   if (!P) return;
-  if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
-  if (P->graph) (void)hipGraphDestroy(P->graph);
+  plan_drop_graph(P);
   if (P->ev0) (void)hipEventDestroy(P->ev0);
   if (P->ev1) (void)hipEventDestroy(P->ev1);
   if (P->blob) (void)hipFree(P->blob);
@@ -1108,6 +1133,7 @@ int cintx_plan_create(cintx_plan** plan, int family, const int* atm, int natm, c
     cintx_plan* P = plan_new(family, atm, natm, bas, nbas, env, 0);
     try {
       plan_add_tuples(P, tuples, ntuples, precision == CINTX_PREC_MIXED ? 1 : 0);
+      P->ntuples_fp32 = precision == CINTX_PREC_MIXED ? (int64_t)P->tuples.size() : 0;
       plan_check_kernels(P);
       plan_upload(P);
     } catch (...) {
@@ -1156,12 +1182,7 @@ int cintx_plan_execute_graph(cintx_plan* plan, double* out_dev) {
     if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
     DevCtx& C = ctx();
     double* out = out_dev ? out_dev : plan->d_out;
-    if (plan->graph_exec && plan->graph_out != out) {
-      HIPCHK(hipGraphExecDestroy(plan->graph_exec));
-      HIPCHK(hipGraphDestroy(plan->graph));
-      plan->graph_exec = nullptr;
-      plan->graph = nullptr;
-    }
+    if (plan->graph_exec && plan->graph_out != out) plan_drop_graph(plan);
     if (!plan->graph_exec) {
       HIPCHK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
       try {
@@ -1350,6 +1371,87 @@ int cintx_plan_bin_stats(const cintx_plan* plan, int64_t nmax, int* cls, int64_t
       if (ms) ms[i] = b.last_ms;
       if (prim_tuples) prim_tuples[i] = b.prim_tuples;
       if (flops) flops[i] = b.flops;
+    }
+  });
+}
+
+int cintx_plan_autotune(cintx_plan* plan, int reps) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    if (reps < 1) fail(CINTX_STATUS_INVALID_INPUT, "reps must be >= 1");
+    DevCtx& C = ctx();
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    struct G {
+      hipEvent_t a, b;
+      ~G() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+      }
+    } guard{e0, e1};
+    plan_drop_graph(plan);
+    for (size_t i = 0; i < plan->bins.size(); ++i) {
+      Bin& b = plan->bins[i];
+      if (b.generic || b.q.empty()) continue;
+      const KernelInfo* ks = bin_kernels(plan, b);
+      int best = b.variant;
+      double best_ms = 1e300;
+      for (int v = 0; v < kVariants; ++v) {
+        if (!ks[v].fn) continue;
+        b.variant = v;
+        plan_launch_bin(plan, i, plan->d_out, false);  // warm-up
+        double vmin = 1e300;
+        for (int r = 0; r < reps; ++r) {
+          HIPCHK(hipEventRecord(e0, C.stream));
+          plan_launch_bin(plan, i, plan->d_out, false);
+          HIPCHK(hipEventRecord(e1, C.stream));
+          HIPCHK(hipEventSynchronize(e1));
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          vmin = std::min(vmin, (double)ms);
+        }
+        if (vmin < best_ms) {
+          best_ms = vmin;
+          best = v;
+        }
+      }
+      b.variant = best;
+      b.last_ms = best_ms;
+      tuned_slot(plan, b) = (signed char)best;
+    }
+  });
+}
+
+int cintx_plan_set_variant(cintx_plan* plan, int variant) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    if (variant < 0 || variant >= kVariants)
+      fail(CINTX_STATUS_INVALID_INPUT, "variant out of range");
+    plan_drop_graph(plan);
+    for (auto& b : plan->bins) {
+      const KernelInfo* ks = bin_kernels(plan, b);
+      b.variant = (!b.generic && ks[variant].fn) ? variant : 0;
+    }
+  });
+}
+
+int cintx_plan_bin_variants(const cintx_plan* plan, int64_t nmax, int* variant, int* nvariants,
+                            int* lanes, int* prims, int* tuples_per_block) {
+  return guarded([&] {
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+    int64_t n = std::min<int64_t>(nmax, (int64_t)plan->bins.size());
+    for (int64_t i = 0; i < n; ++i) {
+      const Bin& b = plan->bins[i];
+      const KernelInfo* ks = bin_kernels(plan, b);
+      int nv = 0;
+      for (int v = 0; v < kVariants; ++v) nv += ks[v].fn ? 1 : 0;
+      const KernelInfo& k = ks[b.variant];
+      if (variant) variant[i] = b.generic ? -1 : b.variant;
+      if (nvariants) nvariants[i] = nv;
+      if (lanes) lanes[i] = b.generic ? 256 : k.lanes_per_tuple;
+      if (prims) prims[i] = b.generic ? 1 : k.prims_per_step;
+      if (tuples_per_block) tuples_per_block[i] = b.generic ? 1 : k.tuples_per_block;
     }
   });
 }

@@ -136,7 +136,9 @@ typedef struct cintx_plan cintx_plan;
  * ignored).  Block t of the output holds the tuple's libcint-layout result
  * (natural dims, components leading) at element offset offsets[t]
  * (cintx_plan_offsets).  Replaces cintx-cubecl's per-tuple launch
- * (crates/cintx-cubecl/src/kernels/two_electron.rs:1309-1430). */
+ * (crates/cintx-cubecl/src/kernels/two_electron.rs:1309-1430).  No Schwarz
+ * bound is computed for an explicit list: CINTX_PREC_MIXED runs every tuple
+ * of it with the FP32 recurrences and mixed_threshold is ignored. */
 int cintx_plan_create(cintx_plan **plan, int family, const int *atm, int natm, const int *bas,
                       int nbas, const double *env, const int *tuples, int64_t ntuples,
                       int precision, double mixed_threshold);
@@ -179,6 +181,22 @@ int cintx_plan_execute_timed(cintx_plan *plan, double *out_dev);
 int cintx_plan_bin_stats(const cintx_plan *plan, int64_t nmax, int *cls, int64_t *ntuples,
                          double *ms, double *prim_tuples, double *flops);
 int cintx_plan_work(cintx_plan *plan, double *prim_tuples, double *root_evals, double *flops);
+/* Launch-shape autotuner (no counterpart in the reference, whose executor
+ * has a single launch shape; crates/cintx-runtime/src/dispatch.rs:187-195).
+ * Every class has up to four compiled variants (lanes per tuple, primitive
+ * tuples per step); cintx_plan_autotune times each on the plan's own bin
+ * (min of `reps` launches after one warm-up, HIP events), keeps the fastest
+ * per bin and records it for the process, so later plans of the same class
+ * start from it.  Results of all variants agree to rounding (summation order
+ * of the primitive loop differs).  cintx_plan_set_variant forces one variant
+ * on every bin that has it (variant 0 otherwise); cintx_plan_bin_variants
+ * reports per bin: variant in use (-1: runtime-class kernel), variants
+ * available, lanes per tuple, primitive tuples per step, tuples per
+ * workgroup. */
+int cintx_plan_autotune(cintx_plan *plan, int reps);
+int cintx_plan_set_variant(cintx_plan *plan, int variant);
+int cintx_plan_bin_variants(const cintx_plan *plan, int64_t nmax, int *variant, int *nvariants,
+                            int *lanes, int *prims, int *tuples_per_block);
 
 /* The n-point Rys rule (roots u = t^2/(1-t^2), weights) that the kernels use,
  * evaluated for nx arguments on the host or on the device; replaces

@@ -42,9 +42,11 @@ def oracle_blocks(O, name, tuples, mol):
     return out, offs, sizes
 
 
-def check_plan(O, name, tuples, mol):
-    say(f"plan {name} ntuples={len(tuples)}")
+def check_plan(O, name, tuples, mol, variant=None):
+    say(f"plan {name} ntuples={len(tuples)} variant={variant}")
     p = capi.Plan(name, mol.atm, mol.bas, mol.env, tuples)
+    if variant is not None:
+        p.set_variant(variant)
     p.execute()
     p.sync()
     got = p.output()
@@ -235,4 +237,100 @@ def test_jk_build_cfg1(oracle):
         f"|J| {np.abs(Jr).max():.3e}")
     assert np.abs(J - Jr).max() <= 1e-11 * max(1.0, np.abs(Jr).max())
     assert np.abs(K - Kr).max() <= 1e-11 * max(1.0, np.abs(Kr).max())
+    p.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_every_launch_variant(oracle, variant):
+    """Every compiled launch shape (lanes per tuple, primitive tuples per
+    step) of every class meets the same parity bar; classes with fewer
+    variants fall back to their default."""
+    mol = M.config(1)
+    n = mol.nbas
+    rng = np.random.default_rng(20 + variant)
+    full = [(i, j, k, l) for l in range(n) for k in range(n) for j in range(n) for i in range(n)]
+    check_plan(oracle, "int2e_sph", full, mol, variant)
+    check_plan(oracle, "int2e_ip1_sph",
+               [tuple(int(s) for s in rng.integers(0, n, 4)) for _ in range(1500)], mol, variant)
+    aux = M.config(4)
+    t3 = [(int(rng.integers(0, aux.aux_start)), int(rng.integers(0, aux.aux_start)),
+           int(rng.integers(aux.aux_start, aux.nbas))) for _ in range(1500)]
+    check_plan(oracle, "int3c2e_sph", t3, aux, variant)
+    check_plan(oracle, "int3c2e_ip1_sph", t3[:700], aux, variant)
+    # FP32 instantiations: every tuple of a mixed plan with a huge threshold
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, full, precision=capi.PREC_MIXED,
+                  mixed_threshold=1e300)
+    assert p.ntuples_fp32 == p.ntuples
+    p.set_variant(variant)
+    p.execute()
+    p.sync()
+    got = p.output()
+    ref, offs, sizes = oracle_blocks(oracle, "int2e_sph", full, mol)
+    # FP32 rounding is relative to the terms summed, not to the (possibly
+    # cancelled) result: bound it by the tuple's Schwarz factor
+    # sqrt(max|(ij|ij)|) sqrt(max|(kl|kl)|)
+    diag = [(i, j, i, j) for i in range(n) for j in range(n)]
+    dref, doffs, dsizes = oracle_blocks(oracle, "int2e_sph", diag, mol)
+    qf = np.sqrt([np.abs(dref[a:a + k]).max() for a, k in zip(doffs, dsizes)]).reshape(n, n)
+    scale = np.repeat([qf[i, j] * qf[k, l] for i, j, k, l in full], sizes)
+    err = np.abs(got - ref)
+    ratio = err[scale > 0] / scale[scale > 0]
+    # a pair whose (ij|ij) libcint's screen drops entirely (e^-cce beyond
+    # the cutoff twice) can still meet a compact ket pair: its integrals are
+    # then below e^-30 or so
+    assert np.all(err[scale == 0] <= 1e-11)
+    worst = np.argsort(err / np.maximum(scale, 1e-300))[-3:]
+    say(f"  fp32 variant {variant}: max err {err.max():.3e}, err/Schwarz bound "
+        f"max {ratio.max():.3e} p99.9 {np.quantile(ratio, 0.999):.3e}; worst "
+        + ", ".join(f"{got[w]:.6e} vs {ref[w]:.6e} (bound {scale[w]:.2e})" for w in worst))
+    # FP32 recurrences: rounding relative to the bound, amplified by the HRR
+    # cancellation of distant pairs (measured up to ~3e-4 of the bound on this
+    # basis); an indexing error would be O(1) here
+    assert ratio.max() <= 1e-2
+    assert np.quantile(ratio, 0.999) <= 1e-4
+    p.close()
+
+
+def test_mixed_plan_error_bound_cfg2(oracle):
+    """The mixed path (FP32 recurrences for tuples whose Schwarz bound is
+    below 1e-6) stays within 1e-10 absolute of the oracle."""
+    mol = M.config(2)
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, screened=(1e-12, 97),
+                  precision=capi.PREC_MIXED, mixed_threshold=1e-6)
+    assert p.ntuples_fp32 > 0
+    p.execute()
+    p.sync()
+    t, _ = p.tuples()
+    rng = np.random.default_rng(11)
+    first32 = p.ntuples - p.ntuples_fp32
+    pick = np.sort(first32 + rng.choice(p.ntuples_fp32, size=min(3000, p.ntuples_fp32),
+                                        replace=False))
+    got, _, _ = p.output_tuples(pick)
+    ref, _, _ = oracle.eval_list("int2e_sph", [tuple(int(v) for v in t[q]) for q in pick],
+                                 mol.atm, mol.bas, mol.env)
+    err = float(np.abs(got - ref).max())
+    say(f"  mixed cfg2: {p.ntuples_fp32}/{p.ntuples} tuples in FP32, max abs err {err:.3e}")
+    assert err <= 1e-10
+    p.close()
+
+
+def test_autotune_keeps_results(oracle):
+    mol = M.config(1)
+    n = mol.nbas
+    tuples = [(i, j, k, l) for i in range(n) for j in range(i + 1) for k in range(n)
+              for l in range(k + 1)]
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, tuples)
+    p.execute()
+    p.sync()
+    before = p.output()
+    p.autotune(reps=2)
+    info = p.bin_variants()
+    say("  variants " + " ".join(f"{b['variant']}/{b['nvariants']}:T{b['lanes']}P{b['prims']}"
+                                 for b in info))
+    assert all(0 <= b["variant"] < b["nvariants"] for b in info)
+    p.execute(graph=True)
+    p.sync()
+    after = p.output()
+    scale = np.abs(before).max()
+    assert np.abs(after - before).max() <= 1e-13 * scale
     p.close()
