@@ -185,10 +185,11 @@ def main():
     print(json.dumps(res), flush=True)
 
 
-def learned_roots_report(nsamples=1 << 22, epochs=2):
+def learned_roots_report(nsamples=1 << 22, epochs=8):
     """Train the MLP initializer on the device (2^22 samples, x log-uniform in
-    [1e-6, 1e3], n = 1..13) and compare, per n, MLP + 2 Newton steps (table
-    fallback when Newton does not settle below 1e-14) with the table rule."""
+    [1e-6, 1e3], n = 1..13) and compare, per n, two Newton steps started from
+    the network with the same steps started from the asymptotic rule; the
+    table rule judges both at 1e-14 relative (fallback otherwise)."""
     from cintx_amd import rys_mlp
     from oracle import oracle as O
     t0 = time.perf_counter()
@@ -198,8 +199,10 @@ def learned_roots_report(nsamples=1 << 22, epochs=2):
     x = np.exp(rng.uniform(np.log(1e-6), np.log(1e3), 1 << 16))
     per_n = {}
     for n in range(1, 10):
-        u, w, st, ms_mlp = rys_mlp.evaluate(params, n, x, newton_steps=2, init_mode=0)
-        _, _, st0, _ = rys_mlp.evaluate(params, n, x, newton_steps=2, init_mode=1)
+        u, w, st, ms_mlp, e = rys_mlp.evaluate(params, n, x, newton_steps=2, init_mode=0,
+                                               with_error=True)
+        _, _, st0, _, e0 = rys_mlp.evaluate(params, n, x, newton_steps=2, init_mode=1,
+                                            with_error=True)
         _, _, _, ms_tab = rys_mlp.evaluate(params, n, x, newton_steps=0, init_mode=2)
         err = 0.0
         for k in range(0, len(x), 2048):
@@ -207,13 +210,16 @@ def learned_roots_report(nsamples=1 << 22, epochs=2):
             err = max(err, float(np.max(np.abs(u[k] - uo) / uo)),
                       float(np.max(np.abs(w[k] - wo) / wo)))
         per_n[n] = {"settled": float(st.mean()), "settled_without_network": float(st0.mean()),
+                    "median_polish_err": float(np.median(e)),
+                    "median_polish_err_without_network": float(np.median(e0)),
                     "fallback_rate": float(1 - st.mean()), "max_rel_err_vs_oracle": err,
-                    "ms_mlp_newton": ms_mlp, "ms_table": ms_tab}
-    summ = ", ".join(f"n={n}: {v['settled']:.2f}/{v['settled_without_network']:.2f}"
+                    "ms_mlp_newton_judge": ms_mlp, "ms_table": ms_tab}
+    summ = ", ".join(f"n={n}: {v['settled']:.2f}/{v['settled_without_network']:.2f} "
+                     f"({v['median_polish_err']:.1e}/{v['median_polish_err_without_network']:.1e})"
                      for n, v in per_n.items())
     return {"train_samples": nsamples, "epochs": epochs, "final_loss": loss,
             "train_s": train_s, "eval_points": len(x), "per_nroots": per_n,
-            "summary": "settled fraction with/without network: " + summ}
+            "summary": "settled fraction (median polish error) with/without network: " + summ}
 
 
 def sample_errors(mol, p64, pmx, n):

@@ -733,9 +733,23 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
     }
     (void)rowid;
   }
-  if (a.counters && t == 0 && nsurv) {
-    atomicAdd(a.counters, (unsigned long long)nsurv);
-    atomicAdd(a.counters + 1, 1ull);
+  if (a.counters) {
+    // work counters (surviving primitive tuples, tuples with any): the wide
+    // path counts in the root-0 lane of every primitive slot, so add the
+    // lanes of a tuple first
+    unsigned int tot = nsurv;
+    if constexpr (T > 1) {
+      __shared__ unsigned int scnt[Q];
+      if (t == 0) scnt[slot] = 0;
+      tuple_sync<T>();
+      if (nsurv) atomicAdd(&scnt[slot], nsurv);
+      tuple_sync<T>();
+      tot = scnt[slot];
+    }
+    if (t == 0 && tot) {
+      atomicAdd(a.counters, (unsigned long long)tot);
+      atomicAdd(a.counters + 1, 1ull);
+    }
   }
 
   // Cartesian -> spherical, then the strided write.

@@ -38,7 +38,7 @@ hipError_t mlp_grad_device(const float* prm, const float* z, const int* nr, cons
                            int batch, float* grad, float* loss, hipStream_t st);
 hipError_t mlp_eval_device(const float* prm, const RysTable& tab, int n, const double* x,
                            int64_t nx, int steps, int mode, double* u, double* w, int* status,
-                           hipStream_t st);
+                           double* perr, hipStream_t st);
 int mlp_nparam();
 hipError_t launch_copy_probe(const double* src, double* dst, int64_t n, hipStream_t s);
 hipError_t launch_jk_digest(const double* eri, int64_t out_base, int64_t block, const int2* q,
@@ -1720,7 +1720,8 @@ int cintx_rys_mlp_grad(const float* params, const float* z, const int* nroots,
 }
 
 int cintx_rys_mlp_eval(const float* params, int n, const double* x, int64_t nx, int newton_steps,
-                       int init_mode, double* u, double* w, int* status, double* ms) {
+                       int init_mode, double* u, double* w, int* status, double* polish_err,
+                       double* ms) {
   return guarded([&] {
     if (n < 1 || n > 13) fail(CINTX_STATUS_INVALID_INPUT, "nroots must be in 1..13");
     if (!params || !x || !u || !w || !status) fail(CINTX_STATUS_NULL_POINTER, "null argument");
@@ -1729,6 +1730,7 @@ int cintx_rys_mlp_eval(const float* params, int n, const double* x, int64_t nx, 
     DevArr<float> dp(np);
     DevArr<double> dx(nx), du((size_t)nx * n), dw((size_t)nx * n);
     DevArr<int> ds(nx);
+    DevArr<double> de(polish_err ? nx : 0);
     HIPCHK(hipMemcpy(dp.p, params, np * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dx.p, x, (size_t)nx * 8, hipMemcpyHostToDevice));
     hipEvent_t e0, e1;
@@ -1736,7 +1738,7 @@ int cintx_rys_mlp_eval(const float* params, int n, const double* x, int64_t nx, 
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, C.stream));
     HIPCHK(mlp_eval_device(dp.p, C.rys, n, dx.p, nx, newton_steps, init_mode, du.p, dw.p, ds.p,
-                           C.stream));
+                           polish_err ? de.p : nullptr, C.stream));
     HIPCHK(hipEventRecord(e1, C.stream));
     HIPCHK(hipStreamSynchronize(C.stream));
     float t = 0;
@@ -1747,6 +1749,8 @@ int cintx_rys_mlp_eval(const float* params, int n, const double* x, int64_t nx, 
     HIPCHK(hipMemcpy(u, du.p, (size_t)nx * n * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(w, dw.p, (size_t)nx * n * 8, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(status, ds.p, (size_t)nx * 4, hipMemcpyDeviceToHost));
+    if (polish_err)
+      HIPCHK(hipMemcpy(polish_err, de.p, (size_t)nx * 8, hipMemcpyDeviceToHost));
   });
 }
 

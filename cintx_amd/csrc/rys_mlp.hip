@@ -1,9 +1,10 @@
 // Learned initializer for the Rys rule: a 2x64 tanh MLP mapping
 // (log x, nroots) to initial roots and weights, trained on the device by the
 // kernels below (forward + backward + Adam, FP32), followed by FP64 Newton
-// steps on the moment equations sum_i w_i s_i^m = F_m(x), m = 0..2n-1, and a
-// fallback to the Chebyshev-table rule (the plain solver) for every argument
-// whose Newton update does not settle below a relative 1e-14.
+// steps on the moment equations sum_i w_i s_i^m = F_m(x), m = 0..2n-1.  The
+// Chebyshev-table rule (the plain solver) judges the polished rule: where any
+// root u_i or weight w_i differs from it by more than 1e-14 relative, the
+// table rule is returned instead.
 //
 // Network: z = ((ln x + 3.45)/10.35, (n - 7)/6); h1 = tanh(W1 z + b1) (64);
 // h2 = tanh(W2 h1 + b2) (64); y = W3 h2 + b3 (26): y[r] ~ ln u_r and
@@ -227,13 +228,12 @@ __device__ void mlp_initial(const float* __restrict__ sp, const RysTable& tab, i
   }
 }
 
-// Newton steps on the moment equations; returns true when the last update
-// is below 1e-14 relative in every root and weight.
+// Newton steps on the moment equations; returns false when an update is not
+// finite or leaves the domain (0 < s < 1, w > 0).
 __device__ bool newton_moments(int n, double x, double* s, double* w, int steps) {
   double F[2 * kMaxRoots];
   boys_fm(2 * n - 1, x, F);
   double J[2 * kMaxRoots][2 * kMaxRoots + 1];
-  bool settled = false;
   for (int it = 0; it < steps; ++it) {
     const int N = 2 * n;
     for (int m = 0; m < N; ++m) {
@@ -271,48 +271,56 @@ __device__ bool newton_moments(int n, double x, double* s, double* w, int steps)
       for (int k = r + 1; k < N; ++k) acc -= J[r][k] * dv[k];
       dv[r] = acc / J[r][r];
     }
-    double worst = 0;
     for (int i = 0; i < n; ++i) {
       w[i] += dv[i];
       s[i] += dv[n + i];
-      worst = fmax(worst, fabs(dv[i] / w[i]));
-      worst = fmax(worst, fabs(dv[n + i] / s[i]));
     }
-    settled = worst < 1e-14;
-    if (!(worst == worst)) return false;
   }
   for (int i = 0; i < n; ++i)
     if (!(s[i] > 0.0 && s[i] < 1.0 && w[i] > 0.0)) return false;
-  return settled;
+  return true;
 }
 
 __global__ void mlp_eval_kernel(const float* __restrict__ prm, RysTable tab, int n,
                                 const double* __restrict__ x, int64_t nx, int steps, int mode,
                                 double* __restrict__ u, double* __restrict__ w,
-                                int* __restrict__ status) {
+                                int* __restrict__ status, double* __restrict__ perr) {
   __shared__ float sp[kNParam];
   for (int i = threadIdx.x; i < kNParam; i += blockDim.x) sp[i] = prm[i];
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nx) return;
   const double xi = x[i];
-  double s[kMaxRoots], ww[kMaxRoots];
-  bool ok = false;
-  if (mode != 2) {  // mode 2: the table rule alone (timing reference)
-    mlp_initial(sp, tab, n, xi, mode, s, ww);
-    ok = newton_moments(n, xi, s, ww, steps);
+  // the plain solver: table rule, u = s/q with q = 1 - s to full accuracy
+  double ut[kMaxRoots], wt[kMaxRoots];
+  for (int r = 0; r < n; ++r) {
+    double sr, qr;
+    rys_eval(tab, n, r, xi, sr, qr, wt[r]);
+    ut[r] = sr / qr;
   }
-  if (!ok) {  // plain solver
-    for (int r = 0; r < n; ++r) {
-      double q;
-      rys_eval(tab, n, r, xi, s[r], q, ww[r]);
+  bool ok = false;
+  double err = -1.0;  // mode 2: not polished
+  double s[kMaxRoots], ww[kMaxRoots];
+  if (mode != 2) {
+    mlp_initial(sp, tab, n, xi, mode, s, ww);
+    err = INFINITY;
+    if (newton_moments(n, xi, s, ww, steps)) {
+      err = 0;
+      for (int r = 0; r < n; ++r) {
+        const double ur = s[r] / (1.0 - s[r]);
+        err = fmax(err, fabs(ur - ut[r]) / ut[r]);
+        err = fmax(err, fabs(ww[r] - wt[r]) / wt[r]);
+      }
+      if (!(err == err)) err = INFINITY;
+      ok = err <= 1e-14;
     }
   }
   for (int r = 0; r < n; ++r) {
-    u[i * n + r] = s[r] / (1.0 - s[r]);
-    w[i * n + r] = ww[r];
+    u[i * n + r] = ok ? s[r] / (1.0 - s[r]) : ut[r];
+    w[i * n + r] = ok ? ww[r] : wt[r];
   }
   status[i] = ok ? 1 : 0;
+  if (perr) perr[i] = err;
 }
 
 hipError_t mlp_train_device(const RysTable& tab, float* prm, int64_t nsamples, int epochs,
@@ -394,10 +402,10 @@ hipError_t mlp_grad_device(const float* prm, const float* z, const int* nr, cons
 
 hipError_t mlp_eval_device(const float* prm, const RysTable& tab, int n, const double* x,
                            int64_t nx, int steps, int mode, double* u, double* w, int* status,
-                           hipStream_t st) {
+                           double* perr, hipStream_t st) {
   if (nx <= 0) return hipSuccess;
   hipLaunchKernelGGL(mlp_eval_kernel, dim3((unsigned)((nx + 127) / 128)), dim3(128), 0, st, prm,
-                     tab, n, x, nx, steps, mode, u, w, status);
+                     tab, n, x, nx, steps, mode, u, w, status, perr);
   return hipGetLastError();
 }
 

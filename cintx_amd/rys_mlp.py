@@ -32,7 +32,7 @@ def _lib():
         L.cintx_rys_mlp_grad.argtypes = [f32p, f32p, capi.IP, f32p, ctypes.c_int, f32p, f32p]
         L.cintx_rys_mlp_eval.argtypes = [f32p, ctypes.c_int, capi.DP, ctypes.c_int64,
                                          ctypes.c_int, ctypes.c_int, capi.DP, capi.DP, capi.IP,
-                                         capi.DP]
+                                         capi.DP, capi.DP]
         L._mlp_ready = True
     return L
 
@@ -116,7 +116,7 @@ def device_grad(p, z, n, tgt):
     return g, loss
 
 
-def train(params=None, nsamples=1 << 22, epochs=8, batch=4096, lr=3e-3, seed=12345):
+def train(params=None, nsamples=1 << 22, epochs=8, batch=1024, lr=3e-3, seed=12345):
     L = _lib()
     p = init_params(0) if params is None else np.array(params, dtype=np.float32)
     loss = ctypes.c_double()
@@ -125,16 +125,21 @@ def train(params=None, nsamples=1 << 22, epochs=8, batch=4096, lr=3e-3, seed=123
     return p, loss.value
 
 
-def evaluate(params, n, x, newton_steps=2, init_mode=0):
-    """(u, w, status, device_ms): status 1 = Newton settled (< 1e-14 rel),
-    0 = table fallback."""
+def evaluate(params, n, x, newton_steps=2, init_mode=0, with_error=False):
+    """(u, w, status, device_ms[, polish_err]): status 1 = the polished rule
+    is within 1e-14 relative of the table rule (and is returned), 0 = table
+    fallback; polish_err = that largest relative difference."""
     L = _lib()
     p = np.ascontiguousarray(params, dtype=np.float32)
     x = np.ascontiguousarray(x, dtype=np.float64)
     u = np.zeros(len(x) * n)
     w = np.zeros(len(x) * n)
     st = np.zeros(len(x), dtype=np.int32)
+    err = np.zeros(len(x))
     ms = ctypes.c_double()
     capi._check(L.cintx_rys_mlp_eval(_fp(p), n, capi._dp(x), len(x), newton_steps, init_mode,
-                                     capi._dp(u), capi._dp(w), capi._ip(st), ctypes.byref(ms)))
+                                     capi._dp(u), capi._dp(w), capi._ip(st), capi._dp(err),
+                                     ctypes.byref(ms)))
+    if with_error:
+        return u.reshape(-1, n), w.reshape(-1, n), st, ms.value, err
     return u.reshape(-1, n), w.reshape(-1, n), st, ms.value

@@ -223,11 +223,16 @@ int cintx_rys_mlp_train(float *params, int64_t nsamples, int epochs, int batch, 
 int cintx_rys_mlp_grad(const float *params, const float *z, const int *nroots,
                        const float *targets, int batch, float *grad, float *loss);
 /* init_mode 0: network, 1: asymptotic rule (the initializer without the
- * network), 2: the table rule alone (no Newton; timing reference).  status[i] = 1 when Newton settled below 1e-14 relative, 0 when
- * the table rule was substituted.  ms: device time of the evaluation. */
+ * network), 2: the table rule alone (no Newton; timing reference).  After
+ * newton_steps FP64 Newton steps on the moment equations the table rule
+ * (the plain solver) judges the polished rule: status[i] = 1 when every
+ * root u_r and weight w_r is within 1e-14 relative of it (the polished rule
+ * is returned), 0 when the table rule was substituted.  polish_err
+ * (nullable): that largest relative difference (inf when Newton left the
+ * domain, -1 for init_mode 2).  ms: device time of the evaluation. */
 int cintx_rys_mlp_eval(const float *params, int n, const double *x, int64_t nx,
                        int newton_steps, int init_mode, double *u, double *w, int *status,
-                       double *ms);
+                       double *polish_err, double *ms);
 
 /* A plain device copy of nelem doubles, reps times (average ms per copy):
  * the reference kernel for calibrating hardware byte counters. */

@@ -334,3 +334,24 @@ def test_autotune_keeps_results(oracle):
     scale = np.abs(before).max()
     assert np.abs(after - before).max() <= 1e-13 * scale
     p.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_work_counters_match_oracle_model(oracle, variant):
+    """The device work counters (surviving primitive tuples after libcint's
+    screen, algorithmic FLOPs) equal the oracle's count for every launch
+    shape."""
+    mol = M.config(1)
+    n = mol.nbas
+    tuples = [(i, j, k, l) for i in range(n) for j in range(i + 1) for k in range(n)
+              for l in range(k + 1)]
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, tuples)
+    p.set_variant(variant)
+    prim, roots, flops = p.work()
+    ref = np.sum([oracle.work_model("int2e_sph", t, mol.atm, mol.bas, mol.env) for t in tuples],
+                 axis=0)
+    say(f"  variant {variant}: prim {prim:.0f} vs {ref[0]:.0f}, flops {flops:.4e} vs {ref[2]:.4e}")
+    assert prim == ref[0]
+    assert roots == ref[1]
+    assert abs(flops - ref[2]) <= 1e-9 * ref[2]
+    p.close()

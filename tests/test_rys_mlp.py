@@ -50,20 +50,30 @@ def test_device_gradient_matches_cpu_backprop():
 
 @pytest.mark.gpu
 def test_trained_network_gains_over_newton_without_it(oracle):
-    p, loss = R.train(nsamples=1 << 18, epochs=6, batch=4096, lr=3e-3)
+    """Ablation: the same two FP64 Newton steps on the moment equations,
+    started from the trained network or from the asymptotic (large-x) rule;
+    the table rule judges both at 1e-14 relative and is returned when the
+    polished rule misses."""
+    p, loss = R.train(nsamples=1 << 20, epochs=8, batch=1024, lr=3e-3)
+    print("training loss", loss)
     assert loss < 0.05
     rng = np.random.default_rng(4)
     x = np.exp(rng.uniform(np.log(1e-3), np.log(60.0), 400))
     report = {}
     for n in (1, 2, 3, 4, 5):
-        u, w, st, _ = R.evaluate(p, n, x, newton_steps=2, init_mode=0)
-        u0, w0, st0, _ = R.evaluate(p, n, x, newton_steps=2, init_mode=1)
-        report[n] = (st.mean(), st0.mean())
+        u, w, st, _, err = R.evaluate(p, n, x, newton_steps=2, init_mode=0, with_error=True)
+        _, _, st0, _, err0 = R.evaluate(p, n, x, newton_steps=2, init_mode=1, with_error=True)
+        med, med0 = np.median(err), np.median(err0)
+        report[n] = (float(st.mean()), float(st0.mean()), float(med), float(med0))
         # whatever the path, the returned rule is the oracle's to 1e-12
         for k in range(0, len(x), 37):
             uo, wo = oracle.rys_roots(n, x[k])
             assert np.max(np.abs(u[k] - uo) / uo) < 1e-12
             assert np.max(np.abs(w[k] - wo) / wo) < 1e-12
-    print("settled fraction (network, asymptotic init):", report)
-    # the network's initial rule lets two Newton steps settle more often
-    assert sum(v[0] for v in report.values()) > sum(v[1] for v in report.values())
+        # one root: Newton converges from either start; from two roots on,
+        # Newton from the asymptotic rule mostly leaves the domain below
+        # x ~ 35 while the network's start stays within reach of the rule
+        assert st.mean() >= st0.mean() - 0.02, (n, st.mean(), st0.mean())
+        if n >= 2:
+            assert med < 1e-4 and med < 1e-2 * med0, (n, med, med0)
+    print("settled (network, asymptotic), median polish error (network, asymptotic):", report)
