@@ -14,7 +14,9 @@
 //     strides (so a canonicalised tuple lands in the requested orientation).
 // Arithmetic follows libcint: per-root coefficients g2e.c:4523-4543, VRR
 // g2e.c:272-411, HRR g2e.c:428-690 (ibase/kbase choice g2e.c:102-150), gout
-// cint2e.c:961-1180, nabla_i g2e.c:4574-4620, screening cint2e.c:199-236.
+// cint2e.c:961-1180, nabla_i g2e.c:4574-4620, screening cint2e.c:199-236
+// (a bra primitive pair with cce >= expcutoff has a zero prefactor in
+// libcint's pair data, optimizer.c:317-341, and is skipped here).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -446,7 +448,9 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
       const bool iin = ip < np_ij;
       const int iidx = boff + (iin ? ip : 0);
       const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
-      const bool ok = klok && iin && cceij <= eijcut;
+      // (libcint zeroes the prefactor of a bra pair with cce >= expcutoff,
+      // optimizer.c:317-341, so it contributes nothing)
+      const bool ok = klok && iin && cceij <= eijcut && cceij < a.expcutoff;
       if (!ok) continue;
       PrimTuple p;
       p.aij = a.aij[iidx];
@@ -588,7 +592,8 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
         const int kidx = koff + (kin ? kp : 0), iidx = boff + (kin ? ip : 0);
         const double ccekl = a.three_center ? 0.0 : a.cce[kidx] - a.lkl_ceil * logd_kl;
         const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
-        const bool ok = active && kin && ccekl <= a.expcutoff && cceij <= a.expcutoff - ccekl;
+        const bool ok = active && kin && ccekl <= a.expcutoff &&
+                        cceij <= a.expcutoff - ccekl && cceij < a.expcutoff;
         double* uwp = suw + pp * 3 * NR;
         double* sc = sps + pp * NSC;
         if (ok) {

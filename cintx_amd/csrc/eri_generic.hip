@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
       for (int ip = 0; ip < np_ij; ++ip) {
         const int iidx = boff + ip;
         const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
-        if (cceij > eijcut) continue;
+        if (cceij > eijcut || cceij >= a.expcutoff) continue;  // zero prefactor (libcint pair data)
         const double aij = a.aij[iidx];
         const double Pv[3] = {a.px[iidx], a.py[iidx], a.pz[iidx]};
         double dx[3], prb[3], qrk[3], rr = 0;

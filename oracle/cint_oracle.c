@@ -951,8 +951,10 @@ void orc_work_model(int api, int *shls, int *atm, int natm, int *bas, int nbas, 
                     if (ccekl > expcutoff) continue;
                     eijcutoff = expcutoff - ccekl;
                 }
+                /* counted: primitive tuples that pass the screen and carry a
+                 * non-zero prefactor (set_pairdata zeroes cce >= expcutoff) */
                 for (int n = 0; n < s[0].nprim * s[1].nprim; n++)
-                    if (pd[n].cce <= eijcutoff) nprimq++;
+                    if (pd[n].cce <= eijcutoff && pd[n].cce < expcutoff) nprimq++;
             }
     }
     free(pd);
