@@ -76,9 +76,10 @@ def main():
         for plan, label in ((p64, "fp64"), (pmx, "mixed")):
             t0 = time.time()
             plan.autotune(reps=2)
-            tuned[label] = ["{}{}{}{}{}:T{}P{}".format(*b["class"], "df"[b["precision"]],
-                                                         v["lanes"], v["prims"])
-                            for b, v in zip(plan.bin_stats(), plan.bin_variants())]
+            tuned[label] = ["{}{}{}{}{}:T{}P{}{}".format(
+                *b["class"], "df"[b["precision"]], v["lanes"], v["prims"],
+                "(fp64 kernel)" if v["variant"] >= 4 else "")
+                for b, v in zip(plan.bin_stats(), plan.bin_variants())]
             log(f"autotune {label}: {time.time() - t0:.1f}s")
 
     def timed(plan, label):
@@ -109,20 +110,28 @@ def main():
     # dominant kernel: time every class bin with HIP events on the plan's
     # stream (averaged over --steps executions)
     per = {}
+    shapes = p64.bin_variants()
     for _ in range(args.steps):
         p64.execute_timed()
-        for b in p64.bin_stats():
-            per.setdefault(b["class"], []).append(b)
+        for b, v in zip(p64.bin_stats(), shapes):
+            per.setdefault(b["class"], []).append(dict(b, **v))
     bins = []
     for cls, lst in per.items():
         bins.append({"class": cls, "ms": float(np.mean([b["ms"] for b in lst])),
                      "flops": lst[0]["flops"], "tuples": lst[0]["tuples"],
-                     "prim_tuples": lst[0]["prim_tuples"], "generic": lst[0]["generic"]})
+                     "prim_tuples": lst[0]["prim_tuples"], "generic": lst[0]["generic"],
+                     "lanes": lst[0]["lanes"], "prims": lst[0]["prims"],
+                     "tuples_per_block": lst[0]["tuples_per_block"]})
     bins.sort(key=lambda b: -b["ms"])
     dom = bins[0]
     achieved = dom["flops"] / (dom["ms"] / 1e3) / 1e12
-    kname = "eri_class_kernel<{}, {}, {}, {}, 0, double>".format(*dom["class"])
-    log(f"dominant kernel {kname}: {dom['ms']:.2f} ms/launch, {achieved:.3f} TFLOP/s")
+    kname = "eri_class_kernel<{}, {}, {}, {}, 0, double, {}, {}>".format(*dom["class"],
+                                                                      dom["lanes"], dom["prims"])
+    q = dom["tuples_per_block"]
+    dom_grid = -(-dom["tuples"] // q) * q * dom["lanes"]  # work-items of the launch
+    log(f"dominant kernel {kname}: {dom['ms']:.2f} ms/launch, {achieved:.3f} TFLOP/s, "
+        f"grid {dom_grid} work-items")
+    traffic = profiled_traffic(kname, dom["tuples"])
     value = pmx.ntuples / (msmx / 1e3)
     fp64_rate = p64.ntuples / step_s
     out_bytes = p64.nelem * 8
@@ -165,8 +174,13 @@ def main():
         "fp64_max_abs_err_vs_oracle": err64,
         "mixed_max_abs_err_vs_oracle": errmx,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "traffic_note": "HBM bytes per launch of the dominant kernel: calibrated "
+                                     "FETCH_SIZE + WRITE_SIZE per tuple from the committed "
+                                     "rocprofv3 counter runs x this launch's tuples (null when "
+                                     "the dominant kernel differs from the profiled one)",
                      "kernel": kname, "kernel_ms_per_launch": dom["ms"],
+                     "kernel_grid_workitems": dom_grid,
                      "kernel_flops_per_launch": dom["flops"],
                      "kernel_tuples_per_launch": dom["tuples"],
                      "note": "FP64 VALU/LDS-bound class kernel (FP64 vector and matrix dense "
@@ -183,6 +197,22 @@ def main():
         "learned_roots": learned,
     }
     print(json.dumps(res), flush=True)
+
+
+def profiled_traffic(kname, tuples):
+    """HBM bytes per launch of `kname` from the counter summary in profiles/
+    (calibrated FETCH_SIZE + WRITE_SIZE per tuple), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "int2e_sph_256atom_hbm_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_tuple "):
+                name, per = line[len("traffic_bytes_per_tuple "):].rsplit(" ", 1)
+                if name.strip() == kname:
+                    return float(per) * tuples
+    except OSError:
+        pass
+    return None
 
 
 def learned_roots_report(nsamples=1 << 22, epochs=8):

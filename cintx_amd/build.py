@@ -67,6 +67,31 @@ def _p2(v):
     return p
 
 
+def register_chunks(deriv, prec, ls):
+    """Row chunks for the one-lane path on a class too large for one pass:
+    the g tensor of one root (3*G4 values) and a chunk's FP64 accumulators
+    should fit about 130 doubles of registers (VGPRs plus AGPRs at one wave
+    per SIMD; measured: beyond that the compiler spills tens of KB to scratch
+    and the kernel runs 100x slower).  Every chunk is an unrolled copy of the primitive loop, so at
+    most 3 (more thrash the instruction cache: 6 chunks ran 100x slower);
+    None when that is not enough."""
+    li, lj, lk, ll = ls
+    nc = [(l + 1) * (l + 2) // 2 for l in ls]
+    g4 = (li + deriv + 1) * (lj + 1) * (lk + 1) * (ll + 1)
+    gdoubles = 3 * g4 * (8 if prec == 0 else 4) / 8
+    nb = nc[0] if deriv else nc[0] * nc[1]
+    nrow = nc[0] * nc[1] * nc[2] * nc[3] // nb
+    if gdoubles > 72:
+        return None
+    # FP32 keeps a per-primitive-tuple partial sum beside each FP64 accumulator
+    per_acc = 1.0 if prec == 0 else 1.5
+    for nch in (2, 3):
+        rpc = -(-nrow // nch)
+        if gdoubles + per_acc * rpc * nb * (3 if deriv else 1) <= 130:
+            return nch
+    return None
+
+
 def variants(deriv, prec, ls):
     """Launch-shape candidates (lanes per tuple T, primitive tuples per step P)
     of a class kernel; the first is the untuned default.  Mirrors the sizes of
@@ -95,6 +120,10 @@ def variants(deriv, prec, ls):
         cands.insert(0, (1, 1))
     elif ne <= 81:
         cands.insert(3, (1, 1))
+    else:
+        nch = register_chunks(deriv, prec, ls)
+        if nch:
+            cands.insert(3, (1, nch))
     out = []
     for c in cands:
         if c not in out:
@@ -102,21 +131,36 @@ def variants(deriv, prec, ls):
     return out[:4]
 
 
-def generate(nparts=24):
+# Translation units of the chunked one-lane variants are compiled with larger
+# alloca-to-register limits: their g tensor (up to 81 doubles) is indexed with
+# compile-time constants only after full unrolling, and the default limits
+# leave it in scratch memory.
+CHUNK_FLAGS = ["-mllvm", "-amdgpu-promote-alloca-to-vector-limit=4096",
+               "-mllvm", "-amdgpu-promote-alloca-to-vector-max-regs=512",
+               "-mllvm", "-amdgpu-promote-alloca-to-vector-vgpr-ratio=1"]
+
+
+def generate(nparts=24, nchunkparts=12):
     os.makedirs(GEN, exist_ok=True)
     classes = class_sets()
-    parts = [classes[i::nparts] for i in range(nparts)]
+    plain, chunked = [], []
+    for deriv, prec, ls in classes:
+        for v, (tt, pp) in enumerate(variants(deriv, prec, ls)):
+            (chunked if (tt == 1 and pp > 1) else plain).append((deriv, prec, ls, v, tt, pp))
+    parts = [plain[i::nparts] for i in range(nparts)]
+    parts += [chunked[i::nchunkparts] for i in range(nchunkparts)]
+    nparts = len(parts)
     files = []
     for p, cls in enumerate(parts):
         lines = ['#include "eri_device.h"', "namespace cx {",
                  f"void register_part_{p}(Registry& reg) {{"]
-        for deriv, prec, (a, b, c, d) in cls:
+        for deriv, prec, (a, b, c, d), v, tt, pp in cls:
             real = "double" if prec == 0 else "float"
-            for v, (tt, pp) in enumerate(variants(deriv, prec, (a, b, c, d))):
-                lines.append(f"  reg.k[{deriv}][{prec}][{a}][{b}][{c}][{d}][{v}] = "
-                             f"class_info<{a}, {b}, {c}, {d}, {deriv}, {real}, {tt}, {pp}>();")
+            lines.append(f"  reg.k[{deriv}][{prec}][{a}][{b}][{c}][{d}][{v}] = "
+                         f"class_info<{a}, {b}, {c}, {d}, {deriv}, {real}, {tt}, {pp}>();")
         lines += ["}", "}  // namespace cx", ""]
-        path = os.path.join(GEN, f"inst_{p:02d}.hip")
+        stem = f"inst_{p:02d}" if p < len(parts) - nchunkparts else f"inst_reg_{p:02d}"
+        path = os.path.join(GEN, stem + ".hip")
         text = "\n".join(lines)
         if not os.path.exists(path) or open(path).read() != text:
             with open(path, "w") as f:
@@ -160,6 +204,9 @@ def build(jobs=None, verbose=True):
             os.path.join(CSRC, "rys_kernels.hip"), os.path.join(CSRC, "eri_generic.hip"),
             os.path.join(CSRC, "rys_mlp.hip"), os.path.join(CSRC, "jk_digest.hip")] + gen
     extra = {os.path.join(CSRC, "rys_table.cpp"): ["-fopenmp"]}
+    for g in gen:
+        if os.path.basename(g).startswith("inst_reg_"):
+            extra[g] = CHUNK_FLAGS
     jobs = jobs or min(16, os.cpu_count() or 4)
     objs = []
     errors = []

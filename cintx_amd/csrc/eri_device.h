@@ -296,6 +296,12 @@ __device__ __forceinline__ void c2s_reg(const double* in, double* out,
   }
 }
 
+// One real-solid-harmonic coefficient (s and p: identity).
+__device__ __forceinline__ double c2s_coef(const double* __restrict__ c2s, int l, int m, int c) {
+  if (l < 2) return m == c ? 1.0 : 0.0;
+  return c2s[((size_t)l * kC2sM + m) * kC2sC + c];
+}
+
 // Lanes per tuple (TT), primitive tuples per step (PP) and tuples per
 // workgroup of one class-kernel variant.  The candidate (TT, PP) pairs of a
 // class are generated by the build and chosen per plan bin by timing them
@@ -321,6 +327,11 @@ struct ClassShape {
   static constexpr int NROW = NF / NB;
   static constexpr int T = TT;
   static constexpr int P = TT == 1 ? 1 : PP;
+  // one-lane path: the rows of the block are done in NCH chunks (PP), each a
+  // full pass over the primitive tuples, so that a chunk's accumulators and
+  // the g tensor of one root fit in registers
+  static constexpr int NCH = TT == 1 ? PP : 1;
+  static constexpr int RPC = (NROW + NCH - 1) / NCH;  // rows per chunk
   static constexpr int G = T >= NROW ? T / NROW : 1;               // groups per row
   static constexpr int RPL = T >= NROW ? 1 : (NROW + T - 1) / T;   // rows per lane
   static constexpr int NFT = (NF + T - 1) / T;                     // T == 1 path
@@ -348,8 +359,12 @@ struct ClassShape {
 };
 
 template <int LI, int LJ, int LK, int LL, int DERIV, typename R, int TT, int PP>
+#ifndef CX_MIN_WAVES
+#define CX_MIN_WAVES 1
+#endif
 __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T *
-                                   ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::template Q<R>()))
+                                   ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::template Q<R>()),
+                                  CX_MIN_WAVES)
     eri_class_kernel(EriArgs a) {
   using S = ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>;
   static_assert(S::template Q<R>() * S::template lds_per_tuple<R>() <= 64 * 1024, "LDS");
@@ -374,7 +389,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
   constexpr int UD = REG ? 0 : S::template union_doubles<R>();
   constexpr int LDS_D = REG ? 1 : Q * (UD + S::P * (3 * NR + S::NSC));
   __shared__ double lds[LDS_D];
-  constexpr int ACC = REG ? NFT * NCOMP : RPL * NB * NCOMP;
+  constexpr int ACC = REG ? S::RPC * NB * NCOMP : RPL * NB * NCOMP;
   double acc[ACC];
 #pragma unroll
   for (int kk = 0; kk < ACC; ++kk) acc[kk] = 0.0;
@@ -422,9 +437,12 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
   double* b1 = nullptr;
   if constexpr (REG) {
   // One lane per tuple, everything in registers: per primitive tuple and
-  // root, the three axis chains (3*G4 values) and then every element.
+  // root, the three axis chains (3*G4 values) and then every element of the
+  // current chunk of rows; the chunk's block is transformed and written (or
+  // added to the output, for chunks after the first) before the next chunk.
   (void)lds;
   constexpr bool F64 = sizeof(R) == sizeof(double);
+  constexpr int NCH = S::NCH, RPC = S::RPC, CE = RPC * NB;  // elements per chunk
   R g[3 * G4];
   R part[F64 ? 1 : ACC];  // FP32: this primitive tuple's sums over roots
   auto tgt = [&](int i) -> R& {
@@ -433,7 +451,12 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
     else
       return part[i];
   };
-
+#pragma unroll
+  for (int ch = 0; ch < NCH; ++ch) {
+  if (ch > 0) {
+#pragma unroll
+    for (int kk = 0; kk < ACC; ++kk) acc[kk] = 0.0;
+  }
   for (int kp = 0; kp < mkl; ++kp) {
     const bool kin = kp < np_kl;
     const int kidx = koff + (kin ? kp : 0);
@@ -474,7 +497,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
       const double fac1 = 4.0 * p.hij * hkl * sqrt(p.inv_sum) * fac;
       const double x = p.a0 * rr;
       const R ai2 = DERIV ? (R)(-2.0 * a.ai[iidx]) : (R)0;
-      ++nsurv;
+      if (ch == 0) ++nsurv;
       if constexpr (!F64) {
 #pragma unroll
         for (int kk = 0; kk < ACC; ++kk) part[kk] = 0;
@@ -489,9 +512,12 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
           build_chain<LIc, LJ, LK, LL, R, 1>(g, axis, 0, sr, qr, wz, p.inv_sum, p.hij, p.hkl,
                                              p.aij, p.akl, p.dx[axis], p.prb[axis], p.qrk[axis],
                                              axis == 2 ? 0.0 : 1.0, rij[axis], rkl[axis]);
-        // every element; all g indices are compile-time constants
+        // the chunk's elements; all g indices are compile-time constants
 #pragma unroll
-        for (int e = 0; e < NF; ++e) {
+        for (int el = 0; el < CE; ++el) {
+          const int row = ch * RPC + el / NB;
+          if (row >= NROW) continue;
+          const int e = row * NB + el % NB;
           int ix, iy, iz, jx, jy, jz, kx, ky, kz, lx, ly, lz;
           cart_decode(LI, e % NFI, ix, iy, iz);
           cart_decode(LJ, (e / NFI) % NFJ, jx, jy, jz);
@@ -504,7 +530,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
           const R* gy = g + G4;
           const R* gz = g + 2 * G4;
           if constexpr (!DERIV) {
-            tgt(e) += gx[px] * gy[py] * gz[pz];
+            tgt(el) += gx[px] * gy[py] * gz[pz];
           } else {
             constexpr int SI = DJ * DK * DL;  // stride of the i power
             R fx = ai2 * gx[px + SI];
@@ -513,9 +539,9 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
             if (ix > 0) fx += (R)ix * gx[px - SI];
             if (iy > 0) fy += (R)iy * gy[py - SI];
             if (iz > 0) fz += (R)iz * gz[pz - SI];
-            tgt(e) += fx * gy[py] * gz[pz];
-            tgt(NF + e) += gx[px] * fy * gz[pz];
-            tgt(2 * NF + e) += gx[px] * gy[py] * fz;
+            tgt(el) += fx * gy[py] * gz[pz];
+            tgt(CE + el) += gx[px] * fy * gz[pz];
+            tgt(2 * CE + el) += gx[px] * gy[py] * fz;
           }
         }
       }
@@ -524,6 +550,61 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
         for (int kk = 0; kk < ACC; ++kk) acc[kk] += (double)part[kk];
       }
     }
+  }
+  // cart -> sph of the chunk, in registers: the row's own axes (i, j; i only
+  // for nabla_i rows) per row, then the row axes as a weighted sum over the
+  // chunk's rows, added to what earlier chunks wrote
+  if (active) {
+    constexpr int NSI = nsph(LI), NSJ = nsph(LJ), NSK = nsph(LK), NSL = nsph(LL);
+    constexpr int NSB = DERIV ? NSI : NSI * NSJ;  // spherical size of a row
+#pragma unroll
+    for (int comp = 0; comp < NCOMP; ++comp) {
+      double d[RPC * NSB];
+#pragma unroll
+      for (int rl = 0; rl < RPC; ++rl) {
+        const double* src = acc + comp * CE + rl * NB;
+        if constexpr (DERIV) {
+          c2s_reg<LI, 1, 1>(src, d + rl * NSB, a.c2s);
+        } else {
+          double t1[NSI * NFJ];
+          c2s_reg<LI, 1, NFJ>(src, t1, a.c2s);
+          c2s_reg<LJ, NSI, 1>(t1, d + rl * NSB, a.c2s);
+        }
+      }
+      double* o = a.out + a.out_base + q * a.block + comp * a.st_comp;
+#pragma unroll
+      for (int ml = 0; ml < NSL; ++ml)
+#pragma unroll
+        for (int mk = 0; mk < NSK; ++mk)
+#pragma unroll
+          for (int mj = 0; mj < (DERIV ? NSJ : 1); ++mj)
+#pragma unroll
+            for (int mb = 0; mb < NSB; ++mb) {
+              double v = 0;
+#pragma unroll
+              for (int rl = 0; rl < RPC; ++rl) {
+                const int row = ch * RPC + rl;
+                if (row >= NROW) continue;
+                int fj = 0, fk, fl;
+                if constexpr (DERIV) {
+                  fj = row % NFJ;
+                  fk = (row / NFJ) % NFK;
+                  fl = row / (NFJ * NFK);
+                } else {
+                  fk = row % NFK;
+                  fl = row / NFK;
+                }
+                double cf = c2s_coef(a.c2s, LK, mk, fk) * c2s_coef(a.c2s, LL, ml, fl);
+                if constexpr (DERIV) cf *= c2s_coef(a.c2s, LJ, mj, fj);
+                v += cf * d[rl * NSB + mb];
+              }
+              const int mi = DERIV ? mb : mb % NSI;
+              const int mjj = DERIV ? mj : mb / NSI;
+              double* dst = o + mi * a.st[0] + mjj * a.st[1] + mk * a.st[2] + ml * a.st[3];
+              *dst = ch == 0 ? v : *dst + v;
+            }
+    }
+  }
   }
   } else {
     // Wide tuples: P primitive tuples per step.  Phase 1 (P*NR root slots):
@@ -662,12 +743,6 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
                                           sc[1], sc[2], sc[3], sc[4], sc[7], sc[10], sc[13], 0.0,
                                           rij[2], rkl[2]);
       }
-      // nabla_i: a_i of every slot, read before the barrier for the same reason
-      R ai2v[DERIV ? P : 1];
-      if constexpr (DERIV) {
-#pragma unroll
-        for (int pp = 0; pp < P; ++pp) ai2v[pp] = (R)(-2.0 * sps[pp * NSC + 14]);
-      }
       tuple_sync<T>();
       if constexpr (!F64) {
 #pragma unroll
@@ -677,10 +752,10 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
       for (int k = 0; k < RPL; ++k) {
         if (rowid[k] < 0) continue;
         const int bx = rb[k] & 1023, by = (rb[k] >> 10) & 1023, bz = rb[k] >> 20;
-        // P is at most 8; unrolled so ai2v[pp] stays in registers
-#pragma unroll
-        for (int pp = 0; pp < P; ++pp) {
-          if (pp % G != grp % G || grp >= G) continue;
+        // not unrolled: unrolling lets the compiler hoist the LDS loads of
+        // every slot and costs occupancy
+#pragma unroll 1
+        for (int pp = grp; pp < P; pp += G) {
           if (!((okmask >> pp) & 1u)) continue;
           const R* gpp = sg + pp * GS;
           if constexpr (!DERIV) {
@@ -703,7 +778,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
               }
             }
           } else {
-            const R ai2 = ai2v[pp];
+            const R ai2 = (R)(-2.0 * sps[pp * NSC + 14]);
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
               R xv[DI], yv[DI], zv[DI];
@@ -735,6 +810,9 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
 #pragma unroll
         for (int kk = 0; kk < ACC; ++kk) acc[kk] += (double)svf[kk];
       }
+      // a tuple wider than a wavefront: phase 1 of the next step rewrites the
+      // slot scalars that phase 3 reads (nabla_i)
+      if constexpr (DERIV && T > 64) __syncthreads();
     }
     (void)rowid;
   }
@@ -761,26 +839,8 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
   constexpr int NSI = nsph(LI), NSJ = nsph(LJ), NSK = nsph(LK), NSL = nsph(LL);
   constexpr int NS = NSI * NSJ * NSK * NSL;
   if constexpr (REG) {
-    // in registers, axis by axis
-#pragma unroll
-    for (int comp = 0; comp < NCOMP; ++comp) {
-      double t1[NSI * NFJ * NFK * NFL], t2[NSI * NSJ * NFK * NFL], t3[NSI * NSJ * NSK * NFL];
-      double t4[NS];
-      c2s_reg<LI, 1, NFJ * NFK * NFL>(acc + comp * NF, t1, a.c2s);
-      c2s_reg<LJ, NSI, NFK * NFL>(t1, t2, a.c2s);
-      c2s_reg<LK, NSI * NSJ, NFL>(t2, t3, a.c2s);
-      c2s_reg<LL, NSI * NSJ * NSK, 1>(t3, t4, a.c2s);
-      if (active) {
-        double* o = a.out + a.out_base + q * a.block + comp * a.st_comp;
-#pragma unroll
-        for (int e = 0; e < NS; ++e) {
-          const int mi = e % NSI, mj = (e / NSI) % NSJ, mk = (e / (NSI * NSJ)) % NSK,
-                    ml = e / (NSI * NSJ * NSK);
-          o[mi * a.st[0] + mj * a.st[1] + mk * a.st[2] + ml * a.st[3]] = t4[e];
-        }
-      }
-    }
-  } else {
+    // written chunk by chunk above
+    } else {
   tuple_sync<T>();
   for (int comp = 0; comp < NCOMP; ++comp) {
     // lane groups add their partial rows in turn
@@ -851,7 +911,7 @@ template <int LI, int LJ, int LK, int LL, int DERIV, typename R, int TT, int PP>
 KernelInfo class_info() {
   using S = ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>;
   return KernelInfo{&launch_class<LI, LJ, LK, LL, DERIV, R, TT, PP>, S::template Q<R>(), S::T,
-                    S::P};
+                    S::T == 1 ? S::NCH : S::P};
 }
 
 }  // namespace cx

@@ -16,7 +16,7 @@ struct KernelInfo {
   LaunchFn fn;
   int tuples_per_block;  // Q
   int lanes_per_tuple;   // T
-  int prims_per_step;    // P
+  int prims_per_step;    // P (one lane per tuple: row chunks)
 };
 
 struct Registry {

@@ -665,6 +665,20 @@ static const KernelInfo* bin_kernels(const cintx_plan* P, const Bin& b) {
   return ctx().reg->k[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
 }
 
+// Variant v of a bin: v < kVariants picks the bin's own precision; an FP32
+// bin of a mixed plan may also run the FP64 instantiation (v - kVariants),
+// which the autotuner picks where FP32 does not pay (FP32 without packed math
+// issues at the FP64 rate on CDNA4, so it only helps through registers and
+// LDS bytes).
+static const KernelInfo& variant_info(const cintx_plan* P, const Bin& b, int v) {
+  static const KernelInfo none{};
+  if (v < 0 || v >= 2 * kVariants) return none;
+  if (v < kVariants) return bin_kernels(P, b)[v];
+  if (b.prec != 1) return none;
+  const FamilyInfo& f = kFam[P->family];
+  return ctx().reg->k[f.deriv][0][b.l[0]][b.l[1]][b.l[2]][b.l[3]][v - kVariants];
+}
+
 static signed char& tuned_slot(const cintx_plan* P, const Bin& b) {
   const FamilyInfo& f = kFam[P->family];
   return ctx().tuned[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
@@ -682,7 +696,7 @@ static void plan_check_kernels(cintx_plan* P) {
     const KernelInfo* ks = bin_kernels(P, b);
     b.generic = (ks[0].fn == nullptr);
     const int v = tuned_slot(P, b);
-    b.variant = (v > 0 && v < kVariants && ks[v].fn) ? v : 0;
+    b.variant = (v > 0 && variant_info(P, b, v).fn) ? v : 0;
   }
 }
 
@@ -691,7 +705,7 @@ static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count) {
   const FamilyInfo& f = kFam[P->family];
   const double common_base = (M_PI * M_PI * M_PI) * 2.0 / std::sqrt(M_PI);
   Bin& b = P->bins[i];
-  const KernelInfo& k = bin_kernels(P, b)[b.variant];
+  const KernelInfo& k = variant_info(P, b, b.variant);
   EriArgs a;
   // (generic bins run the runtime-class kernel below)
   std::memset(&a, 0, sizeof a);
@@ -1394,23 +1408,27 @@ int cintx_plan_autotune(cintx_plan* plan, int reps) {
     for (size_t i = 0; i < plan->bins.size(); ++i) {
       Bin& b = plan->bins[i];
       if (b.generic || b.q.empty()) continue;
-      const KernelInfo* ks = bin_kernels(plan, b);
       int best = b.variant;
       double best_ms = 1e300;
-      for (int v = 0; v < kVariants; ++v) {
-        if (!ks[v].fn) continue;
+      for (int v = 0; v < 2 * kVariants; ++v) {
+        if (!variant_info(plan, b, v).fn) continue;
         b.variant = v;
-        plan_launch_bin(plan, i, plan->d_out, false);  // warm-up
-        double vmin = 1e300;
-        for (int r = 0; r < reps; ++r) {
+        auto timed = [&]() {
           HIPCHK(hipEventRecord(e0, C.stream));
           plan_launch_bin(plan, i, plan->d_out, false);
           HIPCHK(hipEventRecord(e1, C.stream));
           HIPCHK(hipEventSynchronize(e1));
           float ms = 0;
           HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-          vmin = std::min(vmin, (double)ms);
-        }
+          return (double)ms;
+        };
+        // two warm-up launches, timed too (the first may load the code
+        // object): a variant still 4x slower than the best so far is not
+        // timed further
+        double vmin = timed();
+        vmin = std::min(vmin, timed());
+        if (vmin > 4.0 * best_ms) continue;
+        for (int r = 0; r < reps; ++r) vmin = std::min(vmin, timed());
         if (vmin < best_ms) {
           best_ms = vmin;
           best = v;
@@ -1443,10 +1461,9 @@ int cintx_plan_bin_variants(const cintx_plan* plan, int64_t nmax, int* variant, 
     int64_t n = std::min<int64_t>(nmax, (int64_t)plan->bins.size());
     for (int64_t i = 0; i < n; ++i) {
       const Bin& b = plan->bins[i];
-      const KernelInfo* ks = bin_kernels(plan, b);
       int nv = 0;
-      for (int v = 0; v < kVariants; ++v) nv += ks[v].fn ? 1 : 0;
-      const KernelInfo& k = ks[b.variant];
+      for (int v = 0; v < 2 * kVariants; ++v) nv += variant_info(plan, b, v).fn ? 1 : 0;
+      const KernelInfo& k = variant_info(plan, b, b.variant);
       if (variant) variant[i] = b.generic ? -1 : b.variant;
       if (nvariants) nvariants[i] = nv;
       if (lanes) lanes[i] = b.generic ? 256 : k.lanes_per_tuple;

@@ -188,11 +188,14 @@ int cintx_plan_work(cintx_plan *plan, double *prim_tuples, double *root_evals, d
  * (min of `reps` launches after one warm-up, HIP events), keeps the fastest
  * per bin and records it for the process, so later plans of the same class
  * start from it.  Results of all variants agree to rounding (summation order
- * of the primitive loop differs).  cintx_plan_set_variant forces one variant
+ * of the primitive loop differs).  The FP32 bins of a mixed plan also try the
+ * FP64 instantiations (variants 4..7 = FP64 variant 0..3): FP32 without packed
+ * math issues at the FP64 rate on CDNA4 and wins only where it saves
+ * registers or LDS bytes.  cintx_plan_set_variant forces one variant (0..3)
  * on every bin that has it (variant 0 otherwise); cintx_plan_bin_variants
  * reports per bin: variant in use (-1: runtime-class kernel), variants
- * available, lanes per tuple, primitive tuples per step, tuples per
- * workgroup. */
+ * available, lanes per tuple, primitive tuples per step (for one lane per
+ * tuple: row chunks), tuples per workgroup. */
 int cintx_plan_autotune(cintx_plan *plan, int reps);
 int cintx_plan_set_variant(cintx_plan *plan, int variant);
 int cintx_plan_bin_variants(const cintx_plan *plan, int64_t nmax, int *variant, int *nvariants,
