@@ -78,7 +78,7 @@ def main():
             plan.autotune(reps=2)
             tuned[label] = ["{}{}{}{}{}:T{}P{}{}".format(
                 *b["class"], "df"[b["precision"]], v["lanes"], v["prims"],
-                "(fp64 kernel)" if v["variant"] >= 4 else "")
+                "(fp64 kernel)" if v["variant"] >= capi.NVARIANTS else "")
                 for b, v in zip(plan.bin_stats(), plan.bin_variants())]
             log(f"autotune {label}: {time.time() - t0:.1f}s")
 

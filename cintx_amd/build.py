@@ -114,6 +114,13 @@ def variants(deriv, prec, ls):
     p0 = max(1, min(8, wide // (3 * nr)))
     pw = min(plds, max(p0, min(groups, 8)))
     cands = [(wide, pw), (wide, min(plds, 2 * pw, 8)), (wide, 1), (narrow, 1)]
+    # twice and half the lanes (more primitive slots in flight, or more rows
+    # per lane)
+    w2 = min(256, 2 * wide)
+    plds2 = max(1, (60 * 1024) // (gs * 8 + (3 * nr + 16) * 8))
+    cands.append((w2, max(1, min(8, w2 // (3 * nr), plds2))))
+    if wide >= 8:
+        cands.append((wide // 2, 1))
     # one lane per tuple, everything in registers: the default for small
     # classes, a candidate up to 81 elements
     if ne <= 27:
@@ -128,7 +135,9 @@ def variants(deriv, prec, ls):
     for c in cands:
         if c not in out:
             out.append(c)
-    return out[:4]
+    if ne <= 27:  # the register path won every such class by 3-10x
+        return out[:2]
+    return out[:6]
 
 
 # Translation units of the chunked one-lane variants are compiled with larger

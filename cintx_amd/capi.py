@@ -31,6 +31,8 @@ INT3C2E_SPH = 40
 FAMILY = {"int2e_sph": 0, "int2e_ip1_sph": 1, "int3c2e_sph": 2, "int3c2e_ip1_sph": 3}
 API_OF_FAMILY = {0: INT2E_SPH, 1: INT2E_IP1_SPH, 2: INT3C2E_SPH, 3: INT3C2E_IP1_SPH}
 PREC_FP64 = 0
+# launch-shape variants compiled per class (kVariants in csrc/launch.h)
+NVARIANTS = 6
 PREC_MIXED = 1
 
 STATUS = {

@@ -49,6 +49,9 @@ struct EriArgs {
   const double* ai;     // exponent of the first shell (nabla_i)
   const double* cce;    // screening key without the l-dependent term
   const double* hinv;   // 0.5 / a_ij
+  // the same per primitive pair as one 64-byte record (a kernel lane reads
+  // one cache line per primitive pair): aij, Px, Py, Pz, kfac, hinv, cce, ai
+  const double* prim;
   // per pair
   const int* pair_off;  // first prim pair of the pair
   const int* pair_np;   // primitive pairs of the pair

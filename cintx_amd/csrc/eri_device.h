@@ -49,10 +49,11 @@ __host__ __device__ constexpr void cart_decode(int l, int c, int& lx, int& ly, i
 // the weight (rys_roots.c:57-123 regimes: Chebyshev table below 35+5n,
 // asymptotic Hermite rule above).  The table holds, per root and interval,
 // the fit of the smaller of s and q; a fitted q is stored negated.
-__device__ __forceinline__ void rys_eval(const RysTable& tab, int n, int r, double x, double& s,
-                                         double& q, double& w) {
+__device__ __forceinline__ void rys_eval_at(const double* __restrict__ cn,
+                                            const double* __restrict__ large, int n, int r,
+                                            double x, double& s, double& q, double& w) {
   if (!(x < rys_largex(n))) {
-    const double* L = tab.large + n * (n - 1);
+    const double* L = large + n * (n - 1);
     s = L[r] / x;
     q = 1.0 - s;
     w = L[n + r] / sqrt(x);
@@ -61,7 +62,7 @@ __device__ __forceinline__ void rys_eval(const RysTable& tab, int n, int r, doub
   int iv = (int)x;
   double t = 2.0 * (x - (double)iv) - 1.0;
   double t2 = t + t;
-  const double* cu = tab.coef + tab.off[n] + ((size_t)iv * 2 * n + r) * kRysNc;
+  const double* cu = cn + ((size_t)iv * 2 * n + r) * kRysNc;
   const double* cw = cu + (size_t)n * kRysNc;
   double bu1 = 0, bu2 = 0, bw1 = 0, bw2 = 0;
 #pragma unroll
@@ -78,6 +79,16 @@ __device__ __forceinline__ void rys_eval(const RysTable& tab, int n, int r, doub
   s = v < 0 ? 1.0 - small : small;
   q = v < 0 ? small : 1.0 - small;
   w = fma(t, bw1, cw[0] - bw2);
+}
+
+__device__ __forceinline__ void rys_eval(const RysTable& tab, int n, int r, double x, double& s,
+                                         double& q, double& w) {
+  rys_eval_at(tab.coef + tab.off[n], tab.large, n, r, x, s, q, w);
+}
+
+// Doubles of the table block of n roots (intervals x 2n series x coefficients).
+__host__ __device__ constexpr int rys_block_doubles(int n) {
+  return (int)rys_largex(n) * 2 * n * kRysNc;
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; remap
@@ -443,6 +454,17 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
   (void)lds;
   constexpr bool F64 = sizeof(R) == sizeof(double);
   constexpr int NCH = S::NCH, RPC = S::RPC, CE = RPC * NB;  // elements per chunk
+  // up to 3 roots the table block of this n (<= 29 KB) is copied to LDS once
+  // per workgroup: the coefficient reads of every root then hit LDS instead
+  // of scattered L2 lines
+  constexpr bool LTAB = NR <= 3;
+  __shared__ double stab[LTAB ? rys_block_doubles(NR) : 1];
+  const double* coefn = a.rys.coef + a.rys.off[NR];
+  if constexpr (LTAB) {
+    for (int i = threadIdx.x; i < rys_block_doubles(NR); i += blockDim.x) stab[i] = coefn[i];
+    __syncthreads();
+    coefn = stab;
+  }
   R g[3 * G4];
   R part[F64 ? 1 : ACC];  // FP32: this primitive tuple's sums over roots
   auto tgt = [&](int i) -> R& {
@@ -460,25 +482,27 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
   for (int kp = 0; kp < mkl; ++kp) {
     const bool kin = kp < np_kl;
     const int kidx = koff + (kin ? kp : 0);
-    const double akl = a.aij[kidx];
-    const double Qv[3] = {a.px[kidx], a.py[kidx], a.pz[kidx]};
-    const double kkl = a.kfac[kidx];
-    const double hkl = a.hinv[kidx];
-    const double ccekl = a.three_center ? 0.0 : a.cce[kidx] - a.lkl_ceil * logd_kl;
+    const double* pk = a.prim + 8 * (size_t)kidx;
+    const double akl = pk[0];
+    const double Qv[3] = {pk[1], pk[2], pk[3]};
+    const double kkl = pk[4];
+    const double hkl = pk[5];
+    const double ccekl = a.three_center ? 0.0 : pk[6] - a.lkl_ceil * logd_kl;
     const bool klok = active && kin && ccekl <= a.expcutoff;
     const double eijcut = a.expcutoff - ccekl;
     for (int ip = 0; ip < mij; ++ip) {
       const bool iin = ip < np_ij;
       const int iidx = boff + (iin ? ip : 0);
-      const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
+      const double* pi = a.prim + 8 * (size_t)iidx;
+      const double cceij = pi[6] - a.lij_ceil * logd_ij;
       // (libcint zeroes the prefactor of a bra pair with cce >= expcutoff,
       // optimizer.c:317-341, so it contributes nothing)
       const bool ok = klok && iin && cceij <= eijcut && cceij < a.expcutoff;
       if (!ok) continue;
       PrimTuple p;
-      p.aij = a.aij[iidx];
+      p.aij = pi[0];
       p.akl = akl;
-      const double Pv[3] = {a.px[iidx], a.py[iidx], a.pz[iidx]};
+      const double Pv[3] = {pi[1], pi[2], pi[3]};
       double rr = 0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
@@ -488,15 +512,15 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
         p.qrk[d] = Qv[d] - (KBASE ? rk[d] : rl[d]);
       }
       p.inv_sum = 1.0 / (p.aij + akl);
-      p.hij = a.hinv[iidx];
+      p.hij = pi[5];
       p.hkl = hkl;
       p.a1 = p.aij * akl;
       p.a0 = p.a1 * p.inv_sum;
       // sqrt(a0/a1^3) = 1/(a1 sqrt(aij+akl)) = 4 hij hkl sqrt(inv_sum)
-      const double fac = a.common * a.kfac[iidx] * kkl;
+      const double fac = a.common * pi[4] * kkl;
       const double fac1 = 4.0 * p.hij * hkl * sqrt(p.inv_sum) * fac;
       const double x = p.a0 * rr;
-      const R ai2 = DERIV ? (R)(-2.0 * a.ai[iidx]) : (R)0;
+      const R ai2 = DERIV ? (R)(-2.0 * pi[7]) : (R)0;
       if (ch == 0) ++nsurv;
       if constexpr (!F64) {
 #pragma unroll
@@ -505,7 +529,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
 #pragma unroll 1
       for (int r = 0; r < NR; ++r) {
         double sr, qr, w;
-        rys_eval(a.rys, NR, r, x, sr, qr, w);
+        rys_eval_at(coefn, a.rys.large, NR, r, x, sr, qr, w);
         const double wz = w * fac1;
 #pragma unroll
         for (int axis = 0; axis < 3; ++axis)
@@ -671,16 +695,18 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
         const int kp = k / mij, ip = k - kp * mij;
         const bool kin = k < npq && kp < np_kl && ip < np_ij;
         const int kidx = koff + (kin ? kp : 0), iidx = boff + (kin ? ip : 0);
-        const double ccekl = a.three_center ? 0.0 : a.cce[kidx] - a.lkl_ceil * logd_kl;
-        const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
+        const double* pk = a.prim + 8 * (size_t)kidx;
+        const double* pi = a.prim + 8 * (size_t)iidx;
+        const double ccekl = a.three_center ? 0.0 : pk[6] - a.lkl_ceil * logd_kl;
+        const double cceij = pi[6] - a.lij_ceil * logd_ij;
         const bool ok = active && kin && ccekl <= a.expcutoff &&
                         cceij <= a.expcutoff - ccekl && cceij < a.expcutoff;
         double* uwp = suw + pp * 3 * NR;
         double* sc = sps + pp * NSC;
         if (ok) {
-          const double aij = a.aij[iidx], akl = a.aij[kidx];
-          const double Pv[3] = {a.px[iidx], a.py[iidx], a.pz[iidx]};
-          const double Qv[3] = {a.px[kidx], a.py[kidx], a.pz[kidx]};
+          const double aij = pi[0], akl = pk[0];
+          const double Pv[3] = {pi[1], pi[2], pi[3]};
+          const double Qv[3] = {pk[1], pk[2], pk[3]};
           double dx[3], rr = 0;
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
@@ -688,8 +714,8 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
             rr += dx[d] * dx[d];
           }
           const double inv_sum = 1.0 / (aij + akl);
-          const double hij = a.hinv[iidx], hkl = a.hinv[kidx];
-          const double fac = a.common * a.kfac[iidx] * a.kfac[kidx];
+          const double hij = pi[5], hkl = pk[5];
+          const double fac = a.common * pi[4] * pk[4];
           const double fac1 = 4.0 * hij * hkl * sqrt(inv_sum) * fac;
           const double x = aij * akl * inv_sum * rr;
           double sr, qr, w;
@@ -709,7 +735,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
               sc[8 + d] = Pv[d] - (IBASE ? ri[d] : rj[d]);
               sc[11 + d] = Qv[d] - (KBASE ? rk[d] : rl[d]);
             }
-            sc[14] = a.ai[iidx];
+            sc[14] = pi[7];
             ++nsurv;
           }
         }

@@ -10,7 +10,7 @@ namespace cx {
 
 typedef hipError_t (*LaunchFn)(const EriArgs&, hipStream_t);
 
-constexpr int kVariants = 4;
+constexpr int kVariants = 6;
 
 struct KernelInfo {
   LaunchFn fn;

@@ -450,7 +450,8 @@ struct cintx_plan {
   double* graph_out = nullptr;
   // device pointers of the pair SoA
   double *d_aij = nullptr, *d_px = nullptr, *d_py = nullptr, *d_pz = nullptr, *d_kfac = nullptr,
-         *d_ai = nullptr, *d_cce = nullptr, *d_geo = nullptr, *d_hinv = nullptr;
+         *d_ai = nullptr, *d_cce = nullptr, *d_geo = nullptr, *d_hinv = nullptr,
+         *d_rec = nullptr;
   int* d_poff = nullptr;
   int* d_pnp = nullptr;
 };
@@ -604,7 +605,8 @@ static void plan_upload(cintx_plan* P) {
     return o;
   };
   size_t o_aij = take(npp * 8), o_px = take(npp * 8), o_py = take(npp * 8), o_pz = take(npp * 8),
-         o_k = take(npp * 8), o_ai = take(npp * 8), o_cce = take(npp * 8), o_h = take(npp * 8);
+         o_k = take(npp * 8), o_ai = take(npp * 8), o_cce = take(npp * 8), o_h = take(npp * 8),
+         o_rec = take(npp * 64);
   size_t o_poff = take(npair * 4), o_pnp = take(npair * 4), o_geo = take(npair * 64);
   std::vector<size_t> o_q(P->bins.size());
   for (size_t i = 0; i < P->bins.size(); ++i) o_q[i] = take(P->bins[i].q.size() * sizeof(int2));
@@ -632,6 +634,21 @@ static void plan_upload(cintx_plan* P) {
   put(o_ai, S.ai.data(), npp * 8);
   put(o_cce, S.cce.data(), npp * 8);
   put(o_h, S.hinv.data(), npp * 8);
+  {
+    std::vector<double> rec(npp * 8);
+    for (size_t k = 0; k < npp; ++k) {
+      double* r = rec.data() + 8 * k;
+      r[0] = S.aij[k];
+      r[1] = S.px[k];
+      r[2] = S.py[k];
+      r[3] = S.pz[k];
+      r[4] = S.kfac[k];
+      r[5] = S.hinv[k];
+      r[6] = S.cce[k];
+      r[7] = S.ai[k];
+    }
+    put(o_rec, rec.data(), npp * 64);
+  }
   put(o_poff, S.off.data(), npair * 4);
   put(o_pnp, S.np.data(), npair * 4);
   put(o_geo, S.geo.data(), npair * 64);
@@ -650,6 +667,7 @@ static void plan_upload(cintx_plan* P) {
   P->d_ai = (double*)(P->blob + o_ai);
   P->d_cce = (double*)(P->blob + o_cce);
   P->d_hinv = (double*)(P->blob + o_h);
+  P->d_rec = (double*)(P->blob + o_rec);
   P->d_poff = (int*)(P->blob + o_poff);
   P->d_pnp = (int*)(P->blob + o_pnp);
   P->d_geo = (double*)(P->blob + o_geo);
@@ -717,6 +735,7 @@ static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count) {
   a.ai = P->d_ai;
   a.cce = P->d_cce;
   a.hinv = P->d_hinv;
+  a.prim = P->d_rec;
   a.pair_off = P->d_poff;
   a.pair_np = P->d_pnp;
   a.pair_geo = P->d_geo;

@@ -183,15 +183,15 @@ int cintx_plan_bin_stats(const cintx_plan *plan, int64_t nmax, int *cls, int64_t
 int cintx_plan_work(cintx_plan *plan, double *prim_tuples, double *root_evals, double *flops);
 /* Launch-shape autotuner (no counterpart in the reference, whose executor
  * has a single launch shape; crates/cintx-runtime/src/dispatch.rs:187-195).
- * Every class has up to four compiled variants (lanes per tuple, primitive
+ * Every class has up to six compiled variants (lanes per tuple, primitive
  * tuples per step); cintx_plan_autotune times each on the plan's own bin
  * (min of `reps` launches after one warm-up, HIP events), keeps the fastest
  * per bin and records it for the process, so later plans of the same class
  * start from it.  Results of all variants agree to rounding (summation order
  * of the primitive loop differs).  The FP32 bins of a mixed plan also try the
- * FP64 instantiations (variants 4..7 = FP64 variant 0..3): FP32 without packed
+ * FP64 instantiations (variants 6..11 = FP64 variant 0..5): FP32 without packed
  * math issues at the FP64 rate on CDNA4 and wins only where it saves
- * registers or LDS bytes.  cintx_plan_set_variant forces one variant (0..3)
+ * registers or LDS bytes.  cintx_plan_set_variant forces one variant (0..5)
  * on every bin that has it (variant 0 otherwise); cintx_plan_bin_variants
  * reports per bin: variant in use (-1: runtime-class kernel), variants
  * available, lanes per tuple, primitive tuples per step (for one lane per

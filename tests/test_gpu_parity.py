@@ -240,7 +240,7 @@ def test_jk_build_cfg1(oracle):
     p.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", range(capi.NVARIANTS))
 def test_every_launch_variant(oracle, variant):
     """Every compiled launch shape (lanes per tuple, primitive tuples per
     step) of every class meets the same parity bar; classes with fewer
@@ -336,7 +336,7 @@ def test_autotune_keeps_results(oracle):
     p.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", range(capi.NVARIANTS))
 def test_work_counters_match_oracle_model(oracle, variant):
     """The device work counters (surviving primitive tuples after libcint's
     screen, algorithmic FLOPs) equal the oracle's count for every launch
