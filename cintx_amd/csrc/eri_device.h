@@ -755,19 +755,19 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
         const int axis = c / NR, r = c - axis * NR;
         const double* sc = sps + pp * NSC;
         const double* uwp = suw + pp * 3 * NR;
-        R* gp = sg + pp * GS;
-        if (axis == 0)
-          build_chain<LIc, LJ, LK, LL, R>(gp, 0, r, uwp[r], uwp[2 * NR + r], uwp[NR + r], sc[0],
-                                          sc[1], sc[2], sc[3], sc[4], sc[5], sc[8], sc[11], 1.0,
-                                          rij[0], rkl[0]);
-        else if (axis == 1)
-          build_chain<LIc, LJ, LK, LL, R>(gp, 1, r, uwp[r], uwp[2 * NR + r], uwp[NR + r], sc[0],
-                                          sc[1], sc[2], sc[3], sc[4], sc[6], sc[9], sc[12], 1.0,
-                                          rij[1], rkl[1]);
-        else
-          build_chain<LIc, LJ, LK, LL, R>(gp, 2, r, uwp[r], uwp[2 * NR + r], uwp[NR + r], sc[0],
-                                          sc[1], sc[2], sc[3], sc[4], sc[7], sc[10], sc[13], 0.0,
-                                          rij[2], rkl[2]);
+        // the whole chain (VRR and both transfers) runs in registers; LDS
+        // only receives the G4 finished values, so no transfer step waits
+        // on an LDS round trip
+        R loc[G4];
+        const double rij_a = axis == 0 ? rij[0] : (axis == 1 ? rij[1] : rij[2]);
+        const double rkl_a = axis == 0 ? rkl[0] : (axis == 1 ? rkl[1] : rkl[2]);
+        build_chain<LIc, LJ, LK, LL, R, 1>(loc, 0, 0, uwp[r], uwp[2 * NR + r], uwp[NR + r], sc[0],
+                                           sc[1], sc[2], sc[3], sc[4], sc[5 + axis],
+                                           sc[8 + axis], sc[11 + axis], axis == 2 ? 0.0 : 1.0,
+                                           rij_a, rkl_a);
+        R* gp = sg + pp * GS + (size_t)axis * G4 * NR + r;
+#pragma unroll
+        for (int k = 0; k < G4; ++k) gp[k * NR] = loc[k];
       }
       tuple_sync<T>();
       if constexpr (!F64) {
