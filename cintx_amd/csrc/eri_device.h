@@ -526,11 +526,27 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
 #pragma unroll
         for (int kk = 0; kk < ACC; ++kk) part[kk] = 0;
       }
+      // small blocks: all roots first (their Clenshaw recurrences interleave),
+      // then one root at a time through the chains and the elements; larger
+      // blocks evaluate each root in turn to keep registers for g
+      constexpr bool ROOTS_FIRST = 3 * G4 * (int)sizeof(R) / 8 + ACC <= 64;
+      double rs[ROOTS_FIRST ? NR : 1], rq[ROOTS_FIRST ? NR : 1], rw[ROOTS_FIRST ? NR : 1];
+      if constexpr (ROOTS_FIRST) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+          rys_eval_at(coefn, a.rys.large, NR, r, x, rs[r], rq[r], rw[r]);
+      }
 #pragma unroll 1
       for (int r = 0; r < NR; ++r) {
-        double sr, qr, w;
-        rys_eval_at(coefn, a.rys.large, NR, r, x, sr, qr, w);
-        const double wz = w * fac1;
+        double sr, qr, wr;
+        if constexpr (ROOTS_FIRST) {
+          sr = rs[r];
+          qr = rq[r];
+          wr = rw[r];
+        } else {
+          rys_eval_at(coefn, a.rys.large, NR, r, x, sr, qr, wr);
+        }
+        const double wz = wr * fac1;
 #pragma unroll
         for (int axis = 0; axis < 3; ++axis)
           build_chain<LIc, LJ, LK, LL, R, 1>(g, axis, 0, sr, qr, wz, p.inv_sum, p.hij, p.hkl,
