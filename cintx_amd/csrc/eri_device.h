@@ -620,6 +620,31 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
           for (int mj = 0; mj < (DERIV ? NSJ : 1); ++mj)
 #pragma unroll
             for (int mb = 0; mb < NSB; ++mb) {
+              // s and p row axes map rows to outputs one to one: an output
+              // only takes rows whose components match there, so a chunk
+              // writes just the outputs its rows reach (all compile-time),
+              // storing where no earlier chunk did and adding otherwise
+              auto reaches = [&](int row) {
+                int fj = 0, fk, fl;
+                if constexpr (DERIV) {
+                  fj = row % NFJ;
+                  fk = (row / NFJ) % NFK;
+                  fl = row / (NFJ * NFK);
+                } else {
+                  fk = row % NFK;
+                  fl = row / NFK;
+                }
+                return (LK >= 2 || fk == mk) && (LL >= 2 || fl == ml) &&
+                       (!DERIV || LJ >= 2 || fj == mj);
+              };
+              bool mine = false, earlier = false;
+#pragma unroll
+              for (int row = 0; row < NROW; ++row) {
+                if (!reaches(row)) continue;
+                if (row / RPC == ch) mine = true;
+                if (row / RPC < ch) earlier = true;
+              }
+              if (!mine) continue;
               double v = 0;
 #pragma unroll
               for (int rl = 0; rl < RPC; ++rl) {
@@ -641,7 +666,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
               const int mi = DERIV ? mb : mb % NSI;
               const int mjj = DERIV ? mj : mb / NSI;
               double* dst = o + mi * a.st[0] + mjj * a.st[1] + mk * a.st[2] + ml * a.st[3];
-              *dst = ch == 0 ? v : *dst + v;
+              *dst = earlier ? *dst + v : v;
             }
     }
   }
