@@ -47,6 +47,8 @@ def main():
                     help="skip training/evaluating the learned Rys initializer")
     ap.add_argument("--no-autotune", action="store_true",
                     help="run every class with its default launch shape")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the int3c2e (aux-basis lattice) and int2e/ip1 (s..g basis) lines")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the class kernels directly instead of replaying a hipGraph")
     args = ap.parse_args()
@@ -141,6 +143,13 @@ def main():
         learned = learned_roots_report()
         log(f"learned roots: {learned['summary']}")
 
+    secondary = None
+    if not args.no_secondary:
+        secondary = secondary_configs()
+        for k, v in secondary.items():
+            log(f"{k}: {v['per_s']:.3e} {v['unit']}, {v['ms']:.1f} ms, "
+                f"{v['tflops']:.2f} TFLOP/s, max err {v['max_abs_err_vs_oracle']:.2e}")
+
     cpu = None
     if not args.no_cpu:
         t64, _ = p64.tuples_cached()
@@ -194,9 +203,69 @@ def main():
                        "tuples": b["tuples"]} for b in bins],
         "cpu_baseline": cpu,
         "launch_shapes": tuned,
+        "secondary": secondary,
         "learned_roots": learned,
     }
     print(json.dumps(res), flush=True)
+
+
+def secondary_configs():
+    """Throughput of the other families on their own synthetic inputs (FP64,
+    autotuned, graph replay): int3c2e_sph and int3c2e_sph mixed on the 64-atom
+    lattice with the 6s5p4d3f1g aux basis (every 8th orbital pair i >= j with
+    every aux shell), int2e_sph (unique quartets, Schwarz 1e-12, bra stride 64)
+    and int2e_ip1_sph (every 8th of those quartets) on the 32-atom s..g
+    basis (runtime-class kernel for f and g classes).  Max abs error against
+    the oracle on a 300-tuple sample."""
+    from cintx_amd import capi, molecule
+    from oracle import oracle as O
+    out = {}
+
+    def run(key, plan, family, mol, unit, reps=3):
+        plan.autotune(reps=1)
+        plan.execute(graph=True)
+        plan.sync()
+        ms = []
+        for _ in range(reps):
+            plan.execute(graph=True)
+            plan.sync()
+            ms.append(plan.last_ms())
+        _, _, flops = plan.work()
+        t, _ = plan.tuples_cached()
+        rng = np.random.default_rng(5)
+        pick = np.sort(rng.choice(len(t), size=min(300, len(t)), replace=False))
+        got, _, _ = plan.output_tuples(pick)
+        arity = 3 if family.startswith("int3c2e") else 4
+        ref, _, _ = O.eval_list(family, [tuple(int(v) for v in t[q][:arity]) for q in pick],
+                                mol.atm, mol.bas, mol.env)
+        m = float(np.mean(ms))
+        out[key] = {"family": family, "tuples": plan.ntuples, "ms": m,
+                    "per_s": plan.ntuples / (m / 1e3), "unit": unit,
+                    "tflops": flops / (m / 1e3) / 1e12,
+                    "fp32_tuples": plan.ntuples_fp32,
+                    "max_abs_err_vs_oracle": float(np.abs(got - ref).max())}
+        plan.close()
+
+    m4 = molecule.config(4)
+    pairs = [(i, j) for i in range(m4.aux_start) for j in range(i + 1)][::8]
+    trip = np.array([(i, j, k, 0) for i, j in pairs for k in range(m4.aux_start, m4.nbas)],
+                    dtype=np.int32)
+    run("int3c2e_sph_aux_lattice", capi.Plan("int3c2e_sph", m4.atm, m4.bas, m4.env, trip),
+        "int3c2e_sph", m4, "triples/s")
+    # an explicit list has no Schwarz bounds, so the mixed plan runs every
+    # triple with the FP32 recurrences: its error is not held to the 1e-10
+    # of the screened int2e mixed path
+    run("int3c2e_sph_aux_lattice_all_fp32",
+        capi.Plan("int3c2e_sph", m4.atm, m4.bas, m4.env, trip, precision=capi.PREC_MIXED),
+        "int3c2e_sph", m4, "triples/s")
+    m5 = molecule.config(5)
+    p5 = capi.Plan("int2e_sph", m5.atm, m5.bas, m5.env, screened=(1e-12, 64))
+    t5, _ = p5.tuples_cached()
+    t5 = np.array(t5[::8], dtype=np.int32)
+    run("int2e_sph_sg_basis", p5, "int2e_sph", m5, "quartets/s")
+    run("int2e_ip1_sph_sg_basis", capi.Plan("int2e_ip1_sph", m5.atm, m5.bas, m5.env, t5),
+        "int2e_ip1_sph", m5, "quartets/s")
+    return out
 
 
 def profiled_traffic(kname, tuples):

@@ -242,9 +242,13 @@ class Plan:
                                               float(thr), int(stride), precision,
                                               float(mixed_threshold))
         else:
-            t = np.zeros((len(tuples), 4), dtype=np.int32)
-            for i, s in enumerate(tuples):
-                t[i, :len(s)] = s
+            if isinstance(tuples, np.ndarray) and tuples.ndim == 2:
+                t = np.zeros((len(tuples), 4), dtype=np.int32)
+                t[:, :tuples.shape[1]] = tuples
+            else:
+                t = np.zeros((len(tuples), 4), dtype=np.int32)
+                for i, s in enumerate(tuples):
+                    t[i, :len(s)] = s
             st = L.cintx_plan_create(ctypes.byref(h), FAMILY[family], _ip(self._atm),
                                      len(self._atm), _ip(self._bas), len(self._bas),
                                      _dp(self._env), _ip(t), len(t), precision,

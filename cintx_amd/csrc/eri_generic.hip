@@ -1,10 +1,12 @@
 // Runtime-class fallback kernel: any (li, lj, lk, ll) with l <= 4 (+1 for
 // nabla_i), used for the classes that have no compile-time instantiation
-// (high angular momentum: (ff|..), (gg|gg), the cfg-5 workload).  One shell
-// tuple per 256-lane workgroup, grid-stride over tuples; the g tensor, the
-// Cartesian accumulators and the cart->sph buffers live in a per-workgroup
-// global scratch slice (L2/MALL resident), so any class fits.  Arithmetic is
-// the same as eri_class_kernel (libcint g2e.c:272-690, 4425-4620).
+// (high angular momentum: (ff|..), (gg|gg), the s..g basis workload).  One
+// shell tuple per 256-lane workgroup, grid-stride over tuples.  The g tensor
+// lives in LDS when it fits in 56 KB, else in a per-workgroup global scratch
+// slice; the Cartesian accumulators live in registers (16 elements per lane)
+// up to 4096 elements, else in the slice; the cart->sph buffers are in the
+// slice.  Arithmetic is the same as eri_class_kernel (libcint g2e.c:272-690,
+// 4425-4620).
 #include "eri_device.h"
 
 namespace cx {
@@ -73,19 +75,55 @@ __device__ void generic_chain(double* __restrict__ g, const GenShape& S, int axi
   }
 }
 
+// GLDS: the g tensor in (dynamic) LDS instead of the global slice.
+// AREG: every lane keeps its at most kGenAcc elements (t + 256 k) and their
+// packed g positions in registers instead of global accumulators.
+constexpr int kGenAcc = 16;
+constexpr size_t kGenLdsBytes = 56 * 1024;  // dynamic LDS for g (plus ~0.5 KB static)
+
+template <bool GLDS, bool AREG>
 __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
                                                           int64_t per_block) {
+  extern __shared__ double gl[];
   const int t = threadIdx.x;
   const int T = blockDim.x;
   const int NR = S.nr;
-  double* g = a.scratch + (size_t)blockIdx.x * per_block;
-  double* acc = g + (size_t)3 * S.g4 * NR;
+  double* slice = a.scratch + (size_t)blockIdx.x * per_block;
+  double* g = GLDS ? gl : slice;
+  double* acc = slice + (size_t)3 * S.g4 * NR;
   double* b0 = acc + (size_t)S.nf * S.ncomp;
   double* b1 = b0 + S.nf;
   __shared__ double uw[3 * kMaxRoots];
   __shared__ unsigned long long cnt[2];
   const int nfi = ncart(S.l[0]), nfj = ncart(S.l[1]), nfk = ncart(S.l[2]), nfl = ncart(S.l[3]);
   const int SI = S.dj * S.dk * S.dl;
+  // this lane's elements: packed g positions x | y << 10 | z << 20 and the
+  // i powers (nabla_i) x | y << 4 | z << 8
+  int pos[AREG ? kGenAcc : 1], ipw[AREG ? kGenAcc : 1];
+  double racc[AREG ? 3 * kGenAcc : 1];
+  if constexpr (AREG) {
+#pragma unroll
+    for (int k = 0; k < kGenAcc; ++k) {
+      const int e = t + k * 256;
+      pos[k] = 0;
+      ipw[k] = 0;
+      if (e < S.nf) {
+        int fi = e % nfi, e1 = e / nfi;
+        int fj = e1 % nfj, e2 = e1 / nfj;
+        int fk = e2 % nfk, fl = e2 / nfk;
+        int ix, iy, iz, jx, jy, jz, kx, ky, kz, lx, ly, lz;
+        cart_decode(S.l[0], fi, ix, iy, iz);
+        cart_decode(S.l[1], fj, jx, jy, jz);
+        cart_decode(S.l[2], fk, kx, ky, kz);
+        cart_decode(S.l[3], fl, lx, ly, lz);
+        const int px = ((ix * S.dj + jx) * S.dk + kx) * S.dl + lx;
+        const int py = ((iy * S.dj + jy) * S.dk + ky) * S.dl + ly;
+        const int pz = ((iz * S.dj + jz) * S.dk + kz) * S.dl + lz;
+        pos[k] = px | (py << 10) | (pz << 20);
+        ipw[k] = ix | (iy << 4) | (iz << 8);
+      }
+    }
+  }
   for (int64_t q = blockIdx.x; q < a.nq; q += gridDim.x) {
     const int2 pq = a.quartets[q];
     const int boff = a.pair_off[pq.x], koff = a.pair_off[pq.y];
@@ -100,24 +138,32 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
       rij[d] = S.ibase ? ri[d] - rj[d] : rj[d] - ri[d];
       rkl[d] = S.kbase ? rk[d] - rl[d] : rl[d] - rk[d];
     }
-    for (int e = t; e < S.nf * S.ncomp; e += T) acc[e] = 0.0;
+    if constexpr (AREG) {
+#pragma unroll
+      for (int k = 0; k < 3 * kGenAcc; ++k) racc[k] = 0.0;
+    } else {
+      for (int e = t; e < S.nf * S.ncomp; e += T) acc[e] = 0.0;
+    }
     if (t == 0) cnt[0] = 0;
     __syncthreads();
     const double logd_ij = gb[6], logd_kl = gk[6];
     const int np_ij = a.pair_np[pq.x], np_kl = a.pair_np[pq.y];
     for (int kp = 0; kp < np_kl; ++kp) {
       const int kidx = koff + kp;
-      const double akl = a.aij[kidx];
-      const double Qv[3] = {a.px[kidx], a.py[kidx], a.pz[kidx]};
-      const double ccekl = a.three_center ? 0.0 : a.cce[kidx] - a.lkl_ceil * logd_kl;
+      const double* pk = a.prim + 8 * (size_t)kidx;
+      const double akl = pk[0];
+      const double Qv[3] = {pk[1], pk[2], pk[3]};
+      const double ccekl = a.three_center ? 0.0 : pk[6] - a.lkl_ceil * logd_kl;
       if (ccekl > a.expcutoff) continue;  // uniform across the workgroup
       const double eijcut = a.expcutoff - ccekl;
       for (int ip = 0; ip < np_ij; ++ip) {
         const int iidx = boff + ip;
-        const double cceij = a.cce[iidx] - a.lij_ceil * logd_ij;
-        if (cceij > eijcut || cceij >= a.expcutoff) continue;  // zero prefactor (libcint pair data)
-        const double aij = a.aij[iidx];
-        const double Pv[3] = {a.px[iidx], a.py[iidx], a.pz[iidx]};
+        const double* pi = a.prim + 8 * (size_t)iidx;
+        const double cceij = pi[6] - a.lij_ceil * logd_ij;
+        // (libcint zeroes the prefactor of a bra pair with cce >= expcutoff)
+        if (cceij > eijcut || cceij >= a.expcutoff) continue;
+        const double aij = pi[0];
+        const double Pv[3] = {pi[1], pi[2], pi[3]};
         double dx[3], prb[3], qrk[3], rr = 0;
         for (int d = 0; d < 3; ++d) {
           dx[d] = Pv[d] - Qv[d];
@@ -126,9 +172,9 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
           qrk[d] = Qv[d] - (S.kbase ? rk[d] : rl[d]);
         }
         const double inv_sum = 1.0 / (aij + akl);
-        const double hij = a.hinv[iidx], hkl = a.hinv[kidx];
+        const double hij = pi[5], hkl = pk[5];
         const double a0 = aij * akl * inv_sum;
-        const double fac = a.common * a.kfac[iidx] * a.kfac[kidx];
+        const double fac = a.common * pi[4] * pk[4];
         const double fac1 = 4.0 * hij * hkl * sqrt(inv_sum) * fac;
         const double x = a0 * rr;
         if (t < NR) {
@@ -143,44 +189,74 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
         for (int c = t; c < 3 * NR; c += T) {
           const int axis = c / NR, r = c - axis * NR;
           generic_chain(g, S, axis, r, uw[r], uw[2 * NR + r], uw[NR + r], inv_sum, hij, hkl, aij,
-                        akl, dx[axis],
-                        prb[axis], qrk[axis], rij[axis], rkl[axis]);
+                        akl, dx[axis], prb[axis], qrk[axis], rij[axis], rkl[axis]);
         }
         __syncthreads();
-        const double ai2 = -2.0 * a.ai[iidx];
-        for (int e = t; e < S.nf; e += T) {
-          int fi = e % nfi, e1 = e / nfi;
-          int fj = e1 % nfj, e2 = e1 / nfj;
-          int fk = e2 % nfk, fl = e2 / nfk;
-          int ix, iy, iz, jx, jy, jz, kx, ky, kz, lx, ly, lz;
-          cart_decode(S.l[0], fi, ix, iy, iz);
-          cart_decode(S.l[1], fj, jx, jy, jz);
-          cart_decode(S.l[2], fk, kx, ky, kz);
-          cart_decode(S.l[3], fl, lx, ly, lz);
-          const double* gx = g + (size_t)(((ix * S.dj + jx) * S.dk + kx) * S.dl + lx) * NR;
-          const double* gy =
-              g + (size_t)(S.g4 + ((iy * S.dj + jy) * S.dk + ky) * S.dl + ly) * NR;
-          const double* gz =
-              g + (size_t)(2 * S.g4 + ((iz * S.dj + jz) * S.dk + kz) * S.dl + lz) * NR;
-          if (!S.deriv) {
-            double s = 0;
-            for (int r = 0; r < NR; ++r) s += gx[r] * gy[r] * gz[r];
-            acc[e] += s;
-          } else {
-            double sx = 0, sy = 0, sz = 0;
-            for (int r = 0; r < NR; ++r) {
-              double fx = ai2 * gx[SI * NR + r], fy = ai2 * gy[SI * NR + r],
-                     fz = ai2 * gz[SI * NR + r];
-              if (ix > 0) fx += ix * gx[r - SI * NR];
-              if (iy > 0) fy += iy * gy[r - SI * NR];
-              if (iz > 0) fz += iz * gz[r - SI * NR];
-              sx += fx * gy[r] * gz[r];
-              sy += gx[r] * fy * gz[r];
-              sz += gx[r] * gy[r] * fz;
+        const double ai2 = -2.0 * pi[7];
+        if constexpr (AREG) {
+#pragma unroll
+          for (int k = 0; k < kGenAcc; ++k) {
+            if (t + k * 256 >= S.nf) break;
+            const double* gx = g + (size_t)(pos[k] & 1023) * NR;
+            const double* gy = g + (size_t)(S.g4 + ((pos[k] >> 10) & 1023)) * NR;
+            const double* gz = g + (size_t)(2 * S.g4 + (pos[k] >> 20)) * NR;
+            if (!S.deriv) {
+              double sum = 0;
+              for (int r = 0; r < NR; ++r) sum += gx[r] * gy[r] * gz[r];
+              racc[k] += sum;
+            } else {
+              const int ix = ipw[k] & 15, iy = (ipw[k] >> 4) & 15, iz = ipw[k] >> 8;
+              double sx = 0, sy = 0, sz = 0;
+              for (int r = 0; r < NR; ++r) {
+                double fx = ai2 * gx[SI * NR + r], fy = ai2 * gy[SI * NR + r],
+                       fz = ai2 * gz[SI * NR + r];
+                if (ix > 0) fx += ix * gx[r - SI * NR];
+                if (iy > 0) fy += iy * gy[r - SI * NR];
+                if (iz > 0) fz += iz * gz[r - SI * NR];
+                sx += fx * gy[r] * gz[r];
+                sy += gx[r] * fy * gz[r];
+                sz += gx[r] * gy[r] * fz;
+              }
+              racc[k] += sx;
+              racc[kGenAcc + k] += sy;
+              racc[2 * kGenAcc + k] += sz;
             }
-            acc[e] += sx;
-            acc[S.nf + e] += sy;
-            acc[2 * S.nf + e] += sz;
+          }
+        } else {
+          for (int e = t; e < S.nf; e += T) {
+            int fi = e % nfi, e1 = e / nfi;
+            int fj = e1 % nfj, e2 = e1 / nfj;
+            int fk = e2 % nfk, fl = e2 / nfk;
+            int ix, iy, iz, jx, jy, jz, kx, ky, kz, lx, ly, lz;
+            cart_decode(S.l[0], fi, ix, iy, iz);
+            cart_decode(S.l[1], fj, jx, jy, jz);
+            cart_decode(S.l[2], fk, kx, ky, kz);
+            cart_decode(S.l[3], fl, lx, ly, lz);
+            const double* gx = g + (size_t)(((ix * S.dj + jx) * S.dk + kx) * S.dl + lx) * NR;
+            const double* gy =
+                g + (size_t)(S.g4 + ((iy * S.dj + jy) * S.dk + ky) * S.dl + ly) * NR;
+            const double* gz =
+                g + (size_t)(2 * S.g4 + ((iz * S.dj + jz) * S.dk + kz) * S.dl + lz) * NR;
+            if (!S.deriv) {
+              double sum = 0;
+              for (int r = 0; r < NR; ++r) sum += gx[r] * gy[r] * gz[r];
+              acc[e] += sum;
+            } else {
+              double sx = 0, sy = 0, sz = 0;
+              for (int r = 0; r < NR; ++r) {
+                double fx = ai2 * gx[SI * NR + r], fy = ai2 * gy[SI * NR + r],
+                       fz = ai2 * gz[SI * NR + r];
+                if (ix > 0) fx += ix * gx[r - SI * NR];
+                if (iy > 0) fy += iy * gy[r - SI * NR];
+                if (iz > 0) fz += iz * gz[r - SI * NR];
+                sx += fx * gy[r] * gz[r];
+                sy += gx[r] * fy * gz[r];
+                sz += gx[r] * gy[r] * fz;
+              }
+              acc[e] += sx;
+              acc[S.nf + e] += sy;
+              acc[2 * S.nf + e] += sz;
+            }
           }
         }
         __syncthreads();
@@ -193,7 +269,13 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
     const int ns0 = nsph(S.l[0]), ns1 = nsph(S.l[1]), ns2 = nsph(S.l[2]), ns3 = nsph(S.l[3]);
     const int NS = ns0 * ns1 * ns2 * ns3;
     for (int comp = 0; comp < S.ncomp; ++comp) {
-      for (int e = t; e < S.nf; e += T) b0[e] = acc[comp * S.nf + e];
+      if constexpr (AREG) {
+#pragma unroll
+        for (int k = 0; k < kGenAcc; ++k)
+          if (t + k * 256 < S.nf) b0[t + k * 256] = racc[comp * kGenAcc + k];
+      } else {
+        for (int e = t; e < S.nf; e += T) b0[e] = acc[comp * S.nf + e];
+      }
       __syncthreads();
       int dims[4] = {nfi, nfj, nfk, nfl};
       double* src = b0;
@@ -244,7 +326,20 @@ hipError_t launch_generic(const EriArgs& a, const int* l, int deriv, int grid, h
   S.ibase = S.lic > l[1];
   S.kbase = l[2] > l[3];
   int64_t per = generic_scratch_per_block(l, deriv);
-  hipLaunchKernelGGL(eri_generic_kernel, dim3(grid), dim3(256), 0, s, a, S, per);
+  const size_t gbytes = (size_t)3 * S.g4 * S.nr * sizeof(double);
+  const bool glds = gbytes <= kGenLdsBytes;
+  const bool areg = S.nf <= 256 * kGenAcc;
+  const size_t dyn = glds ? gbytes : 0;
+  if (glds && areg)
+    hipLaunchKernelGGL((eri_generic_kernel<true, true>), dim3(grid), dim3(256), dyn, s, a, S, per);
+  else if (glds)
+    hipLaunchKernelGGL((eri_generic_kernel<true, false>), dim3(grid), dim3(256), dyn, s, a, S,
+                       per);
+  else if (areg)
+    hipLaunchKernelGGL((eri_generic_kernel<false, true>), dim3(grid), dim3(256), 0, s, a, S, per);
+  else
+    hipLaunchKernelGGL((eri_generic_kernel<false, false>), dim3(grid), dim3(256), 0, s, a, S,
+                       per);
   return hipGetLastError();
 }
 
