@@ -222,6 +222,7 @@ def secondary_configs():
     out = {}
 
     def run(key, plan, family, mol, unit, reps=3):
+        log(f"secondary {key}: {plan.ntuples} tuples")
         plan.autotune(reps=1)
         plan.execute(graph=True)
         plan.sync()
@@ -246,6 +247,7 @@ def secondary_configs():
                     "max_abs_err_vs_oracle": float(np.abs(got - ref).max())}
         plan.close()
 
+    log("secondary: building the aux-basis lattice plans")
     m4 = molecule.config(4)
     pairs = [(i, j) for i in range(m4.aux_start) for j in range(i + 1)][::8]
     trip = np.array([(i, j, k, 0) for i, j in pairs for k in range(m4.aux_start, m4.nbas)],

@@ -1308,16 +1308,20 @@ int cintx_plan_copy_tuples(cintx_plan* plan, const int64_t* which, int64_t n, do
       const TupleRec& tr = plan->tuples[t];
       if (offsets) offsets[k] = pos;
       // gather this tuple's canonical blocks, then place them
-      std::vector<double> blocks;
-      std::vector<int64_t> where;
+      // size the staging buffer first: copies in flight must not see it move
+      std::vector<int64_t> where((size_t)tr.nvq);
+      size_t total = 0;
+      for (int64_t v = 0; v < tr.nvq; ++v) {
+        where[(size_t)v] = (int64_t)total;
+        total += (size_t)plan->bins[plan->vqs[tr.first_vq + v].bin].block;
+      }
+      std::vector<double> blocks(total);
       for (int64_t v = 0; v < tr.nvq; ++v) {
         const VQ& vq = plan->vqs[tr.first_vq + v];
         const Bin& b = plan->bins[vq.bin];
-        where.push_back((int64_t)blocks.size());
-        size_t old = blocks.size();
-        blocks.resize(old + (size_t)b.block);
-        HIPCHK(hipMemcpyAsync(blocks.data() + old, plan->d_out + b.out_base + vq.idx * b.block,
-                              (size_t)b.block * 8, hipMemcpyDeviceToHost, C.stream));
+        HIPCHK(hipMemcpyAsync(blocks.data() + where[(size_t)v],
+                              plan->d_out + b.out_base + vq.idx * b.block, (size_t)b.block * 8,
+                              hipMemcpyDeviceToHost, C.stream));
       }
       HIPCHK(hipStreamSynchronize(C.stream));
       int nn[4] = {1, 1, 1, 1};
@@ -1621,10 +1625,24 @@ int cintx_plan_create_screened(cintx_plan** plan, const int* atm, int natm, cons
         std::vector<int> lo, hi;
         std::unordered_map<uint64_t, double> qof;
         for (const SP& s : sp) qof[((uint64_t)s.a << 32) | (uint32_t)s.b] = s.q;
+        // FP32 without packed math issues at the FP64 rate on CDNA4: the FP32
+        // recurrences pay only in the classes whose default kernel shares g
+        // through LDS (half the LDS bytes and registers); the one-lane
+        // register kernels are faster in FP64, so those classes stay FP64
+        auto fp32_pays = [&](const int* sh) {
+          int la = B.l[sh[0]], lb = B.l[sh[1]], lc = B.l[sh[2]], ld = B.l[sh[3]];
+          if (lc > la || (lc == la && ld > lb)) {
+            std::swap(la, lc);
+            std::swap(lb, ld);
+          }
+          if (la > kMaxL || lb > kMaxL || lc > kMaxL || ld > kMaxL) return false;
+          const KernelInfo& k = ctx().reg->k[0][1][la][lb][lc][ld][0];
+          return k.fn != nullptr && k.lanes_per_tuple > 1;
+        };
         for (int64_t t = 0; t < nt; ++t) {
           double b = qof[((uint64_t)tuples[4 * t] << 32) | (uint32_t)tuples[4 * t + 1]] *
                      qof[((uint64_t)tuples[4 * t + 2] << 32) | (uint32_t)tuples[4 * t + 3]];
-          auto& dst = b < mixed_threshold ? lo : hi;
+          auto& dst = (b < mixed_threshold && fp32_pays(&tuples[4 * t])) ? lo : hi;
           dst.insert(dst.end(), tuples.begin() + 4 * t, tuples.begin() + 4 * t + 4);
         }
         plan_add_tuples(P, hi.data(), (int64_t)hi.size() / 4, 0);
