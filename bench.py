@@ -39,7 +39,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--bra-stride", type=int, default=256)
     ap.add_argument("--threshold", type=float, default=1e-12)
-    ap.add_argument("--mixed-threshold", type=float, default=1e-6)
+    ap.add_argument("--mixed-threshold", type=float, default=1e-9)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sample", type=int, default=4000)

@@ -1548,9 +1548,9 @@ int cintx_plan_create_screened(cintx_plan** plan, const int* atm, int natm, cons
         diag.push_back(j);
       }
     int64_t npair_all = (int64_t)diag.size() / 4;
-    std::vector<double> qv((size_t)npair_all, 0.0);
-    {
-      cintx_plan* D = plan_new(0, atm, natm, bas, nbas, env, 0);
+    auto diag_bounds = [&](const double* env_use) {
+      std::vector<double> qv((size_t)npair_all, 0.0);
+      cintx_plan* D = plan_new(0, atm, natm, bas, nbas, env_use, 0);
       struct G {
         cintx_plan* p;
         ~G() { plan_free(p); }
@@ -1563,7 +1563,6 @@ int cintx_plan_create_screened(cintx_plan** plan, const int* atm, int natm, cons
       plan_upload(D);
       DevCtx& C = ctx();
       plan_launch_all(D, D->d_out, false);
-      std::vector<double> nat((size_t)D->total);
       std::vector<double> canon((size_t)D->total);
       HIPCHK(hipMemcpyAsync(canon.data(), D->d_out, (size_t)D->total * 8, hipMemcpyDeviceToHost,
                             C.stream));
@@ -1579,6 +1578,28 @@ int cintx_plan_create_screened(cintx_plan** plan, const int* atm, int natm, cons
         }
         qv[t] = std::sqrt(m);
       }
+      return qv;
+    };
+    const std::vector<double> qv = diag_bounds(env);
+    // Mixed plans select FP32 tuples with the same bound taken over the
+    // absolute values of the contraction coefficients: primitive
+    // contributions of mixed sign can cancel far below their own size, and
+    // the FP32 rounding follows the size of the terms, not of the sum.
+    std::vector<double> qabs;
+    if (precision == CINTX_PREC_MIXED) {
+      size_t len = 0;
+      for (int s = 0; s < nbas; ++s) {
+        const int* b = bas + 8 * s;
+        len = std::max(len, (size_t)b[5] + b[2]);
+        len = std::max(len, (size_t)b[6] + (size_t)b[2] * b[3]);
+        len = std::max(len, (size_t)atm[6 * b[0] + 1] + 3);
+      }
+      std::vector<double> eabs(env, env + len);
+      for (int s = 0; s < nbas; ++s) {
+        const int* b = bas + 8 * s;
+        for (int k = 0; k < b[2] * b[3]; ++k) eabs[(size_t)b[6] + k] = std::fabs(eabs[(size_t)b[6] + k]);
+      }
+      qabs = diag_bounds(eabs.data());
     }
     cintx_plan* P = plan_new(0, atm, natm, bas, nbas, env, 0);
     try {
@@ -1624,11 +1645,11 @@ int cintx_plan_create_screened(cintx_plan** plan, const int* atm, int natm, cons
         // split by bound: FP32 bins for tuples below mixed_threshold
         std::vector<int> lo, hi;
         std::unordered_map<uint64_t, double> qof;
-        for (const SP& s : sp) qof[((uint64_t)s.a << 32) | (uint32_t)s.b] = s.q;
+        for (const SP& s : sp) qof[((uint64_t)s.a << 32) | (uint32_t)s.b] = qabs[(size_t)s.id];
         // FP32 without packed math issues at the FP64 rate on CDNA4: the FP32
-        // recurrences pay only in the classes whose default kernel shares g
-        // through LDS (half the LDS bytes and registers); the one-lane
-        // register kernels are faster in FP64, so those classes stay FP64
+        // recurrences pay only in classes that must share g through LDS (half
+        // the LDS bytes and registers); classes with a one-lane register
+        // kernel are faster in FP64, so they stay FP64
         auto fp32_pays = [&](const int* sh) {
           int la = B.l[sh[0]], lb = B.l[sh[1]], lc = B.l[sh[2]], ld = B.l[sh[3]];
           if (lc > la || (lc == la && ld > lb)) {
@@ -1636,8 +1657,11 @@ int cintx_plan_create_screened(cintx_plan** plan, const int* atm, int natm, cons
             std::swap(lb, ld);
           }
           if (la > kMaxL || lb > kMaxL || lc > kMaxL || ld > kMaxL) return false;
-          const KernelInfo& k = ctx().reg->k[0][1][la][lb][lc][ld][0];
-          return k.fn != nullptr && k.lanes_per_tuple > 1;
+          const KernelInfo* k64 = ctx().reg->k[0][0][la][lb][lc][ld];
+          if (!ctx().reg->k[0][1][la][lb][lc][ld][0].fn) return false;  // no FP32 build
+          for (int v = 0; v < kVariants; ++v)
+            if (k64[v].fn && k64[v].lanes_per_tuple == 1) return false;
+          return true;
         };
         for (int64_t t = 0; t < nt; ++t) {
           double b = qof[((uint64_t)tuples[4 * t] << 32) | (uint32_t)tuples[4 * t + 1]] *

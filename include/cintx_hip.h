@@ -146,8 +146,10 @@ int cintx_plan_create(cintx_plan **plan, int family, const int *atm, int natm, c
  * sqrt((ij|ij)) sqrt((kl|kl)) is at least `threshold`; bra pairs restricted to
  * every `bra_stride`-th screened pair (1 = all).  Tuples are generated on the
  * host and kept in HBM.  CINTX_PREC_MIXED puts a tuple in FP32 when its bound
- * is below mixed_threshold and its class runs the LDS-sharing kernel (FP32
- * does not pay in the one-lane register kernels on CDNA4). */
+ * taken over the absolute contraction coefficients (the size of the terms
+ * the FP32 rounding follows) is below mixed_threshold and its class has no
+ * one-lane register kernel (FP32 pays only where g is shared through LDS on
+ * CDNA4). */
 int cintx_plan_create_screened(cintx_plan **plan, const int *atm, int natm, const int *bas,
                                int nbas, const double *env, double threshold, int bra_stride,
                                int precision, double mixed_threshold);

@@ -293,10 +293,10 @@ def test_every_launch_variant(oracle, variant):
 
 def test_mixed_plan_error_bound_cfg2(oracle):
     """The mixed path (FP32 recurrences for tuples whose Schwarz bound is
-    below 1e-6) stays within 1e-10 absolute of the oracle."""
+    below 1e-9) stays within 1e-10 absolute of the oracle."""
     mol = M.config(2)
     p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, screened=(1e-12, 97),
-                  precision=capi.PREC_MIXED, mixed_threshold=1e-6)
+                  precision=capi.PREC_MIXED, mixed_threshold=1e-9)
     assert p.ntuples_fp32 > 0
     p.execute()
     p.sync()
