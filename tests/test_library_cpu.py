@@ -146,3 +146,67 @@ def test_fp64_floor_of_derivative_integrals(oracle):
     scale = np.repeat([np.abs(ref[a:a + n]).max() if n else 0.0 for a, n in zip(offs, sizes)],
                       sizes)
     assert np.all(err <= deriv_tolerance(ref, scale))
+
+
+def _cart(l):
+    return [(x, l - x - z, z) for x in range(l, -1, -1) for z in range(0, l - x + 1)]
+
+
+def test_launch_variants_cover_every_element_and_fit_lds():
+    """Every generated launch shape of every instantiated class: the LDS
+    path's lane -> row mapping covers each Cartesian element exactly once,
+    its g reads stay inside the tensor, its workgroup LDS fits 64 KB; the
+    register path's row chunks cover every row (mirrors ClassShape in
+    csrc/eri_device.h)."""
+    from cintx_amd.build import class_sets, variants
+
+    def nc(l):
+        return (l + 1) * (l + 2) // 2
+
+    for deriv, prec, ls in class_sets():
+        li, lj, lk, ll = ls
+        nr = (li + deriv + lj + lk + ll) // 2 + 1
+        di, dj, dk, dl = li + deriv + 1, lj + 1, lk + 1, ll + 1
+        g4 = di * dj * dk * dl
+        gs = 3 * g4 * nr
+        nf = nc(li) * nc(lj) * nc(lk) * nc(ll)
+        nb = nc(li) if deriv else nc(li) * nc(lj)
+        nrow = nf // nb
+        vs = variants(deriv, prec, ls)
+        assert 1 <= len(vs) <= capi.NVARIANTS
+        for t, p in vs:
+            if t == 1:  # register path, p row chunks
+                rpc = -(-nrow // p)
+                rows = sorted(r for ch in range(p) for r in range(ch * rpc, min(nrow, (ch + 1) * rpc)))
+                assert rows == list(range(nrow))
+                continue
+            sz = 8 if prec == 0 else 4
+            ud = max((p * gs * sz + 7) // 8, 2 * nf)
+            lpt = (ud + p * (3 * nr + 16)) * 8
+            q = max(1, 256 // t)
+            while q > 1 and q * lpt > 32768:
+                q //= 2
+            assert q * lpt <= 65536, (ls, deriv, t, p)
+            groups = t // nrow if t >= nrow else 1
+            rpl = 1 if t >= nrow else -(-nrow // t)
+            seen = []
+            for lane in range(t):
+                grp = lane // nrow if t >= nrow else 0
+                for k in range(rpl):
+                    row = lane - grp * nrow if t >= nrow else lane + t * k
+                    if grp >= groups or row >= nrow:
+                        continue
+                    if grp == 0:
+                        seen += [row * nb + ib for ib in range(nb)]
+                    if deriv:
+                        fj, r1 = row % nc(lj), row // nc(lj)
+                        (jz, kz, lz) = (_cart(lj)[fj][2], _cart(lk)[r1 % nc(lk)][2],
+                                        _cart(ll)[r1 // nc(lk)][2])
+                        bz = (jz * dk + kz) * dl + lz
+                        top = (2 * g4 + (di - 1) * dj * dk * dl + bz) * nr + nr - 1
+                    else:
+                        kz, lz = _cart(lk)[row % nc(lk)][2], _cart(ll)[row // nc(lk)][2]
+                        bz = kz * dl + lz
+                        top = (2 * g4 + (di * dj - 1) * dk * dl + bz) * nr + nr - 1
+                    assert top < gs
+            assert sorted(seen) == list(range(nf)), (ls, deriv, t, p)
