@@ -121,6 +121,12 @@ def variants(deriv, prec, ls):
     cands.append((w2, max(1, min(8, w2 // (3 * nr), plds2))))
     if wide >= 8:
         cands.append((wide // 2, 1))
+    if not deriv and ne > 81:
+        # rows of the i components alone (P | 256): more lanes per tuple
+        nrow_i = nf // nc[0]
+        ti = min(256, max(4, _p2(max(nrow_i, 3 * nr))))
+        pi = max(1, min(8, ti // (3 * nr), plds))
+        cands.insert(2, (ti, pi | 256))
     # one lane per tuple, everything in registers: the default for small
     # classes, a candidate up to 81 elements
     if ne <= 27:

@@ -334,10 +334,13 @@ struct ClassShape {
   static constexpr int NF = ncart(LI) * ncart(LJ) * ncart(LK) * ncart(LL);
   static constexpr int NCOMP = DERIV ? 3 : 1;
   static constexpr int NE = NF * NCOMP;
-  static constexpr int NB = DERIV ? ncart(LI) : ncart(LI) * ncart(LJ);
+  // IROW (PP bit 8, LDS path only): rows of the i components alone, as for
+  // nabla_i, so more lanes share a large class
+  static constexpr bool IROW = !DERIV && TT > 1 && ((PP >> 8) & 1);
+  static constexpr int NB = (DERIV || IROW) ? ncart(LI) : ncart(LI) * ncart(LJ);
   static constexpr int NROW = NF / NB;
   static constexpr int T = TT;
-  static constexpr int P = TT == 1 ? 1 : PP;
+  static constexpr int P = TT == 1 ? 1 : (PP & 255);
   // one-lane path: the rows of the block are done in NCH chunks (PP), each a
   // full pass over the primitive tuples, so that a chunk's accumulators and
   // the g tensor of one root fit in registers
@@ -392,6 +395,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
   constexpr bool IBASE = LIc > LJ, KBASE = LK > LL;
   constexpr int GS = S::GS;
   constexpr int NB = S::NB, NROW = S::NROW, G = S::G, RPL = S::RPL;
+  constexpr bool IROWS = DERIV || S::IROW;  // rows are (j, k, l) components
 
   // One lane per tuple keeps g and the roots in registers (every index is a
   // compile-time constant once the loops are unrolled); wider tuples share
@@ -699,7 +703,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
       rowid[k] = (grp < G && row < NROW) ? row : -1;
       const int rr = rowid[k] < 0 ? 0 : row;
       int bx, by, bz;
-      if constexpr (!DERIV) {
+      if constexpr (!IROWS) {
         int kx, ky, kz, lx, ly, lz;
         cart_decode(LK, rr % NFK, kx, ky, kz);
         cart_decode(LL, rr / NFK, lx, ly, lz);
@@ -825,7 +829,24 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
         for (int pp = grp; pp < P; pp += G) {
           if (!((okmask >> pp) & 1u)) continue;
           const R* gpp = sg + pp * GS;
-          if constexpr (!DERIV) {
+          if constexpr (S::IROW) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+              R xv[DI], yv[DI], zv[DI];
+#pragma unroll
+              for (int i = 0; i < DI; ++i) {
+                xv[i] = gpp[(i * SI + bx) * NR + r];
+                yv[i] = gpp[(G4 + i * SI + by) * NR + r];
+                zv[i] = gpp[(2 * G4 + i * SI + bz) * NR + r];
+              }
+#pragma unroll
+              for (int ib = 0; ib < NB; ++ib) {
+                int ix, iy, iz;
+                cart_decode(LI, ib, ix, iy, iz);
+                tgt(k * NB + ib) += xv[ix] * yv[iy] * zv[iz];
+              }
+            }
+          } else if constexpr (!DERIV) {
             constexpr int NIJ = DI * DJ;
 #pragma unroll
             for (int r = 0; r < NR; ++r) {

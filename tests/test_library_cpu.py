@@ -170,11 +170,13 @@ def test_launch_variants_cover_every_element_and_fit_lds():
         g4 = di * dj * dk * dl
         gs = 3 * g4 * nr
         nf = nc(li) * nc(lj) * nc(lk) * nc(ll)
-        nb = nc(li) if deriv else nc(li) * nc(lj)
-        nrow = nf // nb
         vs = variants(deriv, prec, ls)
         assert 1 <= len(vs) <= capi.NVARIANTS
         for t, p in vs:
+            irow = t > 1 and not deriv and p >= 256  # rows of the i components alone
+            p &= 255
+            nb = nc(li) if (deriv or irow) else nc(li) * nc(lj)
+            nrow = nf // nb
             if t == 1:  # register path, p row chunks
                 rpc = -(-nrow // p)
                 rows = sorted(r for ch in range(p) for r in range(ch * rpc, min(nrow, (ch + 1) * rpc)))
@@ -198,7 +200,7 @@ def test_launch_variants_cover_every_element_and_fit_lds():
                         continue
                     if grp == 0:
                         seen += [row * nb + ib for ib in range(nb)]
-                    if deriv:
+                    if deriv or irow:
                         fj, r1 = row % nc(lj), row // nc(lj)
                         (jz, kz, lz) = (_cart(lj)[fj][2], _cart(lk)[r1 % nc(lk)][2],
                                         _cart(ll)[r1 // nc(lk)][2])
