@@ -105,6 +105,19 @@ def main():
     err64, errmx = sample_errors(mol, p64, pmx, args.sample)
     log(f"sampled max abs err vs oracle: fp64 {err64:.3e}, mixed {errmx:.3e}")
 
+    # J/K against a random symmetric density (N(0,1)/nbf): the FP64 plan's
+    # quartets digested with their 8-fold images (unfused: the plan's output
+    # is written, then read by the digest kernels)
+    nao = mol.nao()
+    rngd = np.random.default_rng(2)
+    dens = rngd.normal(size=(nao, nao)) / nao
+    dens = dens + dens.T
+    p64.jk(dens)  # warm-up
+    t0 = time.perf_counter()
+    p64.jk(dens)
+    jk_ms = (time.perf_counter() - t0) * 1e3
+    log(f"J/K build (plan + digest, host timed): {jk_ms:.1f} ms")
+
     prim64, roots64, flops64 = p64.work()
     log(f"fp64 work: {prim64:.3e} primitive quartets, {flops64:.3e} FLOP")
     step_s = ms64 / 1e3
@@ -180,6 +193,9 @@ def main():
                  "wall_ms_per_step": wall64, "gflops": achieved_step * 1e3,
                  "prim_quartets": prim64, "root_evals": roots64},
         "mixed_over_fp64": value / fp64_rate,
+        "jk": {"ms": jk_ms, "nbf": nao,
+               "note": "J and K from the FP64 plan's unique quartets and a random symmetric "
+                       "density; plan execution + digest kernels + 64 MB copies, host-timed"},
         "fp64_max_abs_err_vs_oracle": err64,
         "mixed_max_abs_err_vs_oracle": errmx,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,

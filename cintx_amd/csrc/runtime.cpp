@@ -1853,20 +1853,23 @@ int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
     DevArr<double> dD(nn), dJ(nn), dK(nn);
     DevArr<int> dao(B.nbas), dl(B.nbas);
     DevArr<int2> dsh(plan->pairs.shells.size());
-    HIPCHK(hipMemcpy(dD.p, D, nn * 8, hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(dJ.p, 0, nn * 8));
-    HIPCHK(hipMemset(dK.p, 0, nn * 8));
-    HIPCHK(hipMemcpy(dao.p, ao_off.data(), B.nbas * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dl.p, shl_l.data(), B.nbas * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(dsh.p, plan->pairs.shells.data(), plan->pairs.shells.size() * sizeof(int2),
-                     hipMemcpyHostToDevice));
+    // everything on the plan's (non-blocking) stream, so the zeroing and the
+    // uploads are ordered before the digest kernels
+    HIPCHK(hipMemcpyAsync(dD.p, D, nn * 8, hipMemcpyHostToDevice, C.stream));
+    HIPCHK(hipMemsetAsync(dJ.p, 0, nn * 8, C.stream));
+    HIPCHK(hipMemsetAsync(dK.p, 0, nn * 8, C.stream));
+    HIPCHK(hipMemcpyAsync(dao.p, ao_off.data(), B.nbas * 4, hipMemcpyHostToDevice, C.stream));
+    HIPCHK(hipMemcpyAsync(dl.p, shl_l.data(), B.nbas * 4, hipMemcpyHostToDevice, C.stream));
+    HIPCHK(hipMemcpyAsync(dsh.p, plan->pairs.shells.data(),
+                          plan->pairs.shells.size() * sizeof(int2), hipMemcpyHostToDevice,
+                          C.stream));
     plan_launch_all(plan, plan->d_out, false);
     for (const Bin& b : plan->bins)
       HIPCHK(launch_jk_digest(plan->d_out, b.out_base, b.block, b.d_q, (int64_t)b.q.size(), dsh.p,
                               dao.p, dl.p, dD.p, dJ.p, dK.p, nao, C.stream));
+    HIPCHK(hipMemcpyAsync(J, dJ.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
+    HIPCHK(hipMemcpyAsync(K, dK.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
     HIPCHK(hipStreamSynchronize(C.stream));
-    HIPCHK(hipMemcpy(J, dJ.p, nn * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(K, dK.p, nn * 8, hipMemcpyDeviceToHost));
   });
 }
 
