@@ -129,6 +129,8 @@ def variants(deriv, prec, ls):
         cands.insert(2, (ti, pi | 256))
     # one lane per tuple, everything in registers: the default for small
     # classes, a candidate up to 81 elements
+    # (a floor of 2-3 waves per SIMD, PP bits 9-10, was measured: the
+    # register kernels then spill and run 1-7x slower, so it is not offered)
     if ne <= 27:
         cands.insert(0, (1, 1))
     elif ne <= 81:
@@ -161,7 +163,7 @@ def generate(nparts=24, nchunkparts=12):
     plain, chunked = [], []
     for deriv, prec, ls in classes:
         for v, (tt, pp) in enumerate(variants(deriv, prec, ls)):
-            (chunked if (tt == 1 and pp > 1) else plain).append((deriv, prec, ls, v, tt, pp))
+            (chunked if (tt == 1 and (pp & 255) > 1) else plain).append((deriv, prec, ls, v, tt, pp))
     parts = [plain[i::nparts] for i in range(nparts)]
     parts += [chunked[i::nchunkparts] for i in range(nchunkparts)]
     nparts = len(parts)

@@ -344,7 +344,10 @@ struct ClassShape {
   // one-lane path: the rows of the block are done in NCH chunks (PP), each a
   // full pass over the primitive tuples, so that a chunk's accumulators and
   // the g tensor of one root fit in registers
-  static constexpr int NCH = TT == 1 ? PP : 1;
+  static constexpr int NCH = TT == 1 ? (PP & 255) : 1;
+  // PP bits 9-10: minimum waves per SIMD the compiler must fit (register
+  // budget 512 / MINW per lane), 1 by default
+  static constexpr int MINW = ((PP >> 9) & 3) + 1;
   static constexpr int RPC = (NROW + NCH - 1) / NCH;  // rows per chunk
   static constexpr int G = T >= NROW ? T / NROW : 1;               // groups per row
   static constexpr int RPL = T >= NROW ? 1 : (NROW + T - 1) / T;   // rows per lane
@@ -373,12 +376,9 @@ struct ClassShape {
 };
 
 template <int LI, int LJ, int LK, int LL, int DERIV, typename R, int TT, int PP>
-#ifndef CX_MIN_WAVES
-#define CX_MIN_WAVES 1
-#endif
 __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T *
                                    ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::template Q<R>()),
-                                  CX_MIN_WAVES)
+                                  (ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::MINW))
     eri_class_kernel(EriArgs a) {
   using S = ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>;
   static_assert(S::template Q<R>() * S::template lds_per_tuple<R>() <= 64 * 1024, "LDS");

@@ -177,7 +177,8 @@ def test_launch_variants_cover_every_element_and_fit_lds():
             p &= 255
             nb = nc(li) if (deriv or irow) else nc(li) * nc(lj)
             nrow = nf // nb
-            if t == 1:  # register path, p row chunks
+            if t == 1:  # register path, p row chunks (bits 9-10: wave floor)
+                p &= 255
                 rpc = -(-nrow // p)
                 rows = sorted(r for ch in range(p) for r in range(ch * rpc, min(nrow, (ch + 1) * rpc)))
                 assert rows == list(range(nrow))
