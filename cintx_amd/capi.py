@@ -27,9 +27,12 @@ INT2E_SPH = 10
 INT3C2E_IP1_SPH = 19
 INT2E_IP1_SPH = 36
 INT3C2E_SPH = 40
+INT2C2E_SPH = 13
 
-FAMILY = {"int2e_sph": 0, "int2e_ip1_sph": 1, "int3c2e_sph": 2, "int3c2e_ip1_sph": 3}
-API_OF_FAMILY = {0: INT2E_SPH, 1: INT2E_IP1_SPH, 2: INT3C2E_SPH, 3: INT3C2E_IP1_SPH}
+FAMILY = {"int2e_sph": 0, "int2e_ip1_sph": 1, "int3c2e_sph": 2, "int3c2e_ip1_sph": 3,
+          "int2c2e_sph": 4}
+API_OF_FAMILY = {0: INT2E_SPH, 1: INT2E_IP1_SPH, 2: INT3C2E_SPH, 3: INT3C2E_IP1_SPH,
+                 4: INT2C2E_SPH}
 PREC_FP64 = 0
 # launch-shape variants compiled per class (kVariants in csrc/launch.h)
 NVARIANTS = 6
@@ -42,8 +45,9 @@ STATUS = {
 }
 
 EXPORTS = [
-    "int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph",
+    "int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph",
     "int2e_optimizer", "int2e_ip1_optimizer", "int3c2e_optimizer", "int3c2e_ip1_optimizer",
+    "int2c2e_optimizer",
     "CINTdel_optimizer",
     "cintrs_query_workspace", "cintrs_eval", "cintrs_last_error_code",
     "cintrs_clear_last_error", "cintrs_copy_last_error_message",
@@ -87,7 +91,7 @@ def lib():
             raise ImportError(f"{LIB_PATH} is not built; run python -m cintx_amd.build")
         L = ctypes.CDLL(LIB_PATH)
         raw = [DP, IP, IP, IP, ctypes.c_int, IP, ctypes.c_int, DP, ctypes.c_void_p, DP]
-        for n in ("int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph"):
+        for n in ("int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph"):
             getattr(L, n).argtypes = raw
             getattr(L, n).restype = ctypes.c_int
         sz = ctypes.c_size_t
@@ -154,9 +158,12 @@ def _check(st):
         raise CintxError(st, msg)
 
 
+def arity(family):
+    return 2 if family.startswith("int2c2e") else 3 if family.startswith("int3c2e") else 4
+
+
 def natural_dims(family, shls, bas):
-    arity = 3 if family.startswith("int3c2e") else 4
-    return [(2 * int(bas[s][1]) + 1) * int(bas[s][3]) for s in shls[:arity]]
+    return [(2 * int(bas[s][1]) + 1) * int(bas[s][3]) for s in shls[:arity(family)]]
 
 
 def ncomp(family):
@@ -197,6 +204,10 @@ def int3c2e_sph(shls, atm, bas, env, dims=None):
 
 def int3c2e_ip1_sph(shls, atm, bas, env, dims=None):
     return raw_call("int3c2e_ip1_sph", shls, atm, bas, env, dims)
+
+
+def int2c2e_sph(shls, atm, bas, env, dims=None):
+    return raw_call("int2c2e_sph", shls, atm, bas, env, dims)
 
 
 def eval_raw(api, out, dims, shls, atm, bas, env, cache=None):
@@ -346,9 +357,8 @@ class Plan:
         return self._tuples
 
     def block_size(self, shls):
-        arity = 3 if self.family.startswith("int3c2e") else 4
         n = 3 if "ip1" in self.family else 1
-        for s in shls[:arity]:
+        for s in shls[:arity(self.family)]:
             n *= (2 * int(self._bas[s][1]) + 1) * int(self._bas[s][3])
         return n
 

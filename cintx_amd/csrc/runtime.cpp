@@ -97,16 +97,18 @@ static int guarded(F&& f) {
 // Families
 // ------------------------------------------------------------------
 struct FamilyInfo {
-  bool three;
+  bool three;  // the ket is one Gaussian (shell k alone)
   int deriv;
   int ncomp;
   int arity;
   const char* name;
+  bool two;  // the bra is one Gaussian too (shell i alone): (i|k)
 };
-static const FamilyInfo kFam[4] = {{false, 0, 1, 4, "int2e_sph"},
-                                   {false, 1, 3, 4, "int2e_ip1_sph"},
-                                   {true, 0, 1, 3, "int3c2e_sph"},
-                                   {true, 1, 3, 3, "int3c2e_ip1_sph"}};
+static const FamilyInfo kFam[5] = {{false, 0, 1, 4, "int2e_sph", false},
+                                   {false, 1, 3, 4, "int2e_ip1_sph", false},
+                                   {true, 0, 1, 3, "int3c2e_sph", false},
+                                   {true, 1, 3, 3, "int3c2e_ip1_sph", false},
+                                   {true, 0, 1, 2, "int2c2e_sph", true}};
 
 static constexpr double kExpCutoff = 60.0;     // cint_config.h.in EXPCUTOFF
 static constexpr double kMinExpCutoff = 40.0;  // cint_config.h.in MIN_EXPCUTOFF
@@ -461,6 +463,13 @@ namespace cx {
 static void canon_perm(int fam, const int* l, int p[4]) {
   for (int c = 0; c < 4; ++c) p[c] = c;
   const FamilyInfo& f = kFam[fam];
+  if (f.two) {
+    // (i|k): i in the i slot, k in the k slot, the j and l slots empty
+    // (g2c2e.c:76-84 sets aj = al = 0 and uses the 2e recurrences)
+    p[1] = 2;
+    p[2] = 1;
+    return;
+  }
   if (!f.deriv && l[0] < l[1]) std::swap(p[0], p[1]);
   if (!f.three && l[2] < l[3]) std::swap(p[2], p[3]);
   if (!f.three && !f.deriv) {
@@ -505,7 +514,7 @@ static void plan_add_tuples(cintx_plan* P, const int* tuples, int64_t nt, int pr
     int cl[4], cn[4], cs[4];
     for (int c = 0; c < 4; ++c) {
       int a = tr.perm[c];
-      bool dummy = f.three && a == 3;
+      bool dummy = a >= f.arity;
       cl[c] = dummy ? 0 : l[a];
       cn[c] = dummy ? 1 : B.nctr[tr.shl[a]];
       cs[c] = dummy ? -1 : tr.shl[a];
@@ -535,8 +544,9 @@ static void plan_add_tuples(cintx_plan* P, const int* tuples, int64_t nt, int pr
       for (int c2 = 0; c2 < cn[2]; ++c2)
         for (int c1 = 0; c1 < cn[1]; ++c1)
           for (int c0 = 0; c0 < cn[0]; ++c0) {
-            int v0 = B.first_v[cs[0]] + c0, v1 = B.first_v[cs[1]] + c1;
-            int bra = P->pairs.add(B, v0, v1);
+            int v0 = B.first_v[cs[0]] + c0;
+            int bra = f.two ? P->pairs.add_pseudo(B, v0)
+                            : P->pairs.add(B, v0, B.first_v[cs[1]] + c1);
             int ket;
             if (f.three)
               ket = P->pairs.add_pseudo(B, B.first_v[cs[2]] + c2);
@@ -739,7 +749,8 @@ static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count) {
   a.pair_off = P->d_poff;
   a.pair_np = P->d_pnp;
   a.pair_geo = P->d_geo;
-  a.common = common_base * common_fac_sp(b.l[0]) * common_fac_sp(b.l[1]) *
+  // one CINTcommon_fac_sp per real shell (g2e.c:54-56, g3c2e.c, g2c2e.c:44-45)
+  a.common = common_base * common_fac_sp(b.l[0]) * (f.two ? 1.0 : common_fac_sp(b.l[1])) *
              common_fac_sp(b.l[2]) * (f.three ? 1.0 : common_fac_sp(b.l[3]));
   a.expcutoff = P->expcutoff;
   a.lij_ceil = b.l[0] + b.l[1] + f.deriv;
@@ -787,7 +798,7 @@ static void plan_launch_all(cintx_plan* P, double* out, bool count,
 
 static cintx_plan* plan_new(int family, const int* atm, int natm, const int* bas, int nbas,
                             const double* env, size_t env_len) {
-  if (family < 0 || family > 3) fail(CINTX_STATUS_UNSUPPORTED_API, "unknown family");
+  if (family < 0 || family > 4) fail(CINTX_STATUS_UNSUPPORTED_API, "unknown family");
   cintx_plan* P = new cintx_plan();
   P->family = family;
   try {
@@ -831,7 +842,7 @@ static void plan_to_natural(const cintx_plan* P, const double* canon, double* na
         ns[c] = nsph(b.l[c]);
         int a = tr.perm[c];
         base[c] = vq.cc[c] * ns[c];
-        st[c] = (f.three && a == 3) ? 0 : so[a];
+        st[c] = a >= f.arity ? 0 : so[a];
       }
       int64_t nsall = (int64_t)ns[0] * ns[1] * ns[2] * ns[3];
       for (int comp = 0; comp < f.ncomp; ++comp) {
@@ -868,6 +879,20 @@ static int cache_size(int fam, const int* shls, const int* bas) {
   int gbits = f.deriv ? 1 : 0;
   int lic = l[0] + f.deriv, ljc = l[1];
   size_t g_size;
+  if (f.two) {
+    // int1e_cache_size (cint1e.c:163-183) as CINT2c2e_drv uses it
+    // (cint2c2e.c:245): shells (i, k), nf = nfi nfk, g_size over (li, lk)
+    int li2 = l[0], lk2 = l[1];
+    int nroots = (li2 + lk2) / 2 + 1;
+    g_size = (size_t)nroots * (li2 + 1) * (lk2 + 1);
+    size_t nf2 = (size_t)ncart(li2) * ncart(lk2);
+    size_t nc2 = nf2 * nc[0] * nc[1];
+    size_t leng = g_size * 3 * 2;
+    size_t lenj = nf2 * nc2, leni = nf2 * nc[0], len0 = nf2;
+    size_t pdata = (size_t)np[0] * np[1] * 5 + np[0] * nc[0] + np[1] * nc[1] +
+                   (np[0] + np[1]) * 2 + nf2 * 3;
+    return (int)std::max(nc2 + leng + lenj + leni + len0 + pdata, nc2 + nf2 * 8 * 2);
+  }
   if (!f.three) {
     int lkc = l[2], llc = l[3];
     int nroots = (lic + ljc + lkc + llc) / 2 + 1;
@@ -913,7 +938,7 @@ static int eval_one(int fam, double* out, const int* dims, const int* shls, cons
     cintx_plan* p;
     ~Guard() { plan_free(p); }
   } guard{P};
-  int tup[4] = {shls[0], shls[1], shls[2], f.arity == 4 ? shls[3] : 0};
+  int tup[4] = {shls[0], shls[1], f.arity >= 3 ? shls[2] : 0, f.arity == 4 ? shls[3] : 0};
   plan_add_tuples(P, tup, 1, CINTX_PREC_FP64);
   plan_check_kernels(P);
   plan_upload(P);
@@ -960,6 +985,8 @@ static int family_of_api(int api) {
       return 2;
     case CINTX_API_INT3C2E_IP1_SPH:
       return 3;
+    case CINTX_API_INT2C2E_SPH:
+      return 4;
     default:
       return -1;
   }
@@ -1028,6 +1055,13 @@ int int3c2e_ip1_sph(double* out, int* dims, int* shls, int* atm, int natm, int* 
   return raw_entry(3, out, dims, shls, atm, natm, bas, nbas, env);
 }
 
+int int2c2e_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
+                double* env, CINTOpt* opt, double* cache) {
+  (void)opt;
+  (void)cache;
+  return raw_entry(4, out, dims, shls, atm, natm, bas, nbas, env);
+}
+
 static void make_opt(CINTOpt** opt, int fam) {
   if (!opt) return;
   *opt = new CINTOpt{fam};
@@ -1036,6 +1070,7 @@ void int2e_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(op
 void int2e_ip1_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 1); }
 void int3c2e_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 2); }
 void int3c2e_ip1_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 3); }
+void int2c2e_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 4); }
 void CINTdel_optimizer(CINTOpt** opt) {
   if (opt && *opt) {
     delete *opt;
@@ -1340,7 +1375,7 @@ int cintx_plan_copy_tuples(cintx_plan* plan, const int64_t* which, int64_t n, do
           ns[c] = nsph(b.l[c]);
           int a = tr.perm[c];
           base[c] = vq.cc[c] * ns[c];
-          st[c] = (f.three && a == 3) ? 0 : so[a];
+          st[c] = a >= f.arity ? 0 : so[a];
         }
         int64_t nsall = (int64_t)ns[0] * ns[1] * ns[2] * ns[3];
         for (int comp = 0; comp < f.ncomp; ++comp) {

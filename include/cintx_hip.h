@@ -49,6 +49,10 @@ int int3c2e_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas,
 /* libcint-master/src/autocode/int3c2e.c:80-87 */
 int int3c2e_ip1_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
                     double *env, CINTOpt *opt, double *cache);
+/* libcint-master/src/cint2c2e.c:351-359 (the density-fitting metric (i|k);
+ * shls holds two shells, out is [dk][di]) */
+int int2c2e_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                double *env, CINTOpt *opt, double *cache);
 
 /* libcint-master/src/cint2e.c:1195, autocode/grad2.c:69,
  * cint3c2e.c:702, autocode/int3c2e.c:68 */
@@ -56,6 +60,8 @@ void int2e_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, doub
 void int2e_ip1_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
 void int3c2e_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
 void int3c2e_ip1_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
+/* libcint-master/src/cint2c2e.c:360 */
+void int2c2e_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
 /* libcint-master/src/optimizer.c:69 */
 void CINTdel_optimizer(CINTOpt **opt);
 
@@ -69,6 +75,7 @@ void CINTdel_optimizer(CINTOpt **opt);
 #define CINTX_API_INT3C2E_IP1_SPH 19
 #define CINTX_API_INT2E_IP1_SPH 36
 #define CINTX_API_INT3C2E_SPH 40
+#define CINTX_API_INT2C2E_SPH 13
 
 /* status codes: crates/cintx-capi/src/errors.rs:9-21 */
 #define CINTX_STATUS_SUCCESS 0

@@ -19,7 +19,8 @@
  *   contraction                   cint2e.c PRIM2CTR
  *   cart -> sph                   cart2sph.c:21 (table), 5324-5376, 5884-5923
  *   entry points / zero fill      cint2e.c:794-869, 1186-1193,
- *                                 cint3c2e.c:556-625, 693-700
+ *                                 cint3c2e.c:556-625, 693-700,
+ *                                 cint2c2e.c:58-150, 238-300, 351-359
  *
  * The memory layout of the intermediate g tensor is this file's own
  * ([axis][root][i][j][k][l]); only the arithmetic and the output layout follow
@@ -544,6 +545,7 @@ typedef struct {
     int three_center; /* int3c2e: ket is one Gaussian on shell k */
     int deriv_i;      /* 1: nabla_i, three components (grad2.c:19-88) */
     int spherical;    /* 1: sph output, 0: cart */
+    int two_center;   /* int2c2e: the bra is one Gaussian on shell i too */
 } Family;
 
 typedef struct {
@@ -630,10 +632,23 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
                         const int *atm, const int *bas, const double *env)
 {
     Shl s[4];
-    int nsh = fam->three_center ? 3 : 4;
-    for (int a = 0; a < nsh; a++) get_shl(&s[a], shls[a], atm, bas, env);
+    int nsh = fam->two_center ? 2 : fam->three_center ? 3 : 4;
     static const double zero3[3] = {0, 0, 0};
     static const double one = 1.0;
+    if (fam->two_center) {
+        /* (i|k): i in the i slot, k in the k slot, exponent-0 s "shells" in
+         * the j and l slots (g2c2e.c:76-84 sets aj = al = 0) */
+        get_shl(&s[0], shls[0], atm, bas, env);
+        get_shl(&s[2], shls[1], atm, bas, env);
+        s[1].l = 0;
+        s[1].nprim = 1;
+        s[1].nctr = 1;
+        s[1].r = s[0].r;
+        s[1].a = zero3;
+        s[1].c = &one;
+    } else {
+        for (int a = 0; a < nsh; a++) get_shl(&s[a], shls[a], atm, bas, env);
+    }
     if (fam->three_center) {
         /* a fourth "shell": s-type, exponent 0, coefficient 1, on k's centre */
         s[3].l = 0;
@@ -650,7 +665,12 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
     for (int a = 0; a < 4; a++)
         cnt[a] = (fam->spherical ? nsph(lv[a]) : ncart(lv[a])) * nctr[a];
     int dd[4];
-    if (dims) {
+    if (dims && fam->two_center) {
+        dd[0] = dims[0];
+        dd[1] = 1;
+        dd[2] = dims[1];
+        dd[3] = 1;
+    } else if (dims) {
         for (int a = 0; a < nsh; a++) dd[a] = dims[a];
         if (nsh == 3) dd[3] = 1;
     } else {
@@ -672,7 +692,8 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
     gd.rl = s[3].r;
 
     double common = (M_PI * M_PI * M_PI) * 2 / sqrt(M_PI) * common_fac_sp(lv[0]) *
-                    common_fac_sp(lv[1]) * common_fac_sp(lv[2]);
+                    common_fac_sp(lv[2]);
+    if (!fam->two_center) common *= common_fac_sp(lv[1]); /* g2c2e.c:44-45: i and k only */
     if (!fam->three_center) common *= common_fac_sp(lv[3]);
     double expcutoff;
     if (env[ORC_PTR_EXPCUTOFF] == 0)
@@ -696,6 +717,16 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
     double *blk = malloc(sizeof(double) * nfall);
 
     int empty = set_pairdata(pdij, &s[0], &s[1], gd.li, gd.lj, expcutoff);
+    if (fam->two_center) {
+        /* CINT2c2e_loop_nopt (cint2c2e.c:58-150) has no screen: every i
+         * primitive enters with fac = common c_i c_k */
+        for (int ip = 0; ip < s[0].nprim; ip++) {
+            pdij[ip].cce = -HUGE_VAL;
+            pdij[ip].e = 1.0;
+            for (int d = 0; d < 3; d++) pdij[ip].P[d] = s[0].r[d];
+        }
+        empty = 0;
+    }
     int not0 = 0;
     if (!empty) {
         int ix[4][NCART_MAX], iy[4][NCART_MAX], iz[4][NCART_MAX];
@@ -877,15 +908,24 @@ int orc_int3c2e_ip1_sph(double *out, int *dims, int *shls, int *atm, int natm, i
     Family f = {1, 1, 1};
     return tuple_driver(&f, out, dims, shls, atm, bas, env);
 }
+/* cint2c2e.c:351-359, CINT2c2e_drv 238-300, loop 58-150 */
+int orc_int2c2e_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                    double *env, void *opt, double *cache)
+{
+    (void)natm; (void)nbas; (void)opt; (void)cache;
+    if (!out) return 0;
+    Family f = {1, 0, 1, 1};
+    return tuple_driver(&f, out, dims, shls, atm, bas, env);
+}
 
 typedef int (*orc_fn)(double *, int *, int *, int *, int, int *, int, double *, void *, double *);
 
 long orc_eval_list(int api, int nq, const int *shls_list, const long *offsets, double *out,
                    int *atm, int natm, int *bas, int nbas, double *env, int nthreads)
 {
-    orc_fn fns[5] = {orc_int2e_sph, orc_int2e_ip1_sph, orc_int3c2e_sph, orc_int3c2e_ip1_sph,
-                     orc_int2e_cart};
-    if (api < 0 || api > 4) return -1;
+    orc_fn fns[6] = {orc_int2e_sph, orc_int2e_ip1_sph, orc_int3c2e_sph, orc_int3c2e_ip1_sph,
+                     orc_int2e_cart, orc_int2c2e_sph};
+    if (api < 0 || api > 5) return -1;
     orc_fn fn = fns[api];
     ensure_c2s();
     long nnz = 0;
@@ -910,12 +950,18 @@ void orc_work_model(int api, int *shls, int *atm, int natm, int *bas, int nbas, 
                     double *counts)
 {
     (void)natm; (void)nbas;
-    Family fam = {api == 2 || api == 3, api == 1 || api == 3, api != 4};
+    Family fam = {api == 2 || api == 3 || api == 5, api == 1 || api == 3, api != 4, api == 5};
     Shl s[4];
     int nsh = fam.three_center ? 3 : 4;
-    for (int a = 0; a < nsh; a++) get_shl(&s[a], shls[a], atm, bas, env);
     static const double zero3[3] = {0, 0, 0};
     static const double one = 1.0;
+    if (fam.two_center) {
+        get_shl(&s[0], shls[0], atm, bas, env);
+        get_shl(&s[2], shls[1], atm, bas, env);
+        s[1].l = 0; s[1].nprim = 1; s[1].nctr = 1; s[1].r = s[0].r; s[1].a = zero3; s[1].c = &one;
+    } else {
+        for (int a = 0; a < nsh; a++) get_shl(&s[a], shls[a], atm, bas, env);
+    }
     if (fam.three_center) {
         s[3].l = 0; s[3].nprim = 1; s[3].nctr = 1; s[3].r = s[2].r; s[3].a = zero3; s[3].c = &one;
     }
@@ -931,7 +977,12 @@ void orc_work_model(int api, int *shls, int *atm, int natm, int *bas, int nbas, 
     }
     PairPrim *pd = malloc(sizeof(PairPrim) * s[0].nprim * s[1].nprim);
     long nprimq = 0;
-    if (!set_pairdata(pd, &s[0], &s[1], Lc[0], Lc[1], expcutoff)) {
+    int pd_empty = set_pairdata(pd, &s[0], &s[1], Lc[0], Lc[1], expcutoff);
+    if (fam.two_center) { /* no screen (cint2c2e.c:58-150) */
+        for (int ip = 0; ip < s[0].nprim; ip++) pd[ip].cce = -HUGE_VAL;
+        pd_empty = 0;
+    }
+    if (!pd_empty) {
         double rr_kl = 0;
         for (int d = 0; d < 3; d++) rr_kl += (s[2].r[d] - s[3].r[d]) * (s[2].r[d] - s[3].r[d]);
         double lck[MAXPRIM], lcl[MAXPRIM];

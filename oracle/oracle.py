@@ -41,7 +41,7 @@ def lib():
         L.orc_c2s_coef.restype = ctypes.c_double
         sig = [dp, ip, ip, ip, ctypes.c_int, ip, ctypes.c_int, dp, ctypes.c_void_p, dp]
         for name in ("orc_int2e_sph", "orc_int2e_cart", "orc_int2e_ip1_sph",
-                     "orc_int3c2e_sph", "orc_int3c2e_ip1_sph"):
+                     "orc_int3c2e_sph", "orc_int3c2e_ip1_sph", "orc_int2c2e_sph"):
             getattr(L, name).argtypes = sig
             getattr(L, name).restype = ctypes.c_int
         L.orc_eval_list.argtypes = [ctypes.c_int, ctypes.c_int, ip,
@@ -90,9 +90,10 @@ _FN = {
     "int2e_ip1_sph": "orc_int2e_ip1_sph",
     "int3c2e_sph": "orc_int3c2e_sph",
     "int3c2e_ip1_sph": "orc_int3c2e_ip1_sph",
+    "int2c2e_sph": "orc_int2c2e_sph",
 }
 _API = {"int2e_sph": 0, "int2e_ip1_sph": 1, "int3c2e_sph": 2, "int3c2e_ip1_sph": 3,
-        "int2e_cart": 4}
+        "int2e_cart": 4, "int2c2e_sph": 5}
 
 
 def ncomp(name):
@@ -106,7 +107,7 @@ def shell_dim(bas, sh, cart=False):
 
 
 def tuple_dims(name, bas, shls):
-    nsh = 3 if name.startswith("int3c2e") else 4
+    nsh = 2 if name.startswith("int2c2e") else 3 if name.startswith("int3c2e") else 4
     cart = name.endswith("cart")
     return [shell_dim(bas, s, cart) for s in shls[:nsh]]
 

@@ -115,6 +115,19 @@ def test_int3c2e_ip1_sph_cfg4_sampled(oracle):
     check_plan(oracle, "int3c2e_ip1_sph", tuples, mol)
 
 
+def test_int2c2e_sph_cfg4_aux_metric(oracle):
+    # every (P|Q) of the cfg-4 auxiliary basis (s..g on 64 centres)
+    mol = M.config(4)
+    aux = np.arange(mol.aux_start, mol.nbas)
+    rng = np.random.default_rng(9)
+    tuples = [(int(i), int(k)) for i in aux for k in aux]
+    tuples = [tuples[j] for j in rng.permutation(len(tuples))[:20000]]
+    check_plan(oracle, "int2c2e_sph", tuples, mol)
+    gc = M.general_contraction_h2()
+    check_plan(oracle, "int2c2e_sph", [(i, k) for i in range(gc.nbas) for k in range(gc.nbas)],
+               gc)
+
+
 def test_raw_signature_dims_and_general_contraction(oracle):
     mol = M.general_contraction_h2()
     low = [s for s in range(mol.nbas) if mol.bas[s][1] <= 2]
@@ -131,12 +144,20 @@ def test_raw_signature_dims_and_general_contraction(oracle):
         rd, _ = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env, dims=dims)
         assert_parity(gd, rd)
     shls = [int(low[0]), int(low[1]), int(low[2])]
-    for name in ("int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph"):
+    for name in ("int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph"):
         sh = shls + [int(low[3])] if name.startswith("int2e") else shls
+        sh = sh[:capi.arity(name)]
         got, not0 = capi.raw_call(name, sh, mol.atm, mol.bas, mol.env)
         ref, rnot0 = oracle.eval_tuple(name, sh, mol.atm, mol.bas, mol.env)
         assert not0 == rnot0
         assert_parity(got, ref, np.abs(ref).max() if "ip1" in name else None)
+    for _ in range(10):
+        sh = [int(low[i]) for i in rng.integers(0, len(low), 2)]
+        dims = [d + 2 for d in capi.natural_dims("int2c2e_sph", sh, mol.bas)]
+        gd, not0 = capi.int2c2e_sph(sh, mol.atm, mol.bas, mol.env, dims=dims)
+        rd, rnot0 = oracle.eval_tuple("int2c2e_sph", sh, mol.atm, mol.bas, mol.env, dims=dims)
+        assert not0 == rnot0
+        assert_parity(gd, rd)
 
 
 def test_cintrs_eval_on_device(oracle):

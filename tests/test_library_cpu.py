@@ -46,6 +46,12 @@ def test_cache_size_query_matches_libcint_formula():
     # int3c2e (ss|s): g_size 1, leng 6, len0 1, nc 1 -> 3,
     # pdata 5 + 1+1+1 + 4 + 1 + 3 + 16 = 32 -> 42
     assert capi.cache_size("int3c2e_sph", [0, 0, 0], mol.atm, mol.bas, mol.env) == 42
+    # int2c2e uses int1e_cache_size (cint1e.c:163-183). (s|s): g_size 1,
+    # leng 6, nf 1, nc 1, lenj 1, leni 1, len0 1, pdata 5+1+1+4+3=14 -> 24
+    assert capi.cache_size("int2c2e_sph", [0, 0], mol.atm, mol.bas, mol.env) == 24
+    # (p|s), p with 2 primitives x 2 contractions: g_size 2, leng 12, nf 3,
+    # nc 6, lenj 18, leni 6, len0 3, pdata 10+4+1+6+9=30 -> 75 (> 6+48)
+    assert capi.cache_size("int2c2e_sph", [1, 0], mol.atm, mol.bas, mol.env) == 75
 
 
 def _eval(api, out, dims, shls, mol):

@@ -1,5 +1,5 @@
 """Pins of the oracle's integrals (int2e_sph, int2e_ip1_sph, int3c2e_sph,
-int3c2e_ip1_sph) against checks that do not reuse its algorithm:
+int3c2e_ip1_sph, int2c2e_sph) against checks that do not reuse its algorithm:
 
 * an independent McMurchie-Davidson implementation (tests/md_eri.py) with
   numerically fitted spherical harmonics;
@@ -7,7 +7,9 @@ int3c2e_ip1_sph) against checks that do not reuse its algorithm:
 * 8-fold permutational symmetry of the full cfg-1 tensor;
 * central finite differences in the nuclear coordinate (ip1 = -d/dA_i) and
   translational invariance;
-* int3c2e as int2e with a zero-exponent s function on the third centre;
+* int3c2e as int2e with a zero-exponent s function on the third centre, and
+  int2c2e as int2e with zero-exponent s functions in the j and l slots;
+* the (s|s) Coulomb-metric closed form;
 * linearity in the contraction coefficients (general contraction layout).
 """
 import math
@@ -148,6 +150,49 @@ def test_int3c2e_equals_int2e_with_unit_function(oracle):
         three, _ = block(oracle, "int3c2e_sph", [i, j, k], mol.atm, mol.bas, mol.env)
         four, _ = block(oracle, "int2e_sph", [i, j, k, m2.nbas - 1], m2.atm, m2.bas, m2.env)
         assert_close(three[0], four[0][..., 0] / 0.282094791773878143, atol=1e-13)
+
+
+def test_int2c2e_ss_closed_form(oracle):
+    mol = M.Molecule()
+    ra, rb = (0.2, -0.1, 0.3), (1.0, 0.6, -0.4)
+    a0, a1 = mol.add_atom(1, ra), mol.add_atom(1, rb)
+    mol.add_shell(a0, 0, [1.7], [[0.8]])
+    mol.add_shell(a1, 0, [0.35], [[1.3]])
+    mol.build()
+    got, _ = block(oracle, "int2c2e_sph", [0, 1], mol.atm, mol.bas, mol.env)
+    a, b = 1.7, 0.35
+    R2 = float(np.dot(np.subtract(ra, rb), np.subtract(ra, rb)))
+    T = a * b / (a + b) * R2
+    f0 = 0.5 * math.sqrt(math.pi / T) * math.erf(math.sqrt(T))
+    ref = 0.8 * 1.3 * 2 * math.pi ** 2.5 / (a * b * math.sqrt(a + b)) * f0
+    ref *= 0.282094791773878143 ** 2  # s normalisation of the two shells
+    assert abs(got[0, 0, 0] - ref) <= 1e-15 * ref
+    # symmetric metric: (i|k) = (k|i)
+    back, _ = block(oracle, "int2c2e_sph", [1, 0], mol.atm, mol.bas, mol.env)
+    assert back[0, 0, 0] == pytest.approx(got[0, 0, 0], rel=1e-15)
+
+
+def test_int2c2e_equals_int2e_with_unit_functions(oracle):
+    mol = M.config(4)
+    gc = M.general_contraction_h2()
+    rng = np.random.default_rng(12)
+    cases = [(mol, int(i), int(k)) for i, k in rng.integers(mol.aux_start, mol.nbas, (14, 2))]
+    cases += [(gc, int(i), int(k)) for i, k in rng.integers(0, gc.nbas, (8, 2))]
+    for m, i, k in cases:
+        m2 = M.Molecule()
+        m2.coords, m2.charges = list(m.coords), list(m.charges)
+        m2.shells = list(m.shells)
+        m2.add_shell(m.shells[i][0], 0, [0.0], [[1.0]])
+        m2.add_shell(m.shells[k][0], 0, [0.0], [[1.0]])
+        m2.build()
+        two, _ = block(oracle, "int2c2e_sph", [i, k], m.atm, m.bas, m.env)
+        four, _ = block(oracle, "int2e_sph", [i, m2.nbas - 2, k, m2.nbas - 1], m2.atm, m2.bas,
+                        m2.env)
+        ref = four[0][:, 0, :, 0] / 0.282094791773878143 ** 2
+        assert_close(two[0], ref, atol=1e-13 * max(1.0, np.abs(ref).max()))
+        # the metric is symmetric: (i|k) = (k|i)^T
+        back, _ = block(oracle, "int2c2e_sph", [k, i], m.atm, m.bas, m.env)
+        assert_close(back[0], two[0].T, atol=1e-13 * max(1.0, np.abs(ref).max()))
 
 
 @pytest.mark.parametrize("ls", [(0, 0, 1), (1, 2, 3), (2, 0, 4)])
