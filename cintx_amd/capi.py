@@ -30,7 +30,7 @@ INT3C2E_SPH = 40
 INT2C2E_SPH = 13
 
 FAMILY = {"int2e_sph": 0, "int2e_ip1_sph": 1, "int3c2e_sph": 2, "int3c2e_ip1_sph": 3,
-          "int2c2e_sph": 4}
+          "int2c2e_sph": 4, "int2e_ip2_sph": 5, "int3c2e_ip2_sph": 6}
 API_OF_FAMILY = {0: INT2E_SPH, 1: INT2E_IP1_SPH, 2: INT3C2E_SPH, 3: INT3C2E_IP1_SPH,
                  4: INT2C2E_SPH}
 PREC_FP64 = 0
@@ -46,8 +46,9 @@ STATUS = {
 
 EXPORTS = [
     "int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph",
+    "int2e_ip2_sph", "int3c2e_ip2_sph",
     "int2e_optimizer", "int2e_ip1_optimizer", "int3c2e_optimizer", "int3c2e_ip1_optimizer",
-    "int2c2e_optimizer",
+    "int2c2e_optimizer", "int2e_ip2_optimizer", "int3c2e_ip2_optimizer",
     "CINTdel_optimizer",
     "cintrs_query_workspace", "cintrs_eval", "cintrs_last_error_code",
     "cintrs_clear_last_error", "cintrs_copy_last_error_message",
@@ -91,7 +92,8 @@ def lib():
             raise ImportError(f"{LIB_PATH} is not built; run python -m cintx_amd.build")
         L = ctypes.CDLL(LIB_PATH)
         raw = [DP, IP, IP, IP, ctypes.c_int, IP, ctypes.c_int, DP, ctypes.c_void_p, DP]
-        for n in ("int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph"):
+        for n in ("int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph",
+                  "int2e_ip2_sph", "int3c2e_ip2_sph"):
             getattr(L, n).argtypes = raw
             getattr(L, n).restype = ctypes.c_int
         sz = ctypes.c_size_t
@@ -167,7 +169,7 @@ def natural_dims(family, shls, bas):
 
 
 def ncomp(family):
-    return 3 if "ip1" in family else 1
+    return 3 if "_ip" in family else 1
 
 
 def raw_call(family, shls, atm, bas, env, dims=None):
@@ -357,7 +359,7 @@ class Plan:
         return self._tuples
 
     def block_size(self, shls):
-        n = 3 if "ip1" in self.family else 1
+        n = 3 if "_ip" in self.family else 1
         for s in shls[:arity(self.family)]:
             n *= (2 * int(self._bas[s][1]) + 1) * int(self._bas[s][3])
         return n

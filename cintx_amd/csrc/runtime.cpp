@@ -96,19 +96,30 @@ static int guarded(F&& f) {
 // ------------------------------------------------------------------
 // Families
 // ------------------------------------------------------------------
+// Canonical slots (0..3) are the kernel's (i j|k l); canon_perm says which
+// argument shell sits in each.  `three`/`two` describe the canonical ket/bra,
+// `deriv` puts nabla on canonical slot 0, `dcenter` names the argument that
+// carries it in the libcint call (-1: none).
 struct FamilyInfo {
-  bool three;  // the ket is one Gaussian (shell k alone)
+  bool three;  // the canonical ket is one Gaussian
   int deriv;
   int ncomp;
   int arity;
   const char* name;
-  bool two;  // the bra is one Gaussian too (shell i alone): (i|k)
+  bool two;     // the canonical bra is one Gaussian
+  int dcenter;  // argument shell under nabla
 };
-static const FamilyInfo kFam[5] = {{false, 0, 1, 4, "int2e_sph", false},
-                                   {false, 1, 3, 4, "int2e_ip1_sph", false},
-                                   {true, 0, 1, 3, "int3c2e_sph", false},
-                                   {true, 1, 3, 3, "int3c2e_ip1_sph", false},
-                                   {true, 0, 1, 2, "int2c2e_sph", true}};
+static constexpr int kNumFamilies = 7;
+static const FamilyInfo kFam[kNumFamilies] = {
+    {false, 0, 1, 4, "int2e_sph", false, -1},
+    {false, 1, 3, 4, "int2e_ip1_sph", false, 0},
+    {true, 0, 1, 3, "int3c2e_sph", false, -1},
+    {true, 1, 3, 3, "int3c2e_ip1_sph", false, 0},
+    {true, 0, 1, 2, "int2c2e_sph", true, -1},
+    // (ij|nabla k l) = (nabla k l|ij): the ip1 kernels with (k l) as the bra
+    {false, 1, 3, 4, "int2e_ip2_sph", false, 2},
+    // (ij|nabla k): bra = k alone, ket = the (i j) pair
+    {false, 1, 3, 3, "int3c2e_ip2_sph", true, 2}};
 
 static constexpr double kExpCutoff = 60.0;     // cint_config.h.in EXPCUTOFF
 static constexpr double kMinExpCutoff = 40.0;  // cint_config.h.in MIN_EXPCUTOFF
@@ -463,11 +474,21 @@ namespace cx {
 static void canon_perm(int fam, const int* l, int p[4]) {
   for (int c = 0; c < 4; ++c) p[c] = c;
   const FamilyInfo& f = kFam[fam];
-  if (f.two) {
+  if (f.arity == 2) {
     // (i|k): i in the i slot, k in the k slot, the j and l slots empty
     // (g2c2e.c:76-84 sets aj = al = 0 and uses the 2e recurrences)
     p[1] = 2;
     p[2] = 1;
+    return;
+  }
+  if (f.dcenter == 2) {
+    // nabla on k (grad2.c:99-101, autocode/int3c2e.c:98-100): the (k l)
+    // side becomes the canonical bra, (i j) the ket, swapped to li >= lj
+    p[0] = 2;
+    p[1] = 3;
+    p[2] = 0;
+    p[3] = 1;
+    if (l[0] < l[1]) std::swap(p[2], p[3]);
     return;
   }
   if (!f.deriv && l[0] < l[1]) std::swap(p[0], p[1]);
@@ -492,7 +513,7 @@ static double fam_expcutoff(int fam, const double* env) {
   double e0 = env[0];
   if (e0 == 0) return kExpCutoff;
   double c = std::max(kMinExpCutoff, e0);
-  if (!kFam[fam].three) c += 1.0;  // g2e.c:57-62 vs g3c2e.c:56-60
+  if (kFam[fam].arity == 4) c += 1.0;  // g2e.c:57-62 vs g3c2e.c:56-60, g2c2e.c:46-50
   return c;
 }
 
@@ -798,7 +819,7 @@ static void plan_launch_all(cintx_plan* P, double* out, bool count,
 
 static cintx_plan* plan_new(int family, const int* atm, int natm, const int* bas, int nbas,
                             const double* env, size_t env_len) {
-  if (family < 0 || family > 4) fail(CINTX_STATUS_UNSUPPORTED_API, "unknown family");
+  if (family < 0 || family >= kNumFamilies) fail(CINTX_STATUS_UNSUPPORTED_API, "unknown family");
   cintx_plan* P = new cintx_plan();
   P->family = family;
   try {
@@ -877,9 +898,11 @@ static int cache_size(int fam, const int* shls, const int* bas) {
   size_t nctr = (size_t)nc[0] * nc[1] * nc[2] * nc[3];
   size_t ncomp = f.ncomp;
   int gbits = f.deriv ? 1 : 0;
-  int lic = l[0] + f.deriv, ljc = l[1];
+  int inc[4] = {0, 0, 0, 0};  // ng[IINC..LINC]
+  if (f.dcenter >= 0) inc[f.dcenter] = 1;
+  int lic = l[0] + inc[0], ljc = l[1] + inc[1];
   size_t g_size;
-  if (f.two) {
+  if (f.arity == 2) {
     // int1e_cache_size (cint1e.c:163-183) as CINT2c2e_drv uses it
     // (cint2c2e.c:245): shells (i, k), nf = nfi nfk, g_size over (li, lk)
     int li2 = l[0], lk2 = l[1];
@@ -893,8 +916,8 @@ static int cache_size(int fam, const int* shls, const int* bas) {
                    (np[0] + np[1]) * 2 + nf2 * 3;
     return (int)std::max(nc2 + leng + lenj + leni + len0 + pdata, nc2 + nf2 * 8 * 2);
   }
-  if (!f.three) {
-    int lkc = l[2], llc = l[3];
+  if (f.arity == 4) {
+    int lkc = l[2] + inc[2], llc = l[3] + inc[3];
     int nroots = (lic + ljc + lkc + llc) / 2 + 1;
     bool ib = lic > ljc, kb = lkc > llc;
     int dli = ib ? lic + ljc + 1 : lic + 1, dlj = ib ? ljc + 1 : lic + ljc + 1;
@@ -909,7 +932,7 @@ static int cache_size(int fam, const int* shls, const int* bas) {
     size_t b = nc_all * ncomp + nf * 4;
     return (int)std::max(a, b);
   } else {
-    int llc = l[2];
+    int llc = l[2] + inc[2];  // g3c2e.c:68-69: k sits in the l slot
     int nroots = (lic + ljc + llc) / 2 + 1;
     bool ib = lic > ljc;
     int dli = ib ? lic + ljc + 1 : lic + 1, dlj = ib ? ljc + 1 : lic + ljc + 1;
@@ -1061,6 +1084,18 @@ int int2c2e_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas,
   (void)cache;
   return raw_entry(4, out, dims, shls, atm, natm, bas, nbas, env);
 }
+int int2e_ip2_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
+                  double* env, CINTOpt* opt, double* cache) {
+  (void)opt;
+  (void)cache;
+  return raw_entry(5, out, dims, shls, atm, natm, bas, nbas, env);
+}
+int int3c2e_ip2_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
+                    double* env, CINTOpt* opt, double* cache) {
+  (void)opt;
+  (void)cache;
+  return raw_entry(6, out, dims, shls, atm, natm, bas, nbas, env);
+}
 
 static void make_opt(CINTOpt** opt, int fam) {
   if (!opt) return;
@@ -1071,6 +1106,8 @@ void int2e_ip1_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_op
 void int3c2e_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 2); }
 void int3c2e_ip1_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 3); }
 void int2c2e_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 4); }
+void int2e_ip2_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 5); }
+void int3c2e_ip2_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 6); }
 void CINTdel_optimizer(CINTOpt** opt) {
   if (opt && *opt) {
     delete *opt;

@@ -53,6 +53,14 @@ int int3c2e_ip1_sph(double *out, int *dims, int *shls, int *atm, int natm, int *
  * shls holds two shells, out is [dk][di]) */
 int int2c2e_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
                 double *env, CINTOpt *opt, double *cache);
+/* libcint-master/src/autocode/grad2.c:163-170 ((ij|nabla k l), out is
+ * [3][dl][dk][dj][di]) */
+int int2e_ip2_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                  double *env, CINTOpt *opt, double *cache);
+/* libcint-master/src/autocode/int3c2e.c:161-168 ((ij|nabla k), out is
+ * [3][dk][dj][di]) */
+int int3c2e_ip2_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                    double *env, CINTOpt *opt, double *cache);
 
 /* libcint-master/src/cint2e.c:1195, autocode/grad2.c:69,
  * cint3c2e.c:702, autocode/int3c2e.c:68 */
@@ -62,6 +70,9 @@ void int3c2e_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, do
 void int3c2e_ip1_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
 /* libcint-master/src/cint2c2e.c:360 */
 void int2c2e_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
+/* libcint-master/src/autocode/grad2.c:145, autocode/int3c2e.c:143 */
+void int2e_ip2_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
+void int3c2e_ip2_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
 /* libcint-master/src/optimizer.c:69 */
 void CINTdel_optimizer(CINTOpt **opt);
 

@@ -546,6 +546,8 @@ typedef struct {
     int deriv_i;      /* 1: nabla_i, three components (grad2.c:19-88) */
     int spherical;    /* 1: sph output, 0: cart */
     int two_center;   /* int2c2e: the bra is one Gaussian on shell i too */
+    int deriv_k;      /* 1: nabla_k, three components (grad2.c:99-170,
+                       * autocode/int3c2e.c:98-168) */
 } Family;
 
 typedef struct {
@@ -628,6 +630,15 @@ static void transform4(double *dst, const double *src, const int lv[4], int sphe
     }
 }
 
+/* g of one axis with the exponent of slot `slot` moved by `delta` */
+static double gshift(const double *G, const GDesc *gd, int axis, int r, const int *p, int slot,
+                     int delta)
+{
+    int q[4] = {p[0], p[1], p[2], p[3]};
+    q[slot] += delta;
+    return G[gidx(gd, axis, r, q[0], q[1], q[2], q[3])];
+}
+
 static int tuple_driver(const Family *fam, double *out, const int *dims, const int *shls,
                         const int *atm, const int *bas, const double *env)
 {
@@ -660,7 +671,7 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
     }
     int lv[4] = {s[0].l, s[1].l, s[2].l, s[3].l};
     int nctr[4] = {s[0].nctr, s[1].nctr, s[2].nctr, s[3].nctr};
-    int ncomp = fam->deriv_i ? 3 : 1;
+    int ncomp = (fam->deriv_i || fam->deriv_k) ? 3 : 1;
     int cnt[4];
     for (int a = 0; a < 4; a++)
         cnt[a] = (fam->spherical ? nsph(lv[a]) : ncart(lv[a])) * nctr[a];
@@ -681,7 +692,7 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
     GDesc gd;
     gd.li = lv[0] + fam->deriv_i;
     gd.lj = lv[1];
-    gd.lk = lv[2];
+    gd.lk = lv[2] + fam->deriv_k;
     gd.ll = lv[3];
     gd.nroots = (gd.li + gd.lj + gd.lk + gd.ll) / 2 + 1;
     gd.ibase = gd.li > gd.lj;
@@ -779,7 +790,7 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
                                         int px[4] = {ix[0][fi], ix[1][fj], ix[2][fk], ix[3][fl]};
                                         int py[4] = {iy[0][fi], iy[1][fj], iy[2][fk], iy[3][fl]};
                                         int pz[4] = {iz[0][fi], iz[1][fj], iz[2][fk], iz[3][fl]};
-                                        if (!fam->deriv_i) {
+                                        if (!fam->deriv_i && !fam->deriv_k) {
                                             double acc = 0;
                                             for (int r = 0; r < gd.nroots; r++)
                                                 acc += G[gidx(&gd, 0, r, px[0], px[1], px[2], px[3])] *
@@ -788,18 +799,21 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
                                             gout[n] = acc;
                                         } else {
                                             /* nabla_i: f(i) = i g(i-1) - 2 ai g(i+1)
-                                             * (g2e.c:4574-4620) */
+                                             * (g2e.c:4574-4620); nabla_k the same on
+                                             * the k index with ak (g2e.c:4675-4720) */
+                                            int ds = fam->deriv_i ? 0 : 2;
+                                            double a2 = fam->deriv_i ? ai2 : -2.0 * ak;
                                             double sx = 0, sy = 0, sz = 0;
                                             for (int r = 0; r < gd.nroots; r++) {
                                                 double gx = G[gidx(&gd, 0, r, px[0], px[1], px[2], px[3])];
                                                 double gy = G[gidx(&gd, 1, r, py[0], py[1], py[2], py[3])];
                                                 double gz = G[gidx(&gd, 2, r, pz[0], pz[1], pz[2], pz[3])];
-                                                double fx = ai2 * G[gidx(&gd, 0, r, px[0] + 1, px[1], px[2], px[3])];
-                                                double fy = ai2 * G[gidx(&gd, 1, r, py[0] + 1, py[1], py[2], py[3])];
-                                                double fz = ai2 * G[gidx(&gd, 2, r, pz[0] + 1, pz[1], pz[2], pz[3])];
-                                                if (px[0] > 0) fx += px[0] * G[gidx(&gd, 0, r, px[0] - 1, px[1], px[2], px[3])];
-                                                if (py[0] > 0) fy += py[0] * G[gidx(&gd, 1, r, py[0] - 1, py[1], py[2], py[3])];
-                                                if (pz[0] > 0) fz += pz[0] * G[gidx(&gd, 2, r, pz[0] - 1, pz[1], pz[2], pz[3])];
+                                                double fx = a2 * gshift(G, &gd, 0, r, px, ds, 1);
+                                                double fy = a2 * gshift(G, &gd, 1, r, py, ds, 1);
+                                                double fz = a2 * gshift(G, &gd, 2, r, pz, ds, 1);
+                                                if (px[ds] > 0) fx += px[ds] * gshift(G, &gd, 0, r, px, ds, -1);
+                                                if (py[ds] > 0) fy += py[ds] * gshift(G, &gd, 1, r, py, ds, -1);
+                                                if (pz[ds] > 0) fz += pz[ds] * gshift(G, &gd, 2, r, pz, ds, -1);
                                                 sx += fx * gy * gz;
                                                 sy += gx * fy * gz;
                                                 sz += gx * gy * fz;
@@ -908,6 +922,24 @@ int orc_int3c2e_ip1_sph(double *out, int *dims, int *shls, int *atm, int natm, i
     Family f = {1, 1, 1};
     return tuple_driver(&f, out, dims, shls, atm, bas, env);
 }
+/* autocode/grad2.c:99-170: (ij|nabla k l) */
+int orc_int2e_ip2_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas,
+                      int nbas, double *env, void *opt, double *cache)
+{
+    (void)natm; (void)nbas; (void)opt; (void)cache;
+    if (!out) return 0;
+    Family f = {0, 0, 1, 0, 1};
+    return tuple_driver(&f, out, dims, shls, atm, bas, env);
+}
+/* autocode/int3c2e.c:98-168: (ij|nabla k) */
+int orc_int3c2e_ip2_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas,
+                        int nbas, double *env, void *opt, double *cache)
+{
+    (void)natm; (void)nbas; (void)opt; (void)cache;
+    if (!out) return 0;
+    Family f = {1, 0, 1, 0, 1};
+    return tuple_driver(&f, out, dims, shls, atm, bas, env);
+}
 /* cint2c2e.c:351-359, CINT2c2e_drv 238-300, loop 58-150 */
 int orc_int2c2e_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
                     double *env, void *opt, double *cache)
@@ -923,9 +955,9 @@ typedef int (*orc_fn)(double *, int *, int *, int *, int, int *, int, double *, 
 long orc_eval_list(int api, int nq, const int *shls_list, const long *offsets, double *out,
                    int *atm, int natm, int *bas, int nbas, double *env, int nthreads)
 {
-    orc_fn fns[6] = {orc_int2e_sph, orc_int2e_ip1_sph, orc_int3c2e_sph, orc_int3c2e_ip1_sph,
-                     orc_int2e_cart, orc_int2c2e_sph};
-    if (api < 0 || api > 5) return -1;
+    orc_fn fns[8] = {orc_int2e_sph,  orc_int2e_ip1_sph, orc_int3c2e_sph,   orc_int3c2e_ip1_sph,
+                     orc_int2e_cart, orc_int2c2e_sph,   orc_int2e_ip2_sph, orc_int3c2e_ip2_sph};
+    if (api < 0 || api > 7) return -1;
     orc_fn fn = fns[api];
     ensure_c2s();
     long nnz = 0;
@@ -950,7 +982,8 @@ void orc_work_model(int api, int *shls, int *atm, int natm, int *bas, int nbas, 
                     double *counts)
 {
     (void)natm; (void)nbas;
-    Family fam = {api == 2 || api == 3 || api == 5, api == 1 || api == 3, api != 4, api == 5};
+    Family fam = {api == 2 || api == 3 || api == 5 || api == 7, api == 1 || api == 3, api != 4,
+                  api == 5, api == 6 || api == 7};
     Shl s[4];
     int nsh = fam.three_center ? 3 : 4;
     static const double zero3[3] = {0, 0, 0};
@@ -966,7 +999,7 @@ void orc_work_model(int api, int *shls, int *atm, int natm, int *bas, int nbas, 
         s[3].l = 0; s[3].nprim = 1; s[3].nctr = 1; s[3].r = s[2].r; s[3].a = zero3; s[3].c = &one;
     }
     int lv[4] = {s[0].l, s[1].l, s[2].l, s[3].l};
-    int Lc[4] = {lv[0] + fam.deriv_i, lv[1], lv[2], lv[3]};
+    int Lc[4] = {lv[0] + fam.deriv_i, lv[1], lv[2] + fam.deriv_k, lv[3]};
     int nr = (Lc[0] + Lc[1] + Lc[2] + Lc[3]) / 2 + 1;
     double expcutoff;
     if (env[ORC_PTR_EXPCUTOFF] == 0)
@@ -1021,10 +1054,13 @@ void orc_work_model(int api, int *shls, int *atm, int natm, int *bas, int nbas, 
         for (int d = 1; d <= D; d++) hk += (C + D - d + 1) * (double)((Lc[0] + 1) * (Lc[1] + 1));
         per += 2.0 * 3.0 * nr * (hb + hk); /* 4D HRR */
     }
-    int ncomp = fam.deriv_i ? 3 : 1;
+    int ncomp = (fam.deriv_i || fam.deriv_k) ? 3 : 1;
     if (fam.deriv_i) {
         per += 9.0 * nf * nr;                                           /* gout_ip1 */
         per += 3.0 * 3.0 * nr * (lv[0] + 1) * (Lc[1] + 1) * (Lc[2] + 1) * (Lc[3] + 1); /* nabla */
+    } else if (fam.deriv_k) {
+        per += 9.0 * nf * nr;                                           /* gout_ip2 */
+        per += 3.0 * 3.0 * nr * (Lc[0] + 1) * (Lc[1] + 1) * (lv[2] + 1) * (Lc[3] + 1); /* nabla */
     } else {
         per += 3.0 * nf * nr; /* gout */
     }

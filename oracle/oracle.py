@@ -41,7 +41,8 @@ def lib():
         L.orc_c2s_coef.restype = ctypes.c_double
         sig = [dp, ip, ip, ip, ctypes.c_int, ip, ctypes.c_int, dp, ctypes.c_void_p, dp]
         for name in ("orc_int2e_sph", "orc_int2e_cart", "orc_int2e_ip1_sph",
-                     "orc_int3c2e_sph", "orc_int3c2e_ip1_sph", "orc_int2c2e_sph"):
+                     "orc_int3c2e_sph", "orc_int3c2e_ip1_sph", "orc_int2c2e_sph",
+                     "orc_int2e_ip2_sph", "orc_int3c2e_ip2_sph"):
             getattr(L, name).argtypes = sig
             getattr(L, name).restype = ctypes.c_int
         L.orc_eval_list.argtypes = [ctypes.c_int, ctypes.c_int, ip,
@@ -91,13 +92,15 @@ _FN = {
     "int3c2e_sph": "orc_int3c2e_sph",
     "int3c2e_ip1_sph": "orc_int3c2e_ip1_sph",
     "int2c2e_sph": "orc_int2c2e_sph",
+    "int2e_ip2_sph": "orc_int2e_ip2_sph",
+    "int3c2e_ip2_sph": "orc_int3c2e_ip2_sph",
 }
 _API = {"int2e_sph": 0, "int2e_ip1_sph": 1, "int3c2e_sph": 2, "int3c2e_ip1_sph": 3,
-        "int2e_cart": 4, "int2c2e_sph": 5}
+        "int2e_cart": 4, "int2c2e_sph": 5, "int2e_ip2_sph": 6, "int3c2e_ip2_sph": 7}
 
 
 def ncomp(name):
-    return 3 if "ip1" in name else 1
+    return 3 if "_ip" in name else 1
 
 
 def shell_dim(bas, sh, cart=False):

@@ -18,14 +18,14 @@ RTOL = 1e-12
 # ~2a g; two FP64 builds of the oracle that differ only in FMA contraction
 # already break the element rule there, so the derivative families use the
 # rule plus the measured FP64 floor (test_library_cpu.deriv_tolerance).
-from test_library_cpu import deriv_tolerance
+from test_library_cpu import IP2_FLOOR, deriv_tolerance
 
 
-def assert_parity(got, ref, block_scale=None):
+def assert_parity(got, ref, block_scale=None, floor=3e-11):
     err = np.abs(got - ref)
     tol = ATOL + RTOL * np.abs(ref)
     if block_scale is not None:
-        tol = deriv_tolerance(ref, block_scale)
+        tol = deriv_tolerance(ref, block_scale, floor)
     bad = err > tol
     say(f"  max abs err {err.max():.3e}  max |ref| {np.abs(ref).max():.3e}  "
         f"max rel(>1e-6) {np.max(err[np.abs(ref) > 1e-6] / np.abs(ref[np.abs(ref) > 1e-6]), initial=0):.3e}")
@@ -54,10 +54,10 @@ def check_plan(O, name, tuples, mol, variant=None):
     t, o = p.tuples()
     assert np.array_equal(o, offs)
     scale = None
-    if "ip1" in name:
+    if "_ip" in name:
         scale = np.repeat([np.abs(ref[a:a + n]).max() if n else 0.0
                            for a, n in zip(offs, sizes)], sizes)
-    assert_parity(got, ref, scale)
+    assert_parity(got, ref, scale, IP2_FLOOR if "_ip2" in name else 3e-11)
     p.close()
     return got
 
@@ -115,6 +115,21 @@ def test_int3c2e_ip1_sph_cfg4_sampled(oracle):
     check_plan(oracle, "int3c2e_ip1_sph", tuples, mol)
 
 
+def test_int2e_ip2_sph_cfg1(oracle):
+    mol = M.config(1)
+    rng = np.random.default_rng(13)
+    tuples = [tuple(int(s) for s in rng.integers(0, mol.nbas, 4)) for _ in range(2000)]
+    check_plan(oracle, "int2e_ip2_sph", tuples, mol)
+
+
+def test_int3c2e_ip2_sph_cfg4_sampled(oracle):
+    mol = M.config(4)
+    rng = np.random.default_rng(14)
+    tuples = [(int(rng.integers(0, mol.aux_start)), int(rng.integers(0, mol.aux_start)),
+               int(rng.integers(mol.aux_start, mol.nbas))) for _ in range(1500)]
+    check_plan(oracle, "int3c2e_ip2_sph", tuples, mol)
+
+
 def test_int2c2e_sph_cfg4_aux_metric(oracle):
     # every (P|Q) of the cfg-4 auxiliary basis (s..g on 64 centres)
     mol = M.config(4)
@@ -144,13 +159,15 @@ def test_raw_signature_dims_and_general_contraction(oracle):
         rd, _ = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env, dims=dims)
         assert_parity(gd, rd)
     shls = [int(low[0]), int(low[1]), int(low[2])]
-    for name in ("int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph"):
+    for name in ("int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph",
+                 "int2e_ip2_sph", "int3c2e_ip2_sph"):
         sh = shls + [int(low[3])] if name.startswith("int2e") else shls
         sh = sh[:capi.arity(name)]
         got, not0 = capi.raw_call(name, sh, mol.atm, mol.bas, mol.env)
         ref, rnot0 = oracle.eval_tuple(name, sh, mol.atm, mol.bas, mol.env)
         assert not0 == rnot0
-        assert_parity(got, ref, np.abs(ref).max() if "ip1" in name else None)
+        assert_parity(got, ref, np.abs(ref).max() if "_ip" in name else None,
+                      IP2_FLOOR if "_ip2" in name else 3e-11)
     for _ in range(10):
         sh = [int(low[i]) for i in rng.integers(0, len(low), 2)]
         dims = [d + 2 for d in capi.natural_dims("int2c2e_sph", sh, mol.bas)]

@@ -52,6 +52,12 @@ def test_cache_size_query_matches_libcint_formula():
     # (p|s), p with 2 primitives x 2 contractions: g_size 2, leng 12, nf 3,
     # nc 6, lenj 18, leni 6, len0 3, pdata 10+4+1+6+9=30 -> 75 (> 6+48)
     assert capi.cache_size("int2c2e_sph", [1, 0], mol.atm, mol.bas, mol.env) == 75
+    # ip2 raises lk_ceil (ng KINC = 1, grad2.c:147). (ss|ps): nroots 2,
+    # kbase, dlk 3 -> g_size 6, leng 54, nf 3, len0 9, nc 6 -> 54, pdata 41
+    assert capi.cache_size("int2e_ip2_sph", [0, 0, 1, 0], mol.atm, mol.bas, mol.env) == 158
+    # int3c2e_ip2 (ss|p): ll_ceil 2, g_size 6, leng 54, len0 9, 54,
+    # pdata 5 + 1+1+4 + 4 + 2 + 9 + 16 = 42 -> 159
+    assert capi.cache_size("int3c2e_ip2_sph", [0, 0, 1], mol.atm, mol.bas, mol.env) == 159
 
 
 def _eval(api, out, dims, shls, mol):
@@ -117,12 +123,18 @@ def test_generated_molecules_have_the_quoted_sizes():
     assert m4.nao(range(m4.aux_start, m4.nbas)) == 4544
 
 
-def deriv_tolerance(ref, block_scale):
-    """Parity rule for nabla_i integrals: the reference's element rule
+def deriv_tolerance(ref, block_scale, floor=3e-11):
+    """Parity rule for nabla integrals: the reference's element rule
     (crates/cintx-oracle/src/compare.rs:256-259) plus the FP64 rounding floor
-    of the algorithm, 1e-13 of the tuple's largest element and 3e-11
-    absolute (see test_fp64_floor_of_derivative_integrals)."""
-    return 1e-12 + 1e-12 * np.abs(ref) + 1e-13 * block_scale + 3e-11
+    of the algorithm, 1e-13 of the tuple's largest element and `floor`
+    absolute: 3e-11 for nabla_i (test_fp64_floor_of_derivative_integrals),
+    IP2_FLOOR for nabla_k (test_fp64_floor_of_nabla_k_routes)."""
+    return 1e-12 + 1e-12 * np.abs(ref) + 1e-13 * block_scale + floor
+
+
+# the device evaluates (ij|nabla k l) as the nabla_i integral of the
+# bra-ket swapped quartet; the oracle follows libcint's nabla_k recurrence
+IP2_FLOOR = 1e-10
 
 
 def test_fp64_floor_of_derivative_integrals(oracle):
@@ -152,6 +164,31 @@ def test_fp64_floor_of_derivative_integrals(oracle):
     scale = np.repeat([np.abs(ref[a:a + n]).max() if n else 0.0 for a, n in zip(offs, sizes)],
                       sizes)
     assert np.all(err <= deriv_tolerance(ref, scale))
+
+
+def test_fp64_floor_of_nabla_k_routes(oracle):
+    """(ij|nabla k l) through libcint's nabla_k recurrence and through the
+    nabla_i recurrence of (k l|i j) are both correct FP64 evaluations; on
+    one-centre quartets of the cfg-1 molecule, whose exact value is 0 and
+    whose computed value is the rounding residue of 2a*g terms with a up to
+    5000, they differ by up to ~4e-11, beyond the nabla_i floor but well
+    inside IP2_FLOOR."""
+    mol = M.config(1)
+    rng = np.random.default_rng(13)
+    tuples = [tuple(int(s) for s in rng.integers(0, mol.nbas, 4)) for _ in range(2000)]
+    ref, offs, sizes = oracle.eval_list("int2e_ip2_sph", tuples, mol.atm, mol.bas, mol.env)
+    swapped = [(k, l, i, j) for (i, j, k, l) in tuples]
+    alt, offs2, sizes2 = oracle.eval_list("int2e_ip1_sph", swapped, mol.atm, mol.bas, mol.env)
+    got = np.zeros_like(ref)
+    for t, sh in enumerate(tuples):
+        d = oracle.tuple_dims("int2e_ip2_sph", mol.bas, sh)
+        b = alt[offs2[t]:offs2[t] + sizes2[t]].reshape([3, d[1], d[0], d[3], d[2]])
+        got[offs[t]:offs[t] + sizes[t]] = b.transpose(0, 3, 4, 1, 2).ravel()
+    err = np.abs(got - ref)
+    scale = np.repeat([np.abs(ref[a:a + n]).max() if n else 0.0 for a, n in zip(offs, sizes)],
+                      sizes)
+    assert (err > deriv_tolerance(ref, scale)).sum() > 0
+    assert np.all(err <= deriv_tolerance(ref, scale, IP2_FLOOR))
 
 
 def _cart(l):

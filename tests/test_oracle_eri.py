@@ -1,12 +1,14 @@
-"""Pins of the oracle's integrals (int2e_sph, int2e_ip1_sph, int3c2e_sph,
-int3c2e_ip1_sph, int2c2e_sph) against checks that do not reuse its algorithm:
+"""Pins of the oracle's integrals (int2e_sph, int2e_ip1_sph, int2e_ip2_sph,
+int3c2e_sph, int3c2e_ip1_sph, int3c2e_ip2_sph, int2c2e_sph) against checks
+that do not reuse its algorithm:
 
 * an independent McMurchie-Davidson implementation (tests/md_eri.py) with
   numerically fitted spherical harmonics;
 * the (ss|ss) closed form;
 * 8-fold permutational symmetry of the full cfg-1 tensor;
-* central finite differences in the nuclear coordinate (ip1 = -d/dA_i) and
-  translational invariance;
+* central finite differences in the nuclear coordinate (ip1 = -d/dA_i,
+  ip2 = -d/dC_k), translational invariance, and ip2 as the ip1 of the
+  bra-ket swapped quartet;
 * int3c2e as int2e with a zero-exponent s function on the third centre, and
   int2c2e as int2e with zero-exponent s functions in the j and l slots;
 * the (s|s) Coulomb-metric closed form;
@@ -133,6 +135,51 @@ def test_ip1_translational_invariance(oracle):
         inv = np.argsort(perm)
         tot = tot + b.transpose([0] + [1 + int(k) for k in inv])
     assert np.abs(tot).max() <= 1e-12 * max(1, np.abs(b).max())
+
+
+@pytest.mark.parametrize("ls", [(0, 0, 0, 0), (1, 0, 1, 0), (2, 1, 0, 1), (0, 2, 2, 1)])
+def test_ip2_finite_difference(oracle, ls):
+    # int2e_ip2 = (ij|nabla_r k l) = -d/dC_k with k alone on atom 2
+    mol = four_centre_molecule(ls, seed=6)
+    shls = [0, 1, 2, 3]
+    ip2, _ = block(oracle, "int2e_ip2_sph", shls, mol.atm, mol.bas, mol.env)
+    h = 1e-4
+    for ax in range(3):
+        fp, _ = block(oracle, "int2e_sph", shls, mol.atm, mol.bas, moved(mol, 2, ax, h))
+        fm, _ = block(oracle, "int2e_sph", shls, mol.atm, mol.bas, moved(mol, 2, ax, -h))
+        fd = -(fp[0] - fm[0]) / (2 * h)
+        assert np.abs(ip2[ax] - fd).max() <= 1e-7 * max(1.0, np.abs(fd).max())
+
+
+def test_ip2_is_ip1_of_the_swapped_quartet(oracle):
+    # (ij|nabla k l) = (nabla k l|ij): the nabla_k recurrence against the
+    # nabla_i one through bra-ket symmetry, general contractions included
+    gc = M.general_contraction_h2()
+    rng = np.random.default_rng(8)
+    for _ in range(10):
+        i, j, k, l = (int(x) for x in rng.integers(0, gc.nbas, 4))
+        ip2, _ = block(oracle, "int2e_ip2_sph", [i, j, k, l], gc.atm, gc.bas, gc.env)
+        ip1, _ = block(oracle, "int2e_ip1_sph", [k, l, i, j], gc.atm, gc.bas, gc.env)
+        ref = ip1.transpose(0, 3, 4, 1, 2)
+        assert_close(ip2, ref, atol=1e-11 * max(1.0, np.abs(ref).max()), rtol=1e-10)
+
+
+@pytest.mark.parametrize("ls", [(0, 0, 1), (1, 2, 3), (2, 0, 4)])
+def test_int3c2e_ip2_finite_difference(oracle, ls):
+    mol = four_centre_molecule(ls + (0,), seed=5)
+    shls = [0, 1, 2]
+    ip2, _ = block(oracle, "int3c2e_ip2_sph", shls, mol.atm, mol.bas, mol.env)
+    h = 1e-4
+    for ax in range(3):
+        fp, _ = block(oracle, "int3c2e_sph", shls, mol.atm, mol.bas, moved(mol, 2, ax, h))
+        fm, _ = block(oracle, "int3c2e_sph", shls, mol.atm, mol.bas, moved(mol, 2, ax, -h))
+        fd = -(fp[0] - fm[0]) / (2 * h)
+        assert np.abs(ip2[ax] - fd).max() <= 1e-7 * max(1.0, np.abs(fd).max())
+    # translational invariance: ip1(i) + ip1(j) + ip2(k) = 0
+    ipi, _ = block(oracle, "int3c2e_ip1_sph", [0, 1, 2], mol.atm, mol.bas, mol.env)
+    ipj, _ = block(oracle, "int3c2e_ip1_sph", [1, 0, 2], mol.atm, mol.bas, mol.env)
+    tot = ipi + ipj.transpose(0, 2, 1, 3) + ip2
+    assert np.abs(tot).max() <= 1e-11 * max(1.0, np.abs(ip2).max())
 
 
 def test_int3c2e_equals_int2e_with_unit_function(oracle):
