@@ -26,7 +26,9 @@ COMMON_FLAGS = [
 
 
 def class_sets():
-    """(family_deriv, precision, (li, lj, lk, ll)) instantiated as templates."""
+    """(family_deriv, form, (li, lj, lk, ll)) instantiated as templates; form
+    0 = FP64, 1 = FP32 recurrences, 2 = FP64 with Cartesian output (the
+    int2e_cart / int3c2e_cart families)."""
     out = []
     pairs = [(a, b) for a in range(3) for b in range(3) if a >= b]
     # int2e_sph: canonical li>=lj, lk>=ll, (li,lj)>=(lk,ll), l<=2; FP64 and FP32
@@ -57,6 +59,7 @@ def class_sets():
         if c not in seen:
             seen.add(c)
             uniq.append(c)
+    uniq += [(0, 2, ls) for deriv, prec, ls in uniq if deriv == 0 and prec == 0]
     return uniq
 
 
@@ -96,6 +99,8 @@ def variants(deriv, prec, ls):
     """Launch-shape candidates (lanes per tuple T, primitive tuples per step P)
     of a class kernel; the first is the untuned default.  Mirrors the sizes of
     ClassShape in csrc/eri_device.h."""
+    if prec == 2:  # Cartesian output: the FP64 shapes with PP bit 11
+        return [(tt, pp | 2048) for tt, pp in variants(deriv, 0, ls)]
     li, lj, lk, ll = ls
     nc = [(l + 1) * (l + 2) // 2 for l in ls]
     nr = (li + deriv + lj + lk + ll) // 2 + 1
@@ -172,7 +177,7 @@ def generate(nparts=24, nchunkparts=12):
         lines = ['#include "eri_device.h"', "namespace cx {",
                  f"void register_part_{p}(Registry& reg) {{"]
         for deriv, prec, (a, b, c, d), v, tt, pp in cls:
-            real = "double" if prec == 0 else "float"
+            real = "float" if prec == 1 else "double"
             lines.append(f"  reg.k[{deriv}][{prec}][{a}][{b}][{c}][{d}][{v}] = "
                          f"class_info<{a}, {b}, {c}, {d}, {deriv}, {real}, {tt}, {pp}>();")
         lines += ["}", "}  // namespace cx", ""]

@@ -28,11 +28,13 @@ INT3C2E_IP1_SPH = 19
 INT2E_IP1_SPH = 36
 INT3C2E_SPH = 40
 INT2C2E_SPH = 13
+INT2E_CART = 9
 
 FAMILY = {"int2e_sph": 0, "int2e_ip1_sph": 1, "int3c2e_sph": 2, "int3c2e_ip1_sph": 3,
-          "int2c2e_sph": 4, "int2e_ip2_sph": 5, "int3c2e_ip2_sph": 6}
+          "int2c2e_sph": 4, "int2e_ip2_sph": 5, "int3c2e_ip2_sph": 6, "int2e_cart": 7,
+          "int3c2e_cart": 8}
 API_OF_FAMILY = {0: INT2E_SPH, 1: INT2E_IP1_SPH, 2: INT3C2E_SPH, 3: INT3C2E_IP1_SPH,
-                 4: INT2C2E_SPH}
+                 4: INT2C2E_SPH, 7: INT2E_CART}
 PREC_FP64 = 0
 # launch-shape variants compiled per class (kVariants in csrc/launch.h)
 NVARIANTS = 6
@@ -46,7 +48,7 @@ STATUS = {
 
 EXPORTS = [
     "int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph",
-    "int2e_ip2_sph", "int3c2e_ip2_sph",
+    "int2e_ip2_sph", "int3c2e_ip2_sph", "int2e_cart", "int3c2e_cart",
     "int2e_optimizer", "int2e_ip1_optimizer", "int3c2e_optimizer", "int3c2e_ip1_optimizer",
     "int2c2e_optimizer", "int2e_ip2_optimizer", "int3c2e_ip2_optimizer",
     "CINTdel_optimizer",
@@ -93,7 +95,7 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         raw = [DP, IP, IP, IP, ctypes.c_int, IP, ctypes.c_int, DP, ctypes.c_void_p, DP]
         for n in ("int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph",
-                  "int2e_ip2_sph", "int3c2e_ip2_sph"):
+                  "int2e_ip2_sph", "int3c2e_ip2_sph", "int2e_cart", "int3c2e_cart"):
             getattr(L, n).argtypes = raw
             getattr(L, n).restype = ctypes.c_int
         sz = ctypes.c_size_t
@@ -164,8 +166,13 @@ def arity(family):
     return 2 if family.startswith("int2c2e") else 3 if family.startswith("int3c2e") else 4
 
 
+def shell_functions(family, l):
+    return (l + 1) * (l + 2) // 2 if family.endswith("_cart") else 2 * l + 1
+
+
 def natural_dims(family, shls, bas):
-    return [(2 * int(bas[s][1]) + 1) * int(bas[s][3]) for s in shls[:arity(family)]]
+    return [shell_functions(family, int(bas[s][1])) * int(bas[s][3])
+            for s in shls[:arity(family)]]
 
 
 def ncomp(family):
@@ -210,6 +217,10 @@ def int3c2e_ip1_sph(shls, atm, bas, env, dims=None):
 
 def int2c2e_sph(shls, atm, bas, env, dims=None):
     return raw_call("int2c2e_sph", shls, atm, bas, env, dims)
+
+
+def int2e_cart(shls, atm, bas, env, dims=None):
+    return raw_call("int2e_cart", shls, atm, bas, env, dims)
 
 
 def eval_raw(api, out, dims, shls, atm, bas, env, cache=None):
@@ -361,7 +372,7 @@ class Plan:
     def block_size(self, shls):
         n = 3 if "_ip" in self.family else 1
         for s in shls[:arity(self.family)]:
-            n *= (2 * int(self._bas[s][1]) + 1) * int(self._bas[s][3])
+            n *= shell_functions(self.family, int(self._bas[s][1])) * int(self._bas[s][3])
         return n
 
     def work(self):

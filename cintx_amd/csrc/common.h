@@ -65,6 +65,7 @@ struct EriArgs {
   double expcutoff;
   int lij_ceil, lkl_ceil;  // li_ceil+lj_ceil, lk_ceil+ll_ceil for the screen
   int three_center;        // ket pairs are pseudo pairs (no kl screen)
+  int cart;                // generic kernel: Cartesian output, no cart -> sph
   // output strides for the four canonical axes and the component
   int64_t st[4];
   int64_t st_comp;

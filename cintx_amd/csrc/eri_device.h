@@ -348,6 +348,9 @@ struct ClassShape {
   // PP bits 9-10: minimum waves per SIMD the compiler must fit (register
   // budget 512 / MINW per lane), 1 by default
   static constexpr int MINW = ((PP >> 9) & 3) + 1;
+  // PP bit 11: Cartesian output (the _cart families), no cart -> sph
+  static constexpr bool CART = (PP >> 11) & 1;
+  static constexpr int nout(int l) { return CART ? ncart(l) : nsph(l); }
   static constexpr int RPC = (NROW + NCH - 1) / NCH;  // rows per chunk
   static constexpr int G = T >= NROW ? T / NROW : 1;               // groups per row
   static constexpr int RPL = T >= NROW ? 1 : (NROW + T - 1) / T;   // rows per lane
@@ -599,15 +602,19 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
   // for nabla_i rows) per row, then the row axes as a weighted sum over the
   // chunk's rows, added to what earlier chunks wrote
   if (active) {
-    constexpr int NSI = nsph(LI), NSJ = nsph(LJ), NSK = nsph(LK), NSL = nsph(LL);
-    constexpr int NSB = DERIV ? NSI : NSI * NSJ;  // spherical size of a row
+    constexpr bool CART = S::CART;
+    constexpr int NSI = S::nout(LI), NSJ = S::nout(LJ), NSK = S::nout(LK), NSL = S::nout(LL);
+    constexpr int NSB = DERIV ? NSI : NSI * NSJ;  // output size of a row
 #pragma unroll
     for (int comp = 0; comp < NCOMP; ++comp) {
       double d[RPC * NSB];
 #pragma unroll
       for (int rl = 0; rl < RPC; ++rl) {
         const double* src = acc + comp * CE + rl * NB;
-        if constexpr (DERIV) {
+        if constexpr (CART) {
+#pragma unroll
+          for (int e = 0; e < NSB; ++e) d[rl * NSB + e] = src[e];
+        } else if constexpr (DERIV) {
           c2s_reg<LI, 1, 1>(src, d + rl * NSB, a.c2s);
         } else {
           double t1[NSI * NFJ];
@@ -638,8 +645,8 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
                   fk = row % NFK;
                   fl = row / NFK;
                 }
-                return (LK >= 2 || fk == mk) && (LL >= 2 || fl == ml) &&
-                       (!DERIV || LJ >= 2 || fj == mj);
+                return ((!CART && LK >= 2) || fk == mk) && ((!CART && LL >= 2) || fl == ml) &&
+                       (!DERIV || (!CART && LJ >= 2) || fj == mj);
               };
               bool mine = false, earlier = false;
 #pragma unroll
@@ -663,8 +670,13 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
                   fk = row % NFK;
                   fl = row / NFK;
                 }
-                double cf = c2s_coef(a.c2s, LK, mk, fk) * c2s_coef(a.c2s, LL, ml, fl);
-                if constexpr (DERIV) cf *= c2s_coef(a.c2s, LJ, mj, fj);
+                double cf;
+                if constexpr (CART) {
+                  cf = (fk == mk && fl == ml && (!DERIV || fj == mj)) ? 1.0 : 0.0;
+                } else {
+                  cf = c2s_coef(a.c2s, LK, mk, fk) * c2s_coef(a.c2s, LL, ml, fl);
+                  if constexpr (DERIV) cf *= c2s_coef(a.c2s, LJ, mj, fj);
+                }
                 v += cf * d[rl * NSB + mb];
               }
               const int mi = DERIV ? mb : mb % NSI;
@@ -923,8 +935,8 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
     }
   }
 
-  // Cartesian -> spherical, then the strided write.
-  constexpr int NSI = nsph(LI), NSJ = nsph(LJ), NSK = nsph(LK), NSL = nsph(LL);
+  // Cartesian -> spherical (unless Cartesian output), then the strided write.
+  constexpr int NSI = S::nout(LI), NSJ = S::nout(LJ), NSK = S::nout(LK), NSL = S::nout(LL);
   constexpr int NS = NSI * NSJ * NSK * NSL;
   if constexpr (REG) {
     // written chunk by chunk above
@@ -959,7 +971,7 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
     double* dst = b1;
 #pragma unroll
     for (int ax = 0; ax < 4; ++ax) {
-      if (ls[ax] < 2) continue;  // s and p: identity
+      if (ls[ax] < 2 || S::CART) continue;  // s and p: identity
       c2s_pass(dst, src, dims, ax, ls[ax], a.c2s, t, T);
       dims[ax] = nsph(ls[ax]);
       double* tmp = src;

@@ -266,7 +266,8 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
       atomicAdd(a.counters, cnt[0]);
       atomicAdd(a.counters + 1, 1ull);
     }
-    const int ns0 = nsph(S.l[0]), ns1 = nsph(S.l[1]), ns2 = nsph(S.l[2]), ns3 = nsph(S.l[3]);
+    const int ns0 = a.cart ? nfi : nsph(S.l[0]), ns1 = a.cart ? nfj : nsph(S.l[1]);
+    const int ns2 = a.cart ? nfk : nsph(S.l[2]), ns3 = a.cart ? nfl : nsph(S.l[3]);
     const int NS = ns0 * ns1 * ns2 * ns3;
     for (int comp = 0; comp < S.ncomp; ++comp) {
       if constexpr (AREG) {
@@ -281,7 +282,7 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
       double* src = b0;
       double* dst = b1;
       for (int ax = 0; ax < 4; ++ax) {
-        if (S.l[ax] < 2) continue;
+        if (S.l[ax] < 2 || a.cart) continue;
         c2s_pass(dst, src, dims, ax, S.l[ax], a.c2s, t, T);
         dims[ax] = nsph(S.l[ax]);
         double* tmp = src;

@@ -1,5 +1,5 @@
 // Class-kernel registry: up to kVariants launchers per (derivative,
-// precision, LI, LJ, LK, LL), instantiated in the generated translation units;
+// form, LI, LJ, LK, LL), instantiated in the generated translation units;
 // variant 0 is the default, the others are autotuning candidates.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -19,8 +19,11 @@ struct KernelInfo {
   int prims_per_step;    // P (one lane per tuple: row chunks)
 };
 
+// forms: 0 FP64, 1 FP32 recurrences, 2 FP64 with Cartesian output
+constexpr int kForms = 3;
+
 struct Registry {
-  KernelInfo k[2][2][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1][kVariants];
+  KernelInfo k[2][kForms][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1][kVariants];
 };
 
 // Filled by the generated instantiation units.

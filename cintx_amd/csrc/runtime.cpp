@@ -108,8 +108,9 @@ struct FamilyInfo {
   const char* name;
   bool two;     // the canonical bra is one Gaussian
   int dcenter;  // argument shell under nabla
+  bool cart = false;  // Cartesian output (no cart -> sph)
 };
-static constexpr int kNumFamilies = 7;
+static constexpr int kNumFamilies = 9;
 static const FamilyInfo kFam[kNumFamilies] = {
     {false, 0, 1, 4, "int2e_sph", false, -1},
     {false, 1, 3, 4, "int2e_ip1_sph", false, 0},
@@ -119,7 +120,12 @@ static const FamilyInfo kFam[kNumFamilies] = {
     // (ij|nabla k l) = (nabla k l|ij): the ip1 kernels with (k l) as the bra
     {false, 1, 3, 4, "int2e_ip2_sph", false, 2},
     // (ij|nabla k): bra = k alone, ket = the (i j) pair
-    {false, 1, 3, 3, "int3c2e_ip2_sph", true, 2}};
+    {false, 1, 3, 3, "int3c2e_ip2_sph", true, 2},
+    {false, 0, 1, 4, "int2e_cart", false, -1, true},
+    {true, 0, 1, 3, "int3c2e_cart", false, -1, true}};
+
+// functions per shell in a family's output
+static int nfun(const FamilyInfo& f, int l) { return f.cart ? ncart(l) : nsph(l); }
 
 static constexpr double kExpCutoff = 60.0;     // cint_config.h.in EXPCUTOFF
 static constexpr double kMinExpCutoff = 40.0;  // cint_config.h.in MIN_EXPCUTOFF
@@ -195,8 +201,8 @@ struct DevCtx {
   double* d_large = nullptr;
   double* d_c2s = nullptr;
   Registry* reg = nullptr;
-  // autotuned class-kernel variant per (deriv, prec, class); -1: untuned
-  signed char tuned[2][2][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1];
+  // autotuned class-kernel variant per (deriv, form, class); -1: untuned
+  signed char tuned[2][kForms][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1];
 };
 
 static std::mutex g_mu;  // guards context creation and the per-call arena
@@ -505,7 +511,7 @@ static void canon_perm(int fam, const int* l, int p[4]) {
 static int64_t nat_size(int fam, const HostBasis& B, const int* shl) {
   const FamilyInfo& f = kFam[fam];
   int64_t n = f.ncomp;
-  for (int a = 0; a < f.arity; ++a) n *= (int64_t)nsph(B.l[shl[a]]) * B.nctr[shl[a]];
+  for (int a = 0; a < f.arity; ++a) n *= (int64_t)nfun(f, B.l[shl[a]]) * B.nctr[shl[a]];
   return n;
 }
 
@@ -519,6 +525,7 @@ static double fam_expcutoff(int fam, const double* env) {
 
 static void plan_add_tuples(cintx_plan* P, const int* tuples, int64_t nt, int prec) {
   const FamilyInfo& f = kFam[P->family];
+  if (prec != 0 && f.cart) fail(CINTX_STATUS_INVALID_INPUT, "Cartesian families run in FP64 only");
   const HostBasis& B = P->basis;
   std::unordered_map<uint64_t, int> bin_of;
   for (int64_t t = 0; t < nt; ++t) {
@@ -554,7 +561,7 @@ static void plan_add_tuples(cintx_plan* P, const int* tuples, int64_t nt, int pr
       Bin b;
       for (int c = 0; c < 4; ++c) b.l[c] = cl[c];
       b.prec = prec;
-      b.block = (int64_t)f.ncomp * nsph(cl[0]) * nsph(cl[1]) * nsph(cl[2]) * nsph(cl[3]);
+      b.block = (int64_t)f.ncomp * nfun(f, cl[0]) * nfun(f, cl[1]) * nfun(f, cl[2]) * nfun(f, cl[3]);
       bi = (int)P->bins.size();
       P->bins.push_back(b);
       bin_of.emplace(key, bi);
@@ -709,9 +716,14 @@ static void plan_upload(cintx_plan* P) {
   HIPCHK(hipEventCreate(&P->ev1));
 }
 
+// registry form of a bin: 0 FP64, 1 FP32, 2 FP64 with Cartesian output
+static int bin_form(const cintx_plan* P, const Bin& b) {
+  return kFam[P->family].cart ? 2 : b.prec;
+}
+
 static const KernelInfo* bin_kernels(const cintx_plan* P, const Bin& b) {
   const FamilyInfo& f = kFam[P->family];
-  return ctx().reg->k[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
+  return ctx().reg->k[f.deriv][bin_form(P, b)][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
 }
 
 // Variant v of a bin: v < kVariants picks the bin's own precision; an FP32
@@ -730,7 +742,7 @@ static const KernelInfo& variant_info(const cintx_plan* P, const Bin& b, int v) 
 
 static signed char& tuned_slot(const cintx_plan* P, const Bin& b) {
   const FamilyInfo& f = kFam[P->family];
-  return ctx().tuned[f.deriv][b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
+  return ctx().tuned[f.deriv][bin_form(P, b)][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
 }
 
 static void plan_drop_graph(cintx_plan* P) {
@@ -777,7 +789,8 @@ static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count) {
   a.lij_ceil = b.l[0] + b.l[1] + f.deriv;
   a.lkl_ceil = b.l[2] + b.l[3];
   a.three_center = f.three ? 1 : 0;
-  int ns0 = nsph(b.l[0]), ns1 = nsph(b.l[1]), ns2 = nsph(b.l[2]);
+  a.cart = f.cart ? 1 : 0;
+  int ns0 = nfun(f, b.l[0]), ns1 = nfun(f, b.l[1]), ns2 = nfun(f, b.l[2]);
   a.st[0] = 1;
   a.st[1] = ns0;
   a.st[2] = (int64_t)ns0 * ns1;
@@ -849,7 +862,7 @@ static void plan_to_natural(const cintx_plan* P, const double* canon, double* na
   const HostBasis& B = P->basis;
   for (const TupleRec& tr : P->tuples) {
     int n[4] = {1, 1, 1, 1};
-    for (int a = 0; a < f.arity; ++a) n[a] = nsph(B.l[tr.shl[a]]) * B.nctr[tr.shl[a]];
+    for (int a = 0; a < f.arity; ++a) n[a] = nfun(f, B.l[tr.shl[a]]) * B.nctr[tr.shl[a]];
     int64_t so[4] = {1, n[0], (int64_t)n[0] * n[1], (int64_t)n[0] * n[1] * n[2]};
     int64_t nout = (int64_t)n[0] * n[1] * n[2] * n[3];
     double* dst = nat + tr.nat_off;
@@ -860,7 +873,7 @@ static void plan_to_natural(const cintx_plan* P, const double* canon, double* na
       int ns[4], base[4];
       int64_t st[4];
       for (int c = 0; c < 4; ++c) {
-        ns[c] = nsph(b.l[c]);
+        ns[c] = nfun(f, b.l[c]);
         int a = tr.perm[c];
         base[c] = vq.cc[c] * ns[c];
         st[c] = a >= f.arity ? 0 : so[a];
@@ -979,7 +992,7 @@ static int eval_one(int fam, double* out, const int* dims, const int* shls, cons
   for (size_t i = 0; i < P->bins.size(); ++i) surv += cnt[2 * i];
   const TupleRec& tr = P->tuples[0];
   int n[4] = {1, 1, 1, 1};
-  for (int a = 0; a < f.arity; ++a) n[a] = nsph(P->basis.l[tr.shl[a]]) * P->basis.nctr[tr.shl[a]];
+  for (int a = 0; a < f.arity; ++a) n[a] = nfun(f, P->basis.l[tr.shl[a]]) * P->basis.nctr[tr.shl[a]];
   int d[4] = {n[0], n[1], n[2], n[3]};
   if (dims)
     for (int a = 0; a < f.arity; ++a) d[a] = dims[a];
@@ -1010,6 +1023,8 @@ static int family_of_api(int api) {
       return 3;
     case CINTX_API_INT2C2E_SPH:
       return 4;
+    case CINTX_API_INT2E_CART:
+      return 7;
     default:
       return -1;
   }
@@ -1084,6 +1099,18 @@ int int2c2e_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas,
   (void)cache;
   return raw_entry(4, out, dims, shls, atm, natm, bas, nbas, env);
 }
+int int2e_cart(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
+               double* env, CINTOpt* opt, double* cache) {
+  (void)opt;
+  (void)cache;
+  return raw_entry(7, out, dims, shls, atm, natm, bas, nbas, env);
+}
+int int3c2e_cart(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
+                 double* env, CINTOpt* opt, double* cache) {
+  (void)opt;
+  (void)cache;
+  return raw_entry(8, out, dims, shls, atm, natm, bas, nbas, env);
+}
 int int2e_ip2_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
                   double* env, CINTOpt* opt, double* cache) {
   (void)opt;
@@ -1135,7 +1162,7 @@ static void shim_common(int fam, const int* dims, size_t dims_len, const int* sh
   int n[4] = {1, 1, 1, 1};
   for (int a = 0; a < f.arity; ++a) {
     if (shls[a] < 0 || shls[a] >= nbas) fail(CINTX_STATUS_INVALID_INPUT, "shell index out of range");
-    n[a] = nsph(B.l[shls[a]]) * B.nctr[shls[a]];
+    n[a] = nfun(f, B.l[shls[a]]) * B.nctr[shls[a]];
   }
   if (dims) {
     // crates/cintx-compat/src/layout.rs:25-64: dims must equal the natural extents
@@ -1357,6 +1384,7 @@ static void class_work(int fam, const int* l, double* per_prim, double* per_tupl
     double other = 1;
     for (int b = 0; b < 4; ++b)
       if (b != a) other *= dim[b];
+    if (f.cart) continue;  // Cartesian output: no transform
     c2s += 2.0 * other * nsph(l[a]) * ncart(l[a]);
     dim[a] = nsph(l[a]);
   }
@@ -1398,7 +1426,7 @@ int cintx_plan_copy_tuples(cintx_plan* plan, const int64_t* which, int64_t n, do
       HIPCHK(hipStreamSynchronize(C.stream));
       int nn[4] = {1, 1, 1, 1};
       for (int a = 0; a < f.arity; ++a)
-        nn[a] = nsph(plan->basis.l[tr.shl[a]]) * plan->basis.nctr[tr.shl[a]];
+        nn[a] = nfun(f, plan->basis.l[tr.shl[a]]) * plan->basis.nctr[tr.shl[a]];
       int64_t so[4] = {1, nn[0], (int64_t)nn[0] * nn[1], (int64_t)nn[0] * nn[1] * nn[2]};
       int64_t nout = (int64_t)nn[0] * nn[1] * nn[2] * nn[3];
       double* dst = host_out + pos;
@@ -1409,7 +1437,7 @@ int cintx_plan_copy_tuples(cintx_plan* plan, const int64_t* which, int64_t n, do
         int ns[4], base[4];
         int64_t st[4];
         for (int c = 0; c < 4; ++c) {
-          ns[c] = nsph(b.l[c]);
+          ns[c] = nfun(f, b.l[c]);
           int a = tr.perm[c];
           base[c] = vq.cc[c] * ns[c];
           st[c] = a >= f.arity ? 0 : so[a];

@@ -53,6 +53,13 @@ int int3c2e_ip1_sph(double *out, int *dims, int *shls, int *atm, int natm, int *
  * shls holds two shells, out is [dk][di]) */
 int int2c2e_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
                 double *env, CINTOpt *opt, double *cache);
+/* libcint-master/src/cint2e.c:1202-1210, cint3c2e.c:709-717: Cartesian
+ * output (ncart per shell, libcint's xyz order), no cart -> sph; they share
+ * int2e_optimizer / int3c2e_optimizer */
+int int2e_cart(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+               double *env, CINTOpt *opt, double *cache);
+int int3c2e_cart(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                 double *env, CINTOpt *opt, double *cache);
 /* libcint-master/src/autocode/grad2.c:163-170 ((ij|nabla k l), out is
  * [3][dl][dk][dj][di]) */
 int int2e_ip2_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
@@ -87,6 +94,7 @@ void CINTdel_optimizer(CINTOpt **opt);
 #define CINTX_API_INT2E_IP1_SPH 36
 #define CINTX_API_INT3C2E_SPH 40
 #define CINTX_API_INT2C2E_SPH 13
+#define CINTX_API_INT2E_CART 9
 
 /* status codes: crates/cintx-capi/src/errors.rs:9-21 */
 #define CINTX_STATUS_SUCCESS 0

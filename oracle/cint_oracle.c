@@ -922,6 +922,15 @@ int orc_int3c2e_ip1_sph(double *out, int *dims, int *shls, int *atm, int natm, i
     Family f = {1, 1, 1};
     return tuple_driver(&f, out, dims, shls, atm, bas, env);
 }
+/* cint3c2e.c:709-717 */
+int orc_int3c2e_cart(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                     double *env, void *opt, double *cache)
+{
+    (void)natm; (void)nbas; (void)opt; (void)cache;
+    if (!out) return 0;
+    Family f = {1, 0, 0};
+    return tuple_driver(&f, out, dims, shls, atm, bas, env);
+}
 /* autocode/grad2.c:99-170: (ij|nabla k l) */
 int orc_int2e_ip2_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas,
                       int nbas, double *env, void *opt, double *cache)
@@ -955,9 +964,10 @@ typedef int (*orc_fn)(double *, int *, int *, int *, int, int *, int, double *, 
 long orc_eval_list(int api, int nq, const int *shls_list, const long *offsets, double *out,
                    int *atm, int natm, int *bas, int nbas, double *env, int nthreads)
 {
-    orc_fn fns[8] = {orc_int2e_sph,  orc_int2e_ip1_sph, orc_int3c2e_sph,   orc_int3c2e_ip1_sph,
-                     orc_int2e_cart, orc_int2c2e_sph,   orc_int2e_ip2_sph, orc_int3c2e_ip2_sph};
-    if (api < 0 || api > 7) return -1;
+    orc_fn fns[9] = {orc_int2e_sph,     orc_int2e_ip1_sph, orc_int3c2e_sph,
+                     orc_int3c2e_ip1_sph, orc_int2e_cart,  orc_int2c2e_sph,
+                     orc_int2e_ip2_sph, orc_int3c2e_ip2_sph, orc_int3c2e_cart};
+    if (api < 0 || api > 8) return -1;
     orc_fn fn = fns[api];
     ensure_c2s();
     long nnz = 0;
@@ -982,8 +992,8 @@ void orc_work_model(int api, int *shls, int *atm, int natm, int *bas, int nbas, 
                     double *counts)
 {
     (void)natm; (void)nbas;
-    Family fam = {api == 2 || api == 3 || api == 5 || api == 7, api == 1 || api == 3, api != 4,
-                  api == 5, api == 6 || api == 7};
+    Family fam = {api == 2 || api == 3 || api == 5 || api == 7 || api == 8, api == 1 || api == 3,
+                  api != 4 && api != 8, api == 5, api == 6 || api == 7};
     Shl s[4];
     int nsh = fam.three_center ? 3 : 4;
     static const double zero3[3] = {0, 0, 0};

@@ -42,7 +42,7 @@ def lib():
         sig = [dp, ip, ip, ip, ctypes.c_int, ip, ctypes.c_int, dp, ctypes.c_void_p, dp]
         for name in ("orc_int2e_sph", "orc_int2e_cart", "orc_int2e_ip1_sph",
                      "orc_int3c2e_sph", "orc_int3c2e_ip1_sph", "orc_int2c2e_sph",
-                     "orc_int2e_ip2_sph", "orc_int3c2e_ip2_sph"):
+                     "orc_int2e_ip2_sph", "orc_int3c2e_ip2_sph", "orc_int3c2e_cart"):
             getattr(L, name).argtypes = sig
             getattr(L, name).restype = ctypes.c_int
         L.orc_eval_list.argtypes = [ctypes.c_int, ctypes.c_int, ip,
@@ -94,9 +94,11 @@ _FN = {
     "int2c2e_sph": "orc_int2c2e_sph",
     "int2e_ip2_sph": "orc_int2e_ip2_sph",
     "int3c2e_ip2_sph": "orc_int3c2e_ip2_sph",
+    "int3c2e_cart": "orc_int3c2e_cart",
 }
 _API = {"int2e_sph": 0, "int2e_ip1_sph": 1, "int3c2e_sph": 2, "int3c2e_ip1_sph": 3,
-        "int2e_cart": 4, "int2c2e_sph": 5, "int2e_ip2_sph": 6, "int3c2e_ip2_sph": 7}
+        "int2e_cart": 4, "int2c2e_sph": 5, "int2e_ip2_sph": 6, "int3c2e_ip2_sph": 7,
+        "int3c2e_cart": 8}
 
 
 def ncomp(name):

@@ -130,6 +130,34 @@ def test_int3c2e_ip2_sph_cfg4_sampled(oracle):
     check_plan(oracle, "int3c2e_ip2_sph", tuples, mol)
 
 
+def test_cart_families(oracle):
+    mol = M.config(1)
+    n = mol.nbas
+    full = [(i, j, k, l) for l in range(n) for k in range(n) for j in range(n) for i in range(n)]
+    check_plan(oracle, "int2e_cart", full, mol)
+    aux = M.config(4)
+    rng = np.random.default_rng(17)
+    t3 = [(int(rng.integers(0, aux.aux_start)), int(rng.integers(0, aux.aux_start)),
+           int(rng.integers(aux.aux_start, aux.nbas))) for _ in range(3000)]
+    check_plan(oracle, "int3c2e_cart", t3, aux)
+    gc = M.general_contraction_h2()
+    low = [s for s in range(gc.nbas) if gc.bas[s][1] <= 2]
+    for _ in range(10):
+        sh = [int(low[i]) for i in rng.integers(0, len(low), 4)]
+        dims = [d + 1 for d in capi.natural_dims("int2e_cart", sh, gc.bas)]
+        got, not0 = capi.raw_call("int2e_cart", sh, gc.atm, gc.bas, gc.env, dims)
+        ref, rnot0 = oracle.eval_tuple("int2e_cart", sh, gc.atm, gc.bas, gc.env, dims=dims)
+        assert not0 == rnot0
+        assert_parity(got, ref)
+        got, not0 = capi.raw_call("int3c2e_cart", sh[:3], gc.atm, gc.bas, gc.env)
+        ref, rnot0 = oracle.eval_tuple("int3c2e_cart", sh[:3], gc.atm, gc.bas, gc.env)
+        assert not0 == rnot0
+        assert_parity(got, ref)
+    with pytest.raises(capi.CintxError):
+        capi.Plan("int2e_cart", mol.atm, mol.bas, mol.env, full[:10], precision=capi.PREC_MIXED,
+                  mixed_threshold=1.0)
+
+
 def test_int2c2e_sph_cfg4_aux_metric(oracle):
     # every (P|Q) of the cfg-4 auxiliary basis (s..g on 64 centres)
     mol = M.config(4)
@@ -184,6 +212,13 @@ def test_cintrs_eval_on_device(oracle):
     out = np.zeros(req)
     s = capi.eval_raw(capi.INT2E_SPH, out, None, shls, mol.atm, mol.bas, mol.env)
     ref, not0 = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env)
+    assert s["not0"] == not0 and s["bytes_written"] == req * 8
+    assert_parity(out, ref)
+    # api 9 (int2e_cart): Cartesian extents
+    req = int(np.prod(capi.natural_dims("int2e_cart", shls, mol.bas)))
+    out = np.zeros(req)
+    s = capi.eval_raw(capi.INT2E_CART, out, None, shls, mol.atm, mol.bas, mol.env)
+    ref, not0 = oracle.eval_tuple("int2e_cart", shls, mol.atm, mol.bas, mol.env)
     assert s["not0"] == not0 and s["bytes_written"] == req * 8
     assert_parity(out, ref)
 
@@ -295,6 +330,8 @@ def test_every_launch_variant(oracle, variant):
            int(rng.integers(aux.aux_start, aux.nbas))) for _ in range(1500)]
     check_plan(oracle, "int3c2e_sph", t3, aux, variant)
     check_plan(oracle, "int3c2e_ip1_sph", t3[:700], aux, variant)
+    check_plan(oracle, "int2e_cart", full[::7], mol, variant)
+    check_plan(oracle, "int3c2e_cart", t3[:700], aux, variant)
     # FP32 instantiations: every tuple of a mixed plan with a huge threshold
     p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, full, precision=capi.PREC_MIXED,
                   mixed_threshold=1e300)

@@ -216,7 +216,8 @@ def test_launch_variants_cover_every_element_and_fit_lds():
         vs = variants(deriv, prec, ls)
         assert 1 <= len(vs) <= capi.NVARIANTS
         for t, p in vs:
-            irow = t > 1 and not deriv and p >= 256  # rows of the i components alone
+            irow = t > 1 and not deriv and (p >> 8) & 1  # rows of the i components alone
+            assert bool((p >> 11) & 1) == (prec == 2)  # Cartesian-output form
             p &= 255
             nb = nc(li) if (deriv or irow) else nc(li) * nc(lj)
             nrow = nf // nb
@@ -226,7 +227,7 @@ def test_launch_variants_cover_every_element_and_fit_lds():
                 rows = sorted(r for ch in range(p) for r in range(ch * rpc, min(nrow, (ch + 1) * rpc)))
                 assert rows == list(range(nrow))
                 continue
-            sz = 8 if prec == 0 else 4
+            sz = 4 if prec == 1 else 8
             ud = max((p * gs * sz + 7) // 8, 2 * nf)
             lpt = (ud + p * (3 * nr + 16)) * 8
             q = max(1, 256 // t)

@@ -1,6 +1,6 @@
-"""Pins of the oracle's integrals (int2e_sph, int2e_ip1_sph, int2e_ip2_sph,
-int3c2e_sph, int3c2e_ip1_sph, int3c2e_ip2_sph, int2c2e_sph) against checks
-that do not reuse its algorithm:
+"""Pins of the oracle's integrals (int2e_sph, int2e_cart, int2e_ip1_sph,
+int2e_ip2_sph, int3c2e_sph, int3c2e_cart, int3c2e_ip1_sph, int3c2e_ip2_sph,
+int2c2e_sph) against checks that do not reuse its algorithm:
 
 * an independent McMurchie-Davidson implementation (tests/md_eri.py) with
   numerically fitted spherical harmonics;
@@ -43,6 +43,33 @@ def test_against_mcmurchie_davidson_cfg1(oracle, cfg1):
         ref = md_eri.int2e_sph_md(shls, cfg1.atm, cfg1.bas, cfg1.env)
         got, _ = block(oracle, "int2e_sph", shls, cfg1.atm, cfg1.bas, cfg1.env)
         assert_close(got[0], ref)
+
+
+def test_int2e_cart_against_mcmurchie_davidson(oracle, cfg1):
+    rng = np.random.default_rng(15)
+    gc = M.general_contraction_h2()
+    cases = [(cfg1, [int(s) for s in rng.integers(0, cfg1.nbas, 4)]) for _ in range(25)]
+    cases += [(gc, [int(s) for s in rng.integers(0, gc.nbas, 4)]) for _ in range(6)]
+    for mol, shls in cases:
+        ref = md_eri.int2e_cart_md(shls, mol.atm, mol.bas, mol.env)
+        got, _ = block(oracle, "int2e_cart", shls, mol.atm, mol.bas, mol.env)
+        assert_close(got[0], ref, atol=1e-11, rtol=1e-11)
+
+
+def test_int3c2e_cart_equals_int2e_cart_with_unit_function(oracle):
+    mol = M.config(4)
+    rng = np.random.default_rng(16)
+    for _ in range(10):
+        i, j = [int(s) for s in rng.integers(0, mol.aux_start, 2)]
+        k = int(rng.integers(mol.aux_start, mol.nbas))
+        m2 = M.Molecule()
+        m2.coords, m2.charges = list(mol.coords), list(mol.charges)
+        m2.shells = list(mol.shells)
+        m2.add_shell(mol.shells[k][0], 0, [0.0], [[1.0]])
+        m2.build()
+        three, _ = block(oracle, "int3c2e_cart", [i, j, k], mol.atm, mol.bas, mol.env)
+        four, _ = block(oracle, "int2e_cart", [i, j, k, m2.nbas - 1], m2.atm, m2.bas, m2.env)
+        assert_close(three[0], four[0][..., 0] / 0.282094791773878143, atol=1e-13)
 
 
 def test_against_mcmurchie_davidson_general_contraction(oracle):
