@@ -1,19 +1,40 @@
 // Coulomb/exchange digestion of unique shell quartets:
-//   J[p][q] += (pq|rs) D[r][s],  K[p][r] += (pq|rs) D[q][s]
-// over every distinct index image of each stored quartet block (the 8-fold
-// permutational symmetry (ij|kl) = (ji|kl) = (ij|lk) = (kl|ij), deduplicated at
-// shell level, so a block whose bra shells coincide is not counted twice).
-// One workgroup per quartet; per image the lanes reduce over the contracted
-// pair and issue one FP64 atomic per J or K element of the image (instead of
-// one per integral).
-// The consumer side of SURVEY's J/K build (a Fock-like contraction against a
-// symmetric density); the block layout is the plan's canonical one
-// ([l][k][j][i], i fastest).
+//   J[p][q] = sum_rs (pq|rs) D[r][s],  K[p][r] = sum_qs (pq|rs) D[q][s]
+// for a symmetric D.  A stored quartet (IJ|KL) stands for its orbit under the
+// 8-fold permutational symmetry; summed over all 8 permutations with weight
+// 1/|stabiliser| (= distinct images / 8) every ordered quartet counts once.
+// With D symmetric the 8 permutations give only six distinct reductions and
+// their transposes:
+//   A[I,J] += 2w sum_kl (ij|kl) D_kl,   A[K,L] += 2w sum_ij (ij|kl) D_ij,
+//   B[I,K] += w sum_jl (ij|kl) D_jl,    B[J,K] += w sum_il (ij|kl) D_il,
+//   B[I,L] += w sum_jk (ij|kl) D_jk,    B[J,L] += w sum_ik (ij|kl) D_ik,
+// and J = A + A^T, K = B + B^T (jk_symmetrize).  Each reduction issues one
+// FP64 atomic per output element.  The block layout is the plan's canonical
+// one ([l][k][j][i], i fastest).
 #include <hip/hip_runtime.h>
 
 #include "common.h"
 
 namespace cx {
+
+// out[(o[a]+ma), (o[b]+mb)] += wgt sum_{mc, md} blk[...] D[(o[c]+mc), (o[d]+md)]
+// over the output elements ln, ln + nl, ...
+__device__ __forceinline__ void jk_reduce(const double* __restrict__ blk, const int* ns,
+                                          const int* st, const int* off, int a, int b, int c,
+                                          int d, double wgt, const double* __restrict__ D,
+                                          double* __restrict__ out, int nao, int ln, int nl) {
+  const int na = ns[a], nb = ns[b], nc = ns[c], nd = ns[d];
+  for (int o = ln; o < na * nb; o += nl) {
+    const int ma = o % na, mb = o / na;
+    const double* e = blk + ma * st[a] + mb * st[b];
+    double sum = 0;
+    for (int md = 0; md < nd; ++md) {
+      const double* drow = D + (size_t)(off[d] + md) + (size_t)off[c] * nao;
+      for (int mc = 0; mc < nc; ++mc) sum += e[mc * st[c] + md * st[d]] * drow[(size_t)mc * nao];
+    }
+    if (sum != 0.0) atomicAdd(&out[(size_t)(off[a] + ma) * nao + off[b] + mb], wgt * sum);
+  }
+}
 
 // The digestion of quartet t by `nl` cooperating lanes (lane index `ln`).
 __device__ __forceinline__ void jk_digest_one(
@@ -25,50 +46,21 @@ __device__ __forceinline__ void jk_digest_one(
   const int sh[4] = {pair_sh[pq.x].x, pair_sh[pq.x].y, pair_sh[pq.y].x, pair_sh[pq.y].y};
   const int ns[4] = {2 * shl_l[sh[0]] + 1, 2 * shl_l[sh[1]] + 1, 2 * shl_l[sh[2]] + 1,
                      2 * shl_l[sh[3]] + 1};
-  // distinct shell-level images: position order of (p, q, r, s)
-  int img[8][4];
-  int nimg = 0;
-  const int perms[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
-                           {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
-  for (int a = 0; a < 8; ++a) {
-    bool dup = false;
-    for (int b = 0; b < nimg && !dup; ++b) {
-      bool same = true;
-      for (int c = 0; c < 4; ++c) same = same && sh[perms[a][c]] == sh[img[b][c]];
-      dup = same;
-    }
-    if (!dup) {
-      for (int c = 0; c < 4; ++c) img[nimg][c] = perms[a][c];
-      ++nimg;
-    }
-  }
+  const int off[4] = {ao_off[sh[0]], ao_off[sh[1]], ao_off[sh[2]], ao_off[sh[3]]};
+  // distinct shell-level images of (IJ|KL): 8 / |stabiliser|
+  const bool ij = sh[0] == sh[1], kl = sh[2] == sh[3];
+  const bool braket = (sh[0] == sh[2] && sh[1] == sh[3]) || (sh[0] == sh[3] && sh[1] == sh[2]);
+  int stab = (ij ? 2 : 1) * (kl ? 2 : 1);
+  if (braket) stab *= 2;
+  const double w = 1.0 / stab;
   const double* blk = eri + out_base + t * block;
-  // per image, reduce over the summed pair first, then one atomic per output
-  // element: J(p,q) = sum_rs (pq|rs) D(r,s), K(p,r) = sum_qs (pq|rs) D(q,s)
   const int st[4] = {1, ns[0], ns[0] * ns[1], ns[0] * ns[1] * ns[2]};
-  for (int a = 0; a < nimg; ++a) {
-    const int P = img[a][0], Q = img[a][1], R = img[a][2], S = img[a][3];
-    const int np = ns[P], nq2 = ns[Q], nr = ns[R], nsv = ns[S];
-    const int op = ao_off[sh[P]], oq = ao_off[sh[Q]], orr = ao_off[sh[R]], os = ao_off[sh[S]];
-    for (int o = ln; o < np * nq2; o += nl) {
-      const int mp = o % np, mq = o / np;
-      double sum = 0;
-      for (int ms = 0; ms < nsv; ++ms)
-        for (int mr = 0; mr < nr; ++mr)
-          sum += blk[mp * st[P] + mq * st[Q] + mr * st[R] + ms * st[S]] *
-                 D[(size_t)(orr + mr) * nao + os + ms];
-      if (sum != 0.0) atomicAdd(&J[(size_t)(op + mp) * nao + oq + mq], sum);
-    }
-    for (int o = ln; o < np * nr; o += nl) {
-      const int mp = o % np, mr = o / np;
-      double sum = 0;
-      for (int ms = 0; ms < nsv; ++ms)
-        for (int mq = 0; mq < nq2; ++mq)
-          sum += blk[mp * st[P] + mq * st[Q] + mr * st[R] + ms * st[S]] *
-                 D[(size_t)(oq + mq) * nao + os + ms];
-      if (sum != 0.0) atomicAdd(&K[(size_t)(op + mp) * nao + orr + mr], sum);
-    }
-  }
+  jk_reduce(blk, ns, st, off, 0, 1, 2, 3, 2 * w, D, J, nao, ln, nl);
+  jk_reduce(blk, ns, st, off, 2, 3, 0, 1, 2 * w, D, J, nao, ln, nl);
+  jk_reduce(blk, ns, st, off, 0, 2, 1, 3, w, D, K, nao, ln, nl);
+  jk_reduce(blk, ns, st, off, 1, 2, 0, 3, w, D, K, nao, ln, nl);
+  jk_reduce(blk, ns, st, off, 0, 3, 1, 2, w, D, K, nao, ln, nl);
+  jk_reduce(blk, ns, st, off, 1, 3, 0, 2, w, D, K, nao, ln, nl);
 }
 
 // Large blocks: one 64-lane workgroup per quartet.
@@ -105,6 +97,26 @@ hipError_t launch_jk_digest(const double* eri, int64_t out_base, int64_t block, 
     hipLaunchKernelGGL(jk_digest_kernel, dim3(grid), dim3(64), 0, s, eri, out_base, block, q,
                        nq, pair_sh, ao_off, shl_l, D, J, K, nao);
   }
+  return hipGetLastError();
+}
+
+// A <- scale (A + A^T), one thread per element pair p <= q (in place).
+__global__ void __launch_bounds__(256) jk_symmetrize_kernel(double* __restrict__ A, int n,
+                                                            double scale) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int p = (int)(e / n), q = (int)(e % n);
+  if (p > q) return;
+  const double s = scale * (A[(size_t)p * n + q] + A[(size_t)q * n + p]);
+  A[(size_t)p * n + q] = s;
+  A[(size_t)q * n + p] = s;
+}
+
+hipError_t launch_jk_symmetrize(double* A, int n, double scale, hipStream_t s) {
+  const int64_t nn = (int64_t)n * n;
+  if (nn == 0) return hipSuccess;
+  hipLaunchKernelGGL(jk_symmetrize_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, A,
+                     n, scale);
   return hipGetLastError();
 }
 

@@ -41,6 +41,7 @@ hipError_t mlp_eval_device(const float* prm, const RysTable& tab, int n, const d
                            double* perr, hipStream_t st);
 int mlp_nparam();
 hipError_t launch_copy_probe(const double* src, double* dst, int64_t n, hipStream_t s);
+hipError_t launch_jk_symmetrize(double* A, int n, double scale, hipStream_t s);
 hipError_t launch_jk_digest(const double* eri, int64_t out_base, int64_t block, const int2* q,
                             int64_t nq, const int2* pair_sh, const int* ao_off, const int* shl_l,
                             const double* D, double* J, double* K, int nao, hipStream_t s);
@@ -1963,10 +1964,15 @@ int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
     HIPCHK(hipMemcpyAsync(dsh.p, plan->pairs.shells.data(),
                           plan->pairs.shells.size() * sizeof(int2), hipMemcpyHostToDevice,
                           C.stream));
+    // D is taken as symmetric: (D + D^T) / 2 (the digest relies on it)
+    HIPCHK(launch_jk_symmetrize(dD.p, nao, 0.5, C.stream));
     plan_launch_all(plan, plan->d_out, false);
     for (const Bin& b : plan->bins)
       HIPCHK(launch_jk_digest(plan->d_out, b.out_base, b.block, b.d_q, (int64_t)b.q.size(), dsh.p,
                               dao.p, dl.p, dD.p, dJ.p, dK.p, nao, C.stream));
+    // the digest accumulates one triangle's worth: J = A + A^T, K = B + B^T
+    HIPCHK(launch_jk_symmetrize(dJ.p, nao, 1.0, C.stream));
+    HIPCHK(launch_jk_symmetrize(dK.p, nao, 1.0, C.stream));
     HIPCHK(hipMemcpyAsync(J, dJ.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
     HIPCHK(hipMemcpyAsync(K, dK.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
     HIPCHK(hipStreamSynchronize(C.stream));
