@@ -3,9 +3,9 @@
 // (high angular momentum: (ff|..), (gg|gg), the s..g basis workload).  One
 // shell tuple per 256-lane workgroup, grid-stride over tuples.  The g tensor
 // lives in LDS when it fits in 56 KB, else in a per-workgroup global scratch
-// slice; the Cartesian accumulators live in registers (16 elements per lane)
-// up to 4096 elements, else in the slice; the cart->sph buffers are in the
-// slice.  Arithmetic is the same as eri_class_kernel (libcint g2e.c:272-690,
+// slice; the Cartesian accumulators live in registers (16 elements per lane,
+// 8 for nabla_i; larger blocks in chunks); the chunk results and the
+// cart->sph buffers are in the slice.  Arithmetic is the same as eri_class_kernel (libcint g2e.c:272-690,
 // 4425-4620).
 #include "eri_device.h"
 
@@ -20,110 +20,135 @@ struct GenShape {
   int ibase, kbase;
 };
 
-__device__ void generic_chain(double* __restrict__ g, const GenShape& S, int axis, int r,
-                              double sr, double qr, double wz, double inv_sum, double hij, double hkl,
-                              double aij, double akl, double dx, double prb, double qrk,
-                              double drb, double drk) {
+// The recurrences of one primitive tuple in two parallel phases.
+// generic_vrr_bra, one (axis, root) per lane: the VRR columns m <= MMAX and
+// the bra transfer of each column into g(i, j, slot m).  generic_ket, one
+// (axis, root, (i, j) block) per lane: the ket transfer of that block from the
+// slots m to (k, l).  The work arrays are indexed with compile-time positions
+// only (loops unrolled to the largest size, guarded by the runtime bounds) so
+// they stay in registers.
+constexpr int kChainMax = 10;  // NMAX, MMAX <= 9 (nabla_i on (gg|..))
+__device__ void generic_vrr_bra(double* __restrict__ g, const GenShape& S, int axis, int r,
+                                double sr, double qr, double wz, double inv_sum, double hij,
+                                double hkl, double aij, double akl, double dx, double prb,
+                                double qrk, double drb) {
   const int NR = S.nr;
   const int NMAX = S.lic + S.l[1], MMAX = S.l[2] + S.l[3];
   const int A = S.ibase ? S.lic : S.l[1], B = S.ibase ? S.l[1] : S.lic;
-  const int C = S.kbase ? S.l[2] : S.l[3], D = S.kbase ? S.l[3] : S.l[2];
-  const int DJ = S.dj, DL = S.dl, BLK = S.dk * S.dl;
+  const int DJ = S.dj, BLK = S.dk * S.dl;
   const double sa = sr * inv_sum, oms = qr;
   const double b00 = 0.5 * sa, b10 = 0.5 * sa + oms * hij, b01 = 0.5 * sa + oms * hkl;
   const double c00 = prb - sa * akl * dx;
   const double c0p = qrk + sa * aij * dx;
   double* ga = g + (size_t)axis * S.g4 * NR + r;
-  double cm1[12], c0[12], c1[12], t[12];
+  double cm1[kChainMax], c0[kChainMax], t[kChainMax];
+#pragma unroll
+  for (int n = 0; n < kChainMax; ++n) {
+    cm1[n] = 0.0;
+    c0[n] = 0.0;
+  }
   c0[0] = (axis == 2) ? wz : 1.0;
-  for (int n = 0; n < NMAX; ++n) c0[n + 1] = c00 * c0[n] + (n > 0 ? n * b10 * c0[n - 1] : 0.0);
+#pragma unroll
+  for (int n = 0; n + 1 < kChainMax; ++n)
+    if (n < NMAX) c0[n + 1] = c00 * c0[n] + (n > 0 ? n * b10 * c0[n - 1] : 0.0);
   for (int m = 0; m <= MMAX; ++m) {
     if (m > 0) {
-      for (int n = 0; n <= NMAX; ++n) {
-        double v = c0p * c0[n];
-        if (m > 1) v += (m - 1) * b01 * cm1[n];
+      // descending n: c0[n - 1] is still the previous column when read
+#pragma unroll
+      for (int n = kChainMax - 1; n >= 0; --n) {
+        double v = c0p * c0[n] + (m - 1) * b01 * cm1[n];
         if (n > 0) v += n * b00 * c0[n - 1];
-        c1[n] = v;
-      }
-      for (int n = 0; n <= NMAX; ++n) {
         cm1[n] = c0[n];
-        c0[n] = c1[n];
+        c0[n] = v;
       }
     }
-    // bra transfer of column m into block(i, j) slot m
-    for (int a = 0; a <= NMAX; ++a) t[a] = c0[a];
-    for (int b = 0; b <= B; ++b) {
-      if (b > 0)
-        for (int a = 0; a <= A + B - b; ++a) t[a] = t[a + 1] + drb * t[a];
-      for (int a = 0; a <= A; ++a) {
+#pragma unroll
+    for (int a = 0; a < kChainMax; ++a) t[a] = c0[a];
+#pragma unroll
+    for (int b = 0; b < kChainMax; ++b) {
+      if (b > B) break;
+      if (b > 0) {
+#pragma unroll
+        for (int a = 0; a + 1 < kChainMax; ++a)
+          if (a <= A + B - b) t[a] = t[a + 1] + drb * t[a];
+      }
+#pragma unroll
+      for (int a = 0; a < kChainMax; ++a) {
+        if (a > A) break;
         const int i = S.ibase ? a : b, j = S.ibase ? b : a;
         ga[((i * DJ + j) * BLK + m) * NR] = t[a];
       }
     }
   }
-  for (int ij = 0; ij < S.di * S.dj; ++ij) {
-    double* blk = ga + (size_t)ij * BLK * NR;
-    for (int c = 0; c <= MMAX; ++c) t[c] = blk[c * NR];
-    for (int d = 0; d <= D; ++d) {
-      if (d > 0)
-        for (int c = 0; c <= C + D - d; ++c) t[c] = t[c + 1] + drk * t[c];
-      for (int c = 0; c <= C; ++c) {
-        const int k = S.kbase ? c : d, l = S.kbase ? d : c;
-        blk[(k * DL + l) * NR] = t[c];
-      }
+}
+
+__device__ void generic_ket(double* __restrict__ g, const GenShape& S, int axis, int r, int ij,
+                            double drk) {
+  const int NR = S.nr;
+  const int MMAX = S.l[2] + S.l[3];
+  const int C = S.kbase ? S.l[2] : S.l[3], D = S.kbase ? S.l[3] : S.l[2];
+  const int DL = S.dl, BLK = S.dk * S.dl;
+  double* blk = g + (size_t)axis * S.g4 * NR + r + (size_t)ij * BLK * NR;
+  double t[kChainMax];
+#pragma unroll
+  for (int c = 0; c < kChainMax; ++c) t[c] = c <= MMAX ? blk[c * NR] : 0.0;
+#pragma unroll
+  for (int d = 0; d < kChainMax; ++d) {
+    if (d > D) break;
+    if (d > 0) {
+#pragma unroll
+      for (int c = 0; c + 1 < kChainMax; ++c)
+        if (c <= C + D - d) t[c] = t[c + 1] + drk * t[c];
+    }
+#pragma unroll
+    for (int c = 0; c < kChainMax; ++c) {
+      if (c > C) break;
+      const int k = S.kbase ? c : d, l = S.kbase ? d : c;
+      blk[(k * DL + l) * NR] = t[c];
     }
   }
 }
 
 // GLDS: the g tensor in (dynamic) LDS instead of the global slice.
-// AREG: every lane keeps its at most kGenAcc elements (t + 256 k) and their
-// packed g positions in registers instead of global accumulators.
-constexpr int kGenAcc = 16;
+// Every lane keeps ACC elements of the current element chunk (t + 256 k;
+// kGenAcc, halved for nabla_i's three components) and their packed g
+// positions in registers.  A block larger than one chunk (256 ACC Cartesian
+// elements: (ff|ff), (gg|..)) is done chunk by chunk,
+// each chunk a full pass over the primitive tuples with the roots and chains
+// recomputed, so that the accumulators stay in registers instead of taking a
+// global read-modify-write per element and primitive tuple.
+constexpr int kGenAcc = 8;
 constexpr size_t kGenLdsBytes = 56 * 1024;  // dynamic LDS for g (plus ~0.5 KB static)
 
-template <bool GLDS, bool AREG>
-__global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
+template <bool GLDS, bool DERIV>
+__global__ void __launch_bounds__(256, 2) eri_generic_kernel(EriArgs a, GenShape S,
                                                           int64_t per_block) {
+  constexpr int NC = DERIV ? 3 : 1;
+  constexpr int ACC = DERIV ? kGenAcc / 2 : kGenAcc;  // elements per lane
+  constexpr int CHUNK = 256 * ACC;
   extern __shared__ double gl[];
   const int t = threadIdx.x;
   const int T = blockDim.x;
   const int NR = S.nr;
   double* slice = a.scratch + (size_t)blockIdx.x * per_block;
   double* g = GLDS ? gl : slice;
-  double* acc = slice + (size_t)3 * S.g4 * NR;
+  double* acc = slice + (size_t)3 * S.g4 * NR;  // [ncomp][nf]: chunk results
   double* b0 = acc + (size_t)S.nf * S.ncomp;
   double* b1 = b0 + S.nf;
   __shared__ double uw[3 * kMaxRoots];
   __shared__ unsigned long long cnt[2];
   const int nfi = ncart(S.l[0]), nfj = ncart(S.l[1]), nfk = ncart(S.l[2]), nfl = ncart(S.l[3]);
   const int SI = S.dj * S.dk * S.dl;
-  // this lane's elements: packed g positions x | y << 10 | z << 20 and the
-  // i powers (nabla_i) x | y << 4 | z << 8
-  int pos[AREG ? kGenAcc : 1], ipw[AREG ? kGenAcc : 1];
-  double racc[AREG ? 3 * kGenAcc : 1];
-  if constexpr (AREG) {
-#pragma unroll
-    for (int k = 0; k < kGenAcc; ++k) {
-      const int e = t + k * 256;
-      pos[k] = 0;
-      ipw[k] = 0;
-      if (e < S.nf) {
-        int fi = e % nfi, e1 = e / nfi;
-        int fj = e1 % nfj, e2 = e1 / nfj;
-        int fk = e2 % nfk, fl = e2 / nfk;
-        int ix, iy, iz, jx, jy, jz, kx, ky, kz, lx, ly, lz;
-        cart_decode(S.l[0], fi, ix, iy, iz);
-        cart_decode(S.l[1], fj, jx, jy, jz);
-        cart_decode(S.l[2], fk, kx, ky, kz);
-        cart_decode(S.l[3], fl, lx, ly, lz);
-        const int px = ((ix * S.dj + jx) * S.dk + kx) * S.dl + lx;
-        const int py = ((iy * S.dj + jy) * S.dk + ky) * S.dl + ly;
-        const int pz = ((iz * S.dj + jz) * S.dk + kz) * S.dl + lz;
-        pos[k] = px | (py << 10) | (pz << 20);
-        ipw[k] = ix | (iy << 4) | (iz << 8);
-      }
-    }
-  }
+  const int IJ = S.di * S.dj;
+  const int ket_tasks = (S.l[2] > 0 && S.l[3] > 0) ? 3 * IJ * NR : 0;
+  // this lane's elements: offsets (in doubles) of their x, y, z g entries;
+  // for nabla_i also the i-1 entries (the element's own when the i power is
+  // 0) and the i powers as multipliers
+  constexpr int ND = DERIV ? ACC : 1;
+  int ox[ACC], oy[ACC], oz[ACC];
+  int lx[ND], ly[ND], lz[ND];
+  double mx[ND], my[ND], mz[ND];
+  double racc[NC * ACC];
   for (int64_t q = blockIdx.x; q < a.nq; q += gridDim.x) {
     const int2 pq = a.quartets[q];
     const int boff = a.pair_off[pq.x], koff = a.pair_off[pq.y];
@@ -138,130 +163,146 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
       rij[d] = S.ibase ? ri[d] - rj[d] : rj[d] - ri[d];
       rkl[d] = S.kbase ? rk[d] - rl[d] : rl[d] - rk[d];
     }
-    if constexpr (AREG) {
-#pragma unroll
-      for (int k = 0; k < 3 * kGenAcc; ++k) racc[k] = 0.0;
-    } else {
-      for (int e = t; e < S.nf * S.ncomp; e += T) acc[e] = 0.0;
-    }
     if (t == 0) cnt[0] = 0;
-    __syncthreads();
     const double logd_ij = gb[6], logd_kl = gk[6];
     const int np_ij = a.pair_np[pq.x], np_kl = a.pair_np[pq.y];
-    for (int kp = 0; kp < np_kl; ++kp) {
-      const int kidx = koff + kp;
-      const double* pk = a.prim + 8 * (size_t)kidx;
-      const double akl = pk[0];
-      const double Qv[3] = {pk[1], pk[2], pk[3]};
-      const double ccekl = a.three_center ? 0.0 : pk[6] - a.lkl_ceil * logd_kl;
-      if (ccekl > a.expcutoff) continue;  // uniform across the workgroup
-      const double eijcut = a.expcutoff - ccekl;
-      for (int ip = 0; ip < np_ij; ++ip) {
-        const int iidx = boff + ip;
-        const double* pi = a.prim + 8 * (size_t)iidx;
-        const double cceij = pi[6] - a.lij_ceil * logd_ij;
-        // (libcint zeroes the prefactor of a bra pair with cce >= expcutoff)
-        if (cceij > eijcut || cceij >= a.expcutoff) continue;
-        const double aij = pi[0];
-        const double Pv[3] = {pi[1], pi[2], pi[3]};
-        double dx[3], prb[3], qrk[3], rr = 0;
-        for (int d = 0; d < 3; ++d) {
-          dx[d] = Pv[d] - Qv[d];
-          rr += dx[d] * dx[d];
-          prb[d] = Pv[d] - (S.ibase ? ri[d] : rj[d]);
-          qrk[d] = Qv[d] - (S.kbase ? rk[d] : rl[d]);
-        }
-        const double inv_sum = 1.0 / (aij + akl);
-        const double hij = pi[5], hkl = pk[5];
-        const double a0 = aij * akl * inv_sum;
-        const double fac = a.common * pi[4] * pk[4];
-        const double fac1 = 4.0 * hij * hkl * sqrt(inv_sum) * fac;
-        const double x = a0 * rr;
-        if (t < NR) {
-          double sr, qr, w;
-          rys_eval(a.rys, NR, t, x, sr, qr, w);
-          uw[t] = sr;
-          uw[NR + t] = w * fac1;
-          uw[2 * NR + t] = qr;
-        }
-        if (t == 0) ++cnt[0];
-        __syncthreads();
-        for (int c = t; c < 3 * NR; c += T) {
-          const int axis = c / NR, r = c - axis * NR;
-          generic_chain(g, S, axis, r, uw[r], uw[2 * NR + r], uw[NR + r], inv_sum, hij, hkl, aij,
-                        akl, dx[axis], prb[axis], qrk[axis], rij[axis], rkl[axis]);
-        }
-        __syncthreads();
-        const double ai2 = -2.0 * pi[7];
-        if constexpr (AREG) {
+    for (int e0 = 0; e0 < S.nf; e0 += CHUNK) {
 #pragma unroll
-          for (int k = 0; k < kGenAcc; ++k) {
-            if (t + k * 256 >= S.nf) break;
-            const double* gx = g + (size_t)(pos[k] & 1023) * NR;
-            const double* gy = g + (size_t)(S.g4 + ((pos[k] >> 10) & 1023)) * NR;
-            const double* gz = g + (size_t)(2 * S.g4 + (pos[k] >> 20)) * NR;
-            if (!S.deriv) {
-              double sum = 0;
-              for (int r = 0; r < NR; ++r) sum += gx[r] * gy[r] * gz[r];
-              racc[k] += sum;
-            } else {
-              const int ix = ipw[k] & 15, iy = (ipw[k] >> 4) & 15, iz = ipw[k] >> 8;
-              double sx = 0, sy = 0, sz = 0;
-              for (int r = 0; r < NR; ++r) {
-                double fx = ai2 * gx[SI * NR + r], fy = ai2 * gy[SI * NR + r],
-                       fz = ai2 * gz[SI * NR + r];
-                if (ix > 0) fx += ix * gx[r - SI * NR];
-                if (iy > 0) fy += iy * gy[r - SI * NR];
-                if (iz > 0) fz += iz * gz[r - SI * NR];
-                sx += fx * gy[r] * gz[r];
-                sy += gx[r] * fy * gz[r];
-                sz += gx[r] * gy[r] * fz;
-              }
-              racc[k] += sx;
-              racc[kGenAcc + k] += sy;
-              racc[2 * kGenAcc + k] += sz;
-            }
-          }
-        } else {
-          for (int e = t; e < S.nf; e += T) {
-            int fi = e % nfi, e1 = e / nfi;
-            int fj = e1 % nfj, e2 = e1 / nfj;
-            int fk = e2 % nfk, fl = e2 / nfk;
-            int ix, iy, iz, jx, jy, jz, kx, ky, kz, lx, ly, lz;
-            cart_decode(S.l[0], fi, ix, iy, iz);
-            cart_decode(S.l[1], fj, jx, jy, jz);
-            cart_decode(S.l[2], fk, kx, ky, kz);
-            cart_decode(S.l[3], fl, lx, ly, lz);
-            const double* gx = g + (size_t)(((ix * S.dj + jx) * S.dk + kx) * S.dl + lx) * NR;
-            const double* gy =
-                g + (size_t)(S.g4 + ((iy * S.dj + jy) * S.dk + ky) * S.dl + ly) * NR;
-            const double* gz =
-                g + (size_t)(2 * S.g4 + ((iz * S.dj + jz) * S.dk + kz) * S.dl + lz) * NR;
-            if (!S.deriv) {
-              double sum = 0;
-              for (int r = 0; r < NR; ++r) sum += gx[r] * gy[r] * gz[r];
-              acc[e] += sum;
-            } else {
-              double sx = 0, sy = 0, sz = 0;
-              for (int r = 0; r < NR; ++r) {
-                double fx = ai2 * gx[SI * NR + r], fy = ai2 * gy[SI * NR + r],
-                       fz = ai2 * gz[SI * NR + r];
-                if (ix > 0) fx += ix * gx[r - SI * NR];
-                if (iy > 0) fy += iy * gy[r - SI * NR];
-                if (iz > 0) fz += iz * gz[r - SI * NR];
-                sx += fx * gy[r] * gz[r];
-                sy += gx[r] * fy * gz[r];
-                sz += gx[r] * gy[r] * fz;
-              }
-              acc[e] += sx;
-              acc[S.nf + e] += sy;
-              acc[2 * S.nf + e] += sz;
-            }
-          }
+      for (int k = 0; k < ACC; ++k) {
+        const int e = e0 + t + k * 256;
+        int px = 0, py = 0, pz = 0, ix = 0, iy = 0, iz = 0;
+        if (e < S.nf) {  // elements past the block read entry 0 and are not stored
+          int fi = e % nfi, e1 = e / nfi;
+          int fj = e1 % nfj, e2 = e1 / nfj;
+          int fk = e2 % nfk, fl = e2 / nfk;
+          int jx, jy, jz, kx, ky, kz, lx_, ly_, lz_;
+          cart_decode(S.l[0], fi, ix, iy, iz);
+          cart_decode(S.l[1], fj, jx, jy, jz);
+          cart_decode(S.l[2], fk, kx, ky, kz);
+          cart_decode(S.l[3], fl, lx_, ly_, lz_);
+          px = ((ix * S.dj + jx) * S.dk + kx) * S.dl + lx_;
+          py = ((iy * S.dj + jy) * S.dk + ky) * S.dl + ly_;
+          pz = ((iz * S.dj + jz) * S.dk + kz) * S.dl + lz_;
         }
-        __syncthreads();
+        ox[k] = px * NR;
+        oy[k] = (S.g4 + py) * NR;
+        oz[k] = (2 * S.g4 + pz) * NR;
+        if constexpr (DERIV) {
+          lx[k] = ix > 0 ? ox[k] - SI * NR : ox[k];
+          ly[k] = iy > 0 ? oy[k] - SI * NR : oy[k];
+          lz[k] = iz > 0 ? oz[k] - SI * NR : oz[k];
+          mx[k] = ix;
+          my[k] = iy;
+          mz[k] = iz;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NC * ACC; ++k) racc[k] = 0.0;
+      __syncthreads();
+      for (int kp = 0; kp < np_kl; ++kp) {
+        const int kidx = koff + kp;
+        const double* pk = a.prim + 8 * (size_t)kidx;
+        const double akl = pk[0];
+        const double Qv[3] = {pk[1], pk[2], pk[3]};
+        const double ccekl = a.three_center ? 0.0 : pk[6] - a.lkl_ceil * logd_kl;
+        if (ccekl > a.expcutoff) continue;  // uniform across the workgroup
+        const double eijcut = a.expcutoff - ccekl;
+        for (int ip = 0; ip < np_ij; ++ip) {
+          const int iidx = boff + ip;
+          const double* pi = a.prim + 8 * (size_t)iidx;
+          const double cceij = pi[6] - a.lij_ceil * logd_ij;
+          // (libcint zeroes the prefactor of a bra pair with cce >= expcutoff)
+          if (cceij > eijcut || cceij >= a.expcutoff) continue;
+          const double aij = pi[0];
+          const double Pv[3] = {pi[1], pi[2], pi[3]};
+          double dx[3], prb[3], qrk[3], rr = 0;
+          for (int d = 0; d < 3; ++d) {
+            dx[d] = Pv[d] - Qv[d];
+            rr += dx[d] * dx[d];
+            prb[d] = Pv[d] - (S.ibase ? ri[d] : rj[d]);
+            qrk[d] = Qv[d] - (S.kbase ? rk[d] : rl[d]);
+          }
+          const double inv_sum = 1.0 / (aij + akl);
+          const double hij = pi[5], hkl = pk[5];
+          const double a0 = aij * akl * inv_sum;
+          const double fac = a.common * pi[4] * pk[4];
+          const double fac1 = 4.0 * hij * hkl * sqrt(inv_sum) * fac;
+          const double x = a0 * rr;
+          if (t < NR) {
+            double sr, qr, w;
+            rys_eval(a.rys, NR, t, x, sr, qr, w);
+            uw[t] = sr;
+            uw[NR + t] = w * fac1;
+            uw[2 * NR + t] = qr;
+          }
+          if (t == 0 && e0 == 0) ++cnt[0];
+          __syncthreads();
+          for (int c = t; c < 3 * NR; c += T) {
+            const int axis = c / NR, r = c - axis * NR;
+            generic_vrr_bra(g, S, axis, r, uw[r], uw[2 * NR + r], uw[NR + r], inv_sum, hij, hkl,
+                            aij, akl, dx[axis], prb[axis], qrk[axis], rij[axis]);
+          }
+          __syncthreads();
+          if (ket_tasks) {  // (with min(lk, ll) = 0 the slots are already (k, l))
+            for (int c = t; c < ket_tasks; c += T) {
+              const int r = c % NR, rest = c / NR;
+              const int ij = rest % IJ, axis = rest / IJ;
+              generic_ket(g, S, axis, r, ij, rkl[axis]);
+            }
+            __syncthreads();
+          }
+          // roots outer, elements inner: the ACC elements' LDS reads of one
+          // root are independent, so they overlap instead of each element's
+          // root loop waiting on its own reads; per element the sum over the
+          // roots is formed first, as before
+          const double ai2 = -2.0 * pi[7];
+          const int up = SI * NR;
+          double s0[ACC], s1[ND], s2[ND];
+#pragma unroll
+          for (int k = 0; k < ACC; ++k) s0[k] = 0.0;
+#pragma unroll
+          for (int k = 0; k < ND; ++k) {
+            s1[k] = 0.0;
+            s2[k] = 0.0;
+          }
+          for (int r = 0; r < NR; ++r) {
+#pragma unroll
+            for (int k = 0; k < ACC; ++k) {
+              const double gx = g[ox[k] + r], gy = g[oy[k] + r], gz = g[oz[k] + r];
+              if constexpr (!DERIV) {
+                s0[k] += gx * gy * gz;
+              } else {
+                const double fx = ai2 * g[ox[k] + up + r] + mx[k] * g[lx[k] + r];
+                const double fy = ai2 * g[oy[k] + up + r] + my[k] * g[ly[k] + r];
+                const double fz = ai2 * g[oz[k] + up + r] + mz[k] * g[lz[k] + r];
+                s0[k] += fx * gy * gz;
+                s1[k] += gx * fy * gz;
+                s2[k] += gx * gy * fz;
+              }
+            }
+          }
+#pragma unroll
+          for (int k = 0; k < ACC; ++k) racc[k] += s0[k];
+          if constexpr (DERIV) {
+#pragma unroll
+            for (int k = 0; k < ACC; ++k) {
+              racc[ACC + k] += s1[k];
+              racc[2 * ACC + k] += s2[k];
+            }
+          }
+          __syncthreads();
+        }
+      }
+      // the chunk's results to the slice
+#pragma unroll
+      for (int k = 0; k < ACC; ++k) {
+        const int e = e0 + t + k * 256;
+#pragma unroll
+        for (int comp = 0; comp < NC; ++comp)
+          if (e < S.nf) acc[(size_t)comp * S.nf + e] = racc[comp * ACC + k];
       }
     }
+    __syncthreads();
     if (a.counters && t == 0 && cnt[0]) {
       atomicAdd(a.counters, cnt[0]);
       atomicAdd(a.counters + 1, 1ull);
@@ -270,13 +311,7 @@ __global__ void __launch_bounds__(256) eri_generic_kernel(EriArgs a, GenShape S,
     const int ns2 = a.cart ? nfk : nsph(S.l[2]), ns3 = a.cart ? nfl : nsph(S.l[3]);
     const int NS = ns0 * ns1 * ns2 * ns3;
     for (int comp = 0; comp < S.ncomp; ++comp) {
-      if constexpr (AREG) {
-#pragma unroll
-        for (int k = 0; k < kGenAcc; ++k)
-          if (t + k * 256 < S.nf) b0[t + k * 256] = racc[comp * kGenAcc + k];
-      } else {
-        for (int e = t; e < S.nf; e += T) b0[e] = acc[comp * S.nf + e];
-      }
+      for (int e = t; e < S.nf; e += T) b0[e] = acc[(size_t)comp * S.nf + e];
       __syncthreads();
       int dims[4] = {nfi, nfj, nfk, nfl};
       double* src = b0;
@@ -329,18 +364,15 @@ hipError_t launch_generic(const EriArgs& a, const int* l, int deriv, int grid, h
   int64_t per = generic_scratch_per_block(l, deriv);
   const size_t gbytes = (size_t)3 * S.g4 * S.nr * sizeof(double);
   const bool glds = gbytes <= kGenLdsBytes;
-  const bool areg = S.nf <= 256 * kGenAcc;
   const size_t dyn = glds ? gbytes : 0;
-  if (glds && areg)
+  if (glds && deriv)
     hipLaunchKernelGGL((eri_generic_kernel<true, true>), dim3(grid), dim3(256), dyn, s, a, S, per);
   else if (glds)
-    hipLaunchKernelGGL((eri_generic_kernel<true, false>), dim3(grid), dim3(256), dyn, s, a, S,
-                       per);
-  else if (areg)
+    hipLaunchKernelGGL((eri_generic_kernel<true, false>), dim3(grid), dim3(256), dyn, s, a, S, per);
+  else if (deriv)
     hipLaunchKernelGGL((eri_generic_kernel<false, true>), dim3(grid), dim3(256), 0, s, a, S, per);
   else
-    hipLaunchKernelGGL((eri_generic_kernel<false, false>), dim3(grid), dim3(256), 0, s, a, S,
-                       per);
+    hipLaunchKernelGGL((eri_generic_kernel<false, false>), dim3(grid), dim3(256), 0, s, a, S, per);
   return hipGetLastError();
 }
 
