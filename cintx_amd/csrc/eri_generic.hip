@@ -20,28 +20,28 @@ struct GenShape {
   int ibase, kbase;
 };
 
-// The recurrences of one primitive tuple in two parallel phases.
-// generic_vrr_bra, one (axis, root) per lane: the VRR columns m <= MMAX and
-// the bra transfer of each column into g(i, j, slot m).  generic_ket, one
-// (axis, root, (i, j) block) per lane: the ket transfer of that block from the
-// slots m to (k, l).  The work arrays are indexed with compile-time positions
-// only (loops unrolled to the largest size, guarded by the runtime bounds) so
-// they stay in registers.
+// The recurrences of one primitive tuple in three lane-parallel phases.
+// generic_vrr, one (axis, root) per lane: the VRR columns m <= MMAX, column m
+// staged in slot m of the first NMAX + 1 (i, j) blocks (entry n in block n;
+// (A+1)(B+1) >= A+B+1 blocks exist).  generic_bra, one (axis, root, m) per
+// lane: reads its staged column, then the bra transfer into g(i, j, slot m).
+// generic_ket, one (axis, root, (i, j) block) per lane: the ket transfer of
+// that block from the slots m to (k, l).  The work arrays are indexed with
+// compile-time positions only (loops unrolled to the largest size, guarded by
+// the runtime bounds) so they stay in registers.
 constexpr int kChainMax = 10;  // NMAX, MMAX <= 9 (nabla_i on (gg|..))
-__device__ void generic_vrr_bra(double* __restrict__ g, const GenShape& S, int axis, int r,
-                                double sr, double qr, double wz, double inv_sum, double hij,
-                                double hkl, double aij, double akl, double dx, double prb,
-                                double qrk, double drb) {
+__device__ void generic_vrr(double* __restrict__ g, const GenShape& S, int axis, int r, double sr,
+                            double qr, double wz, double inv_sum, double hij, double hkl,
+                            double aij, double akl, double dx, double prb, double qrk) {
   const int NR = S.nr;
   const int NMAX = S.lic + S.l[1], MMAX = S.l[2] + S.l[3];
-  const int A = S.ibase ? S.lic : S.l[1], B = S.ibase ? S.l[1] : S.lic;
-  const int DJ = S.dj, BLK = S.dk * S.dl;
+  const int BLK = S.dk * S.dl;
   const double sa = sr * inv_sum, oms = qr;
   const double b00 = 0.5 * sa, b10 = 0.5 * sa + oms * hij, b01 = 0.5 * sa + oms * hkl;
   const double c00 = prb - sa * akl * dx;
   const double c0p = qrk + sa * aij * dx;
   double* ga = g + (size_t)axis * S.g4 * NR + r;
-  double cm1[kChainMax], c0[kChainMax], t[kChainMax];
+  double cm1[kChainMax], c0[kChainMax];
 #pragma unroll
   for (int n = 0; n < kChainMax; ++n) {
     cm1[n] = 0.0;
@@ -63,21 +63,34 @@ __device__ void generic_vrr_bra(double* __restrict__ g, const GenShape& S, int a
       }
     }
 #pragma unroll
-    for (int a = 0; a < kChainMax; ++a) t[a] = c0[a];
+    for (int n = 0; n < kChainMax; ++n)
+      if (n <= NMAX) ga[((size_t)n * BLK + m) * NR] = c0[n];
+  }
+}
+
+__device__ void generic_bra(double* __restrict__ g, const GenShape& S, int axis, int r, int m,
+                            double drb) {
+  const int NR = S.nr;
+  const int NMAX = S.lic + S.l[1];
+  const int A = S.ibase ? S.lic : S.l[1], B = S.ibase ? S.l[1] : S.lic;
+  const int DJ = S.dj, BLK = S.dk * S.dl;
+  double* ga = g + (size_t)axis * S.g4 * NR + r + (size_t)m * NR;
+  double t[kChainMax];
 #pragma unroll
-    for (int b = 0; b < kChainMax; ++b) {
-      if (b > B) break;
-      if (b > 0) {
+  for (int a = 0; a < kChainMax; ++a) t[a] = a <= NMAX ? ga[(size_t)a * BLK * NR] : 0.0;
 #pragma unroll
-        for (int a = 0; a + 1 < kChainMax; ++a)
-          if (a <= A + B - b) t[a] = t[a + 1] + drb * t[a];
-      }
+  for (int b = 0; b < kChainMax; ++b) {
+    if (b > B) break;
+    if (b > 0) {
 #pragma unroll
-      for (int a = 0; a < kChainMax; ++a) {
-        if (a > A) break;
-        const int i = S.ibase ? a : b, j = S.ibase ? b : a;
-        ga[((i * DJ + j) * BLK + m) * NR] = t[a];
-      }
+      for (int a = 0; a + 1 < kChainMax; ++a)
+        if (a <= A + B - b) t[a] = t[a + 1] + drb * t[a];
+    }
+#pragma unroll
+    for (int a = 0; a < kChainMax; ++a) {
+      if (a > A) break;
+      const int i = S.ibase ? a : b, j = S.ibase ? b : a;
+      ga[(size_t)(i * DJ + j) * BLK * NR] = t[a];
     }
   }
 }
@@ -141,6 +154,8 @@ __global__ void __launch_bounds__(256, 2) eri_generic_kernel(EriArgs a, GenShape
   const int SI = S.dj * S.dk * S.dl;
   const int IJ = S.di * S.dj;
   const int ket_tasks = (S.l[2] > 0 && S.l[3] > 0) ? 3 * IJ * NR : 0;
+  const int NM = S.l[2] + S.l[3] + 1;  // VRR columns
+  const int bra_tasks = 3 * NM * NR;
   // this lane's elements: offsets (in doubles) of their x, y, z g entries;
   // for nabla_i also the i-1 entries (the element's own when the i power is
   // 0) and the i powers as multipliers
@@ -239,8 +254,14 @@ __global__ void __launch_bounds__(256, 2) eri_generic_kernel(EriArgs a, GenShape
           __syncthreads();
           for (int c = t; c < 3 * NR; c += T) {
             const int axis = c / NR, r = c - axis * NR;
-            generic_vrr_bra(g, S, axis, r, uw[r], uw[2 * NR + r], uw[NR + r], inv_sum, hij, hkl,
-                            aij, akl, dx[axis], prb[axis], qrk[axis], rij[axis]);
+            generic_vrr(g, S, axis, r, uw[r], uw[2 * NR + r], uw[NR + r], inv_sum, hij, hkl, aij,
+                        akl, dx[axis], prb[axis], qrk[axis]);
+          }
+          __syncthreads();
+          for (int c = t; c < bra_tasks; c += T) {
+            const int r = c % NR, rest = c / NR;
+            const int m = rest % NM, axis = rest / NM;
+            generic_bra(g, S, axis, r, m, rij[axis]);
           }
           __syncthreads();
           if (ket_tasks) {  // (with min(lk, ll) = 0 the slots are already (k, l))
