@@ -2,7 +2,7 @@
 // nabla_i), used for the classes that have no compile-time instantiation
 // (high angular momentum: (ff|..), (gg|gg), the s..g basis workload).  One
 // shell tuple per 256-lane workgroup, grid-stride over tuples.  The g tensor
-// lives in LDS when it fits in 56 KB, else in a per-workgroup global scratch
+// lives in LDS when it fits in 150 KB, else in a per-workgroup global scratch
 // slice; the Cartesian accumulators live in registers (16 elements per lane,
 // 8 for nabla_i; larger blocks in chunks); the chunk results and the
 // cart->sph buffers are in the slice.  Arithmetic is the same as eri_class_kernel (libcint g2e.c:272-690,
@@ -131,7 +131,7 @@ __device__ void generic_ket(double* __restrict__ g, const GenShape& S, int axis,
 // recomputed, so that the accumulators stay in registers instead of taking a
 // global read-modify-write per element and primitive tuple.
 constexpr int kGenAcc = 8;
-constexpr size_t kGenLdsBytes = 56 * 1024;  // dynamic LDS for g (plus ~0.5 KB static)
+constexpr size_t kGenLdsBytes = 150 * 1024;  // dynamic LDS for g (gfx950: 160 KB per CU)
 
 template <bool GLDS, bool DERIV>
 __global__ void __launch_bounds__(256, 2) eri_generic_kernel(EriArgs a, GenShape S,
