@@ -3,10 +3,10 @@
 // (high angular momentum: (ff|..), (gg|gg), the s..g basis workload).  One
 // shell tuple per 256-lane workgroup, grid-stride over tuples.  The g tensor
 // lives in LDS when it fits in 150 KB, else in a per-workgroup global scratch
-// slice; the Cartesian accumulators live in registers (16 elements per lane,
-// 8 for nabla_i; larger blocks in chunks); the chunk results and the
-// cart->sph buffers are in the slice.  Arithmetic is the same as eri_class_kernel (libcint g2e.c:272-690,
-// 4425-4620).
+// slice; the Cartesian accumulators live in registers (8 elements per lane,
+// 4 for nabla_i; larger blocks in chunks); the chunk results and the
+// cart->sph buffers are in the slice.  Arithmetic is the same as
+// eri_class_kernel (libcint g2e.c:272-690, 4425-4620).
 #include "eri_device.h"
 
 namespace cx {
