@@ -260,7 +260,10 @@ def secondary_configs():
                     "per_s": plan.ntuples / (m / 1e3), "unit": unit,
                     "tflops": flops / (m / 1e3) / 1e12,
                     "fp32_tuples": plan.ntuples_fp32,
-                    "max_abs_err_vs_oracle": float(np.abs(got - ref).max())}
+                    "max_abs_err_vs_oracle": float(np.abs(got - ref).max()),
+                    # relative to each element's scale, |ref| floored at 1
+                    "max_err_over_max_1_ref": float(
+                        (np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)).max())}
         plan.close()
 
     log("secondary: building the aux-basis lattice plans")
