@@ -249,6 +249,46 @@ __device__ __forceinline__ void c2s_pass(double* __restrict__ dst, const double*
   }
 }
 
+// The same pass as small dense products on the matrix cores, for lane
+// groups of whole wavefronts (t, T multiples of 64 apart): dst = T_l src with
+// T_l the nsph x ncart table, as D(16 x 16) += A(16 x 4) B(4 x 16) steps of
+// v_mfma_f64_16x16x4f64 over 16-column tiles of the flattened other axes.
+// Operand layout (checked on gfx950): lane L holds A[L % 16][L / 16] and
+// B[L / 16][L % 16]; element j of its result is D[L / 16 + 4 j][L % 16].
+typedef double c2s_v4d __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void c2s_pass_mfma(double* __restrict__ dst,
+                                              const double* __restrict__ src, const int* dims,
+                                              int ax, int lax, const double* __restrict__ c2s,
+                                              int t, int T) {
+  int inner = 1, outer = 1;
+  for (int b = 0; b < ax; ++b) inner *= dims[b];
+  for (int b = ax + 1; b < 4; ++b) outer *= dims[b];
+  const int nc = dims[ax], ns = nsph(lax);
+  const int ncol = outer * inner;
+  const double* tab = c2s + (size_t)lax * kC2sM * kC2sC;
+  const int lane = t & 63, wave = t >> 6, nwave = T >> 6;
+  const int col = lane & 15, kq = lane >> 4;
+  for (int n0 = wave * 16; n0 < ncol; n0 += nwave * 16) {  // uniform per wavefront
+    const int n = n0 + col;
+    const int ou = n / inner, in = n - ou * inner;
+    c2s_v4d d = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < nc; k0 += 4) {
+      const int k = k0 + kq;
+      const double av = (col < ns && k < nc) ? tab[col * kC2sC + k] : 0.0;
+      double bv = 0.0;
+      if (n < ncol && k < nc) bv = src[((size_t)ou * nc + k) * inner + in];
+      d = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, d, 0, 0, 0);
+    }
+    if (n < ncol) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int m = kq + 4 * j;
+        if (m < ns) dst[((size_t)ou * ns + m) * inner + in] = d[j];
+      }
+    }
+  }
+}
+
 // Synchronisation of the T lanes of one tuple.  A tuple of at most 64 lanes
 // lies inside one wavefront (T divides 64), whose LDS operations complete in
 // issue order, so a wavefront barrier with LDS fences is enough; wider tuples
@@ -972,7 +1012,10 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
 #pragma unroll
     for (int ax = 0; ax < 4; ++ax) {
       if (ls[ax] < 2 || S::CART) continue;  // s and p: identity
-      c2s_pass(dst, src, dims, ax, ls[ax], a.c2s, t, T);
+      if constexpr (T % 64 == 0)
+        c2s_pass_mfma(dst, src, dims, ax, ls[ax], a.c2s, t, T);
+      else
+        c2s_pass(dst, src, dims, ax, ls[ax], a.c2s, t, T);
       dims[ax] = nsph(ls[ax]);
       double* tmp = src;
       src = dst;

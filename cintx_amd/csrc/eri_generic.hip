@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(256, 2) eri_generic_kernel(EriArgs a, GenShape
       double* dst = b1;
       for (int ax = 0; ax < 4; ++ax) {
         if (S.l[ax] < 2 || a.cart) continue;
-        c2s_pass(dst, src, dims, ax, S.l[ax], a.c2s, t, T);
+        c2s_pass_mfma(dst, src, dims, ax, S.l[ax], a.c2s, t, T);
         dims[ax] = nsph(S.l[ax]);
         double* tmp = src;
         src = dst;
