@@ -114,9 +114,17 @@ def main():
     dens = dens + dens.T
     p64.jk(dens)  # warm-up
     t0 = time.perf_counter()
-    p64.jk(dens)
+    J64, K64 = p64.jk(dens)
     jk_ms = (time.perf_counter() - t0) * 1e3
     log(f"J/K build (plan + digest, host timed): {jk_ms:.1f} ms")
+    # the mixed plan's J/K against the FP64 plan's (whose ERIs match the
+    # oracle to the sampled error above)
+    Jm, Km = pmx.jk(dens)
+    jk_err = {"J_max_abs_err_mixed_vs_fp64": float(np.abs(Jm - J64).max()),
+              "K_max_abs_err_mixed_vs_fp64": float(np.abs(Km - K64).max()),
+              "J_max_abs": float(np.abs(J64).max()), "K_max_abs": float(np.abs(K64).max())}
+    log(f"J/K mixed vs FP64: J err {jk_err['J_max_abs_err_mixed_vs_fp64']:.3e}, "
+        f"K err {jk_err['K_max_abs_err_mixed_vs_fp64']:.3e}")
 
     prim64, roots64, flops64 = p64.work()
     log(f"fp64 work: {prim64:.3e} primitive quartets, {flops64:.3e} FLOP")
@@ -193,7 +201,7 @@ def main():
                  "wall_ms_per_step": wall64, "gflops": achieved_step * 1e3,
                  "prim_quartets": prim64, "root_evals": roots64},
         "mixed_over_fp64": value / fp64_rate,
-        "jk": {"ms": jk_ms, "nbf": nao,
+        "jk": {"ms": jk_ms, "nbf": nao, **jk_err,
                "note": "J and K from the FP64 plan's unique quartets and a random symmetric "
                        "density; plan execution + digest kernels + 64 MB copies, host-timed"},
         "fp64_max_abs_err_vs_oracle": err64,
