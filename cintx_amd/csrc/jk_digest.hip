@@ -41,7 +41,7 @@ __device__ __forceinline__ void jk_digest_one(
     const double* __restrict__ eri, int64_t out_base, int64_t block, const int2* __restrict__ q,
     int64_t t, int ln, int nl, const int2* __restrict__ pair_sh, const int* __restrict__ ao_off,
     const int* __restrict__ shl_l, const double* __restrict__ D, double* __restrict__ J,
-    double* __restrict__ K, int nao) {
+    double* __restrict__ K, int nao, bool skip_bra_j = false) {
   const int2 pq = q[t];
   const int sh[4] = {pair_sh[pq.x].x, pair_sh[pq.x].y, pair_sh[pq.y].x, pair_sh[pq.y].y};
   const int ns[4] = {2 * shl_l[sh[0]] + 1, 2 * shl_l[sh[1]] + 1, 2 * shl_l[sh[2]] + 1,
@@ -55,7 +55,7 @@ __device__ __forceinline__ void jk_digest_one(
   const double w = 1.0 / stab;
   const double* blk = eri + out_base + t * block;
   const int st[4] = {1, ns[0], ns[0] * ns[1], ns[0] * ns[1] * ns[2]};
-  jk_reduce(blk, ns, st, off, 0, 1, 2, 3, 2 * w, D, J, nao, ln, nl);
+  if (!skip_bra_j) jk_reduce(blk, ns, st, off, 0, 1, 2, 3, 2 * w, D, J, nao, ln, nl);
   jk_reduce(blk, ns, st, off, 2, 3, 0, 1, 2 * w, D, J, nao, ln, nl);
   jk_reduce(blk, ns, st, off, 0, 2, 1, 3, w, D, K, nao, ln, nl);
   jk_reduce(blk, ns, st, off, 1, 2, 0, 3, w, D, K, nao, ln, nl);
@@ -81,7 +81,48 @@ __global__ void __launch_bounds__(256) jk_digest_lane_kernel(
     const int* __restrict__ shl_l, const double* __restrict__ D, double* __restrict__ J,
     double* __restrict__ K, int nao) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nq) jk_digest_one(eri, out_base, block, q, t, 0, 1, pair_sh, ao_off, shl_l, D, J, K, nao);
+  const bool live = t < nq;
+  // consecutive quartets of a bin mostly share their bra pair; when every
+  // live lane of the wavefront does, the J[I,J] sums are added across the
+  // wavefront first and issued as one atomic per element instead of up to
+  // 64 atomics on the same address (lane 0 is live whenever any lane is)
+  const int bra = live ? q[t].x : -1;
+  const int bra0 = __shfl(bra, 0);
+  const bool shared = __all(!live || bra == bra0) && bra0 >= 0;
+  if (live)
+    jk_digest_one(eri, out_base, block, q, t, 0, 1, pair_sh, ao_off, shl_l, D, J, K, nao, shared);
+  if (shared) {
+    const int si = pair_sh[bra0].x, sj = pair_sh[bra0].y;
+    const int ni = 2 * shl_l[si] + 1, nj = 2 * shl_l[sj] + 1;
+    const int oi = ao_off[si], oj = ao_off[sj];
+    int nk = 0, nl = 0, ok = 0, ol = 0;
+    double wgt = 0.0;
+    const double* blk = eri;
+    if (live) {
+      const int2 pq = q[t];
+      const int sk = pair_sh[pq.y].x, sl = pair_sh[pq.y].y;
+      nk = 2 * shl_l[sk] + 1;
+      nl = 2 * shl_l[sl] + 1;
+      ok = ao_off[sk];
+      ol = ao_off[sl];
+      const bool braket = (si == sk && sj == sl) || (si == sl && sj == sk);
+      const int stab = (si == sj ? 2 : 1) * (sk == sl ? 2 : 1) * (braket ? 2 : 1);
+      wgt = 2.0 / stab;
+      blk = eri + out_base + t * block;
+    }
+    const int st2 = ni * nj, st3 = ni * nj * nk;
+    for (int o = 0; o < ni * nj; ++o) {  // uniform over the wavefront
+      double sum = 0.0;
+      for (int ml = 0; ml < nl; ++ml)
+        for (int mk = 0; mk < nk; ++mk)
+          sum += blk[o + mk * st2 + ml * st3] * D[(size_t)(ok + mk) * nao + ol + ml];
+      sum *= wgt;
+#pragma unroll
+      for (int d = 32; d >= 1; d >>= 1) sum += __shfl_xor(sum, d);
+      if ((threadIdx.x & 63) == 0 && sum != 0.0)
+        atomicAdd(&J[(size_t)(oi + o % ni) * nao + oj + o / ni], sum);
+    }
+  }
 }
 
 hipError_t launch_jk_digest(const double* eri, int64_t out_base, int64_t block, const int2* q,
