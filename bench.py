@@ -236,8 +236,8 @@ def main():
 def secondary_configs():
     """Throughput of the other families on their own synthetic inputs (FP64,
     autotuned, graph replay): int3c2e_sph and int3c2e_sph mixed on the 64-atom
-    lattice with the 6s5p4d3f1g aux basis (every 8th orbital pair i >= j with
-    every aux shell), int2e_sph (unique quartets, Schwarz 1e-12, bra stride 64)
+    lattice with the 6s5p4d3f1g aux basis (every orbital pair i >= j with
+    every aux shell, 3.97e7 triples), int2e_sph (unique quartets, Schwarz 1e-12, bra stride 64)
     and int2e_ip1_sph (every 8th of those quartets) on the 32-atom s..g
     basis (runtime-class kernel for f and g classes).  Max abs error against
     the oracle on a 300-tuple sample."""
@@ -276,9 +276,12 @@ def secondary_configs():
 
     log("secondary: building the aux-basis lattice plans")
     m4 = molecule.config(4)
-    pairs = [(i, j) for i in range(m4.aux_start) for j in range(i + 1)][::8]
-    trip = np.array([(i, j, k, 0) for i, j in pairs for k in range(m4.aux_start, m4.nbas)],
-                    dtype=np.int32)
+    ii, jj = np.tril_indices(m4.aux_start)
+    aux = np.arange(m4.aux_start, m4.nbas, dtype=np.int32)
+    trip = np.zeros((len(ii) * len(aux), 4), dtype=np.int32)
+    trip[:, 0] = np.repeat(ii, len(aux))
+    trip[:, 1] = np.repeat(jj, len(aux))
+    trip[:, 2] = np.tile(aux, len(ii))
     run("int3c2e_sph_aux_lattice", capi.Plan("int3c2e_sph", m4.atm, m4.bas, m4.env, trip),
         "int3c2e_sph", m4, "triples/s")
     # an explicit list has no Schwarz bounds, so the mixed plan runs every
