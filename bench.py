@@ -51,11 +51,18 @@ def main():
                     help="skip the int3c2e (aux-basis lattice) and int2e/ip1 (s..g basis) lines")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the class kernels directly instead of replaying a hipGraph")
+    ap.add_argument("--secondary-only", action="store_true",
+                    help="run only the int3c2e and s..g-basis lines and print them as JSON "
+                         "(for profiling those kernels; not the benchmark line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the integral work shards without exchange; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.secondary_only:
+        print(json.dumps(secondary_configs()), flush=True)
+        return
 
     from cintx_amd import capi, molecule
     mol = molecule.config(3)
