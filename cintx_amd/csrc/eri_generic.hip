@@ -4,7 +4,7 @@
 // shell tuple per 256-lane workgroup, grid-stride over tuples.  The g tensor
 // lives in LDS when it fits in 150 KB, else in a per-workgroup global scratch
 // slice; the Cartesian accumulators live in registers (8 elements per lane,
-// 4 for nabla_i; larger blocks in chunks); the chunk results and the
+// larger blocks in chunks); the chunk results and the
 // cart->sph buffers are in the slice.  Arithmetic is the same as
 // eri_class_kernel (libcint g2e.c:272-690, 4425-4620).
 #include "eri_device.h"
@@ -123,9 +123,9 @@ __device__ void generic_ket(double* __restrict__ g, const GenShape& S, int axis,
 }
 
 // GLDS: the g tensor in (dynamic) LDS instead of the global slice.
-// Every lane keeps ACC elements of the current element chunk (t + 256 k;
-// kGenAcc, halved for nabla_i's three components) and their packed g
-// positions in registers.  A block larger than one chunk (256 ACC Cartesian
+// Every lane keeps ACC elements of the current element chunk (t + 256 k)
+// and their g offsets in registers (two workgroups per CU; one for nabla_i,
+// whose three components per element need the larger register budget).  A block larger than one chunk (256 ACC Cartesian
 // elements: (ff|ff), (gg|..)) is done chunk by chunk,
 // each chunk a full pass over the primitive tuples with the roots and chains
 // recomputed, so that the accumulators stay in registers instead of taking a
@@ -134,10 +134,10 @@ constexpr int kGenAcc = 8;
 constexpr size_t kGenLdsBytes = 150 * 1024;  // dynamic LDS for g (gfx950: 160 KB per CU)
 
 template <bool GLDS, bool DERIV>
-__global__ void __launch_bounds__(256, 2) eri_generic_kernel(EriArgs a, GenShape S,
+__global__ void __launch_bounds__(256, (DERIV ? 1 : 2)) eri_generic_kernel(EriArgs a, GenShape S,
                                                           int64_t per_block) {
   constexpr int NC = DERIV ? 3 : 1;
-  constexpr int ACC = DERIV ? kGenAcc / 2 : kGenAcc;  // elements per lane
+  constexpr int ACC = kGenAcc;  // elements per lane
   constexpr int CHUNK = 256 * ACC;
   extern __shared__ double gl[];
   const int t = threadIdx.x;
