@@ -125,11 +125,12 @@ __device__ void generic_ket(double* __restrict__ g, const GenShape& S, int axis,
 // GLDS: the g tensor in (dynamic) LDS instead of the global slice.
 // Every lane keeps ACC elements of the current element chunk (t + 256 k)
 // and their g offsets in registers (two workgroups per CU; one for nabla_i,
-// whose three components per element need the larger register budget).  A block larger than one chunk (256 ACC Cartesian
-// elements: (ff|ff), (gg|..)) is done chunk by chunk,
-// each chunk a full pass over the primitive tuples with the roots and chains
-// recomputed, so that the accumulators stay in registers instead of taking a
-// global read-modify-write per element and primitive tuple.
+// whose three components per element need the larger register budget).  A
+// block larger than one chunk (256 ACC Cartesian elements: (ff|ff), (gg|..))
+// is done chunk by chunk, each chunk a full pass over the primitive tuples
+// with the roots and chains recomputed, so that the accumulators stay in
+// registers instead of taking a global read-modify-write per element and
+// primitive tuple.
 constexpr int kGenAcc = 8;
 constexpr size_t kGenLdsBytes = 150 * 1024;  // dynamic LDS for g (gfx950: 160 KB per CU)
 
