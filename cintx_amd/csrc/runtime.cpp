@@ -197,6 +197,7 @@ static void build_c2s_table(std::vector<double>& tab) {
 struct DevCtx {
   int device = -1;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // J/K digestion beside the class kernels
   RysTable rys{};
   double* d_rys = nullptr;
   double* d_large = nullptr;
@@ -219,6 +220,7 @@ static DevCtx& ctx() {
   DevCtx* c = new DevCtx();
   HIPCHK(hipGetDevice(&c->device));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
   std::vector<double> coef, large;
   rys_table_build(coef, large, c->rys.off);
   HIPCHK(hipMalloc(&c->d_rys, coef.size() * sizeof(double)));
@@ -1966,15 +1968,35 @@ int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
                           C.stream));
     // D is taken as symmetric: (D + D^T) / 2 (the digest relies on it)
     HIPCHK(launch_jk_symmetrize(dD.p, nao, 0.5, C.stream));
-    plan_launch_all(plan, plan->d_out, false);
-    for (const Bin& b : plan->bins)
+    // the class kernels run on the plan stream, and each bin's digest on the
+    // side stream as soon as that bin's block is written, so the
+    // atomic-bound digestion overlaps the compute-bound kernels of the next
+    // bins (disjoint output regions; only the side stream touches J and K)
+    const size_t nb = plan->bins.size();
+    std::vector<hipEvent_t> done(nb, nullptr);
+    struct EvGuard {
+      std::vector<hipEvent_t>& e;
+      ~EvGuard() {
+        for (hipEvent_t v : e)
+          if (v) (void)hipEventDestroy(v);
+      }
+    } evguard{done};
+    for (size_t i = 0; i < nb; ++i)
+      HIPCHK(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
+    for (size_t i = 0; i < nb; ++i) {
+      plan_launch_bin(plan, i, plan->d_out, false);
+      HIPCHK(hipEventRecord(done[i], C.stream));
+      const Bin& b = plan->bins[i];
+      HIPCHK(hipStreamWaitEvent(C.side, done[i], 0));
       HIPCHK(launch_jk_digest(plan->d_out, b.out_base, b.block, b.d_q, (int64_t)b.q.size(), dsh.p,
-                              dao.p, dl.p, dD.p, dJ.p, dK.p, nao, C.stream));
+                              dao.p, dl.p, dD.p, dJ.p, dK.p, nao, C.side));
+    }
     // the digest accumulates one triangle's worth: J = A + A^T, K = B + B^T
-    HIPCHK(launch_jk_symmetrize(dJ.p, nao, 1.0, C.stream));
-    HIPCHK(launch_jk_symmetrize(dK.p, nao, 1.0, C.stream));
-    HIPCHK(hipMemcpyAsync(J, dJ.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
-    HIPCHK(hipMemcpyAsync(K, dK.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
+    HIPCHK(launch_jk_symmetrize(dJ.p, nao, 1.0, C.side));
+    HIPCHK(launch_jk_symmetrize(dK.p, nao, 1.0, C.side));
+    HIPCHK(hipMemcpyAsync(J, dJ.p, nn * 8, hipMemcpyDeviceToHost, C.side));
+    HIPCHK(hipMemcpyAsync(K, dK.p, nn * 8, hipMemcpyDeviceToHost, C.side));
+    HIPCHK(hipStreamSynchronize(C.side));
     HIPCHK(hipStreamSynchronize(C.stream));
   });
 }
