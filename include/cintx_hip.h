@@ -240,7 +240,9 @@ int cintx_rys_eval_device(int n, const double *x, int64_t nx, double *u, double 
 /* Coulomb and exchange matrices from an int2e_sph plan whose tuples are
  * unique 8-fold shell quartets (each unordered quartet once) of segmented
  * shells: J_pq = sum_rs (pq|rs) D_rs, K_pr = sum_qs (pq|rs) D_qs for a
- * symmetric density D (nao x nao, row-major, host memory). */
+ * symmetric density D (nao x nao, row-major, host memory; a non-symmetric D
+ * is replaced by (D + D^T) / 2).  No counterpart in the reference: the
+ * Fock-like consumer of the unique-quartet workload. */
 int cintx_plan_jk(cintx_plan *plan, const double *D, double *J, double *K);
 
 /* Learned Rys initializer (no counterpart in the reference): a 2x64 tanh
