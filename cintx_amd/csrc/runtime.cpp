@@ -198,6 +198,7 @@ struct DevCtx {
   int device = -1;
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // J/K digestion beside the class kernels
+  hipStream_t branch = nullptr;  // second chain of a captured plan graph
   RysTable rys{};
   double* d_rys = nullptr;
   double* d_large = nullptr;
@@ -221,6 +222,7 @@ static DevCtx& ctx() {
   HIPCHK(hipGetDevice(&c->device));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&c->branch, hipStreamNonBlocking));
   std::vector<double> coef, large;
   rys_table_build(coef, large, c->rys.off);
   HIPCHK(hipMalloc(&c->d_rys, coef.size() * sizeof(double)));
@@ -764,8 +766,10 @@ static void plan_check_kernels(cintx_plan* P) {
   }
 }
 
-static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count) {
+static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count,
+                            hipStream_t st = nullptr) {
   DevCtx& C = ctx();
+  if (!st) st = C.stream;
   const FamilyInfo& f = kFam[P->family];
   const double common_base = (M_PI * M_PI * M_PI) * 2.0 / std::sqrt(M_PI);
   Bin& b = P->bins[i];
@@ -814,9 +818,9 @@ static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count) {
     if (b.generic) {
       a.scratch = P->d_scratch;
       int grid = (int)std::min<int64_t>(a.nq, kGenericGrid);
-      e = launch_generic(a, b.l, f.deriv, grid, C.stream);
+      e = launch_generic(a, b.l, f.deriv, grid, st);
     } else {
-      e = k.fn(a, C.stream);
+      e = k.fn(a, st);
     }
     if (e != hipSuccess)
       fail(CINTX_STATUS_EXECUTION_FAILED, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -1319,16 +1323,42 @@ int cintx_plan_execute_graph(cintx_plan* plan, double* out_dev) {
     double* out = out_dev ? out_dev : plan->d_out;
     if (plan->graph_exec && plan->graph_out != out) plan_drop_graph(plan);
     if (!plan->graph_exec) {
-      HIPCHK(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+      // two chains of bins (fork/join on the plan stream), so that one
+      // bin's tail overlaps the other chain's kernel; the runtime-class bins
+      // share the workspace and stay on one chain
+      hipEvent_t fork = nullptr, join = nullptr;
+      HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+      if (hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipEventDestroy(fork);
+        fail(CINTX_STATUS_EXECUTION_FAILED, "hipEventCreate failed");
+      }
+      if (hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipEventDestroy(fork);
+        (void)hipEventDestroy(join);
+        fail(CINTX_STATUS_EXECUTION_FAILED, "hipStreamBeginCapture failed");
+      }
       try {
-        plan_launch_all(plan, out, false);
+        HIPCHK(hipEventRecord(fork, C.stream));
+        HIPCHK(hipStreamWaitEvent(C.branch, fork, 0));
+        size_t n_other = 0;
+        for (size_t i = 0; i < plan->bins.size(); ++i) {
+          const bool side = !plan->bins[i].generic && (n_other++ % 2 == 1);
+          plan_launch_bin(plan, i, out, false, side ? C.branch : C.stream);
+        }
+        HIPCHK(hipEventRecord(join, C.branch));
+        HIPCHK(hipStreamWaitEvent(C.stream, join, 0));
       } catch (...) {
-        hipGraph_t g;
+        hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(C.stream, &g);
         if (g) (void)hipGraphDestroy(g);
+        (void)hipEventDestroy(fork);
+        (void)hipEventDestroy(join);
         throw;
       }
-      HIPCHK(hipStreamEndCapture(C.stream, &plan->graph));
+      const hipError_t ec = hipStreamEndCapture(C.stream, &plan->graph);
+      (void)hipEventDestroy(fork);
+      (void)hipEventDestroy(join);
+      HIPCHK(ec);
       HIPCHK(hipGraphInstantiate(&plan->graph_exec, plan->graph, nullptr, nullptr, 0));
       plan->graph_out = out;
     }
