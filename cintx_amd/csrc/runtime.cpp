@@ -194,11 +194,13 @@ static void build_c2s_table(std::vector<double>& tab) {
 // ------------------------------------------------------------------
 // Device context (one per process; current device at first use).
 // ------------------------------------------------------------------
+constexpr int kChains = 4;  // parallel chains of a captured plan graph
+
 struct DevCtx {
   int device = -1;
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // J/K digestion beside the class kernels
-  hipStream_t branch = nullptr;  // second chain of a captured plan graph
+  hipStream_t branch[kChains - 1] = {};  // further chains of a captured plan graph
   RysTable rys{};
   double* d_rys = nullptr;
   double* d_large = nullptr;
@@ -222,7 +224,8 @@ static DevCtx& ctx() {
   HIPCHK(hipGetDevice(&c->device));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&c->branch, hipStreamNonBlocking));
+  for (int b = 0; b < kChains - 1; ++b)
+    HIPCHK(hipStreamCreateWithFlags(&c->branch[b], hipStreamNonBlocking));
   std::vector<double> coef, large;
   rys_table_build(coef, large, c->rys.off);
   HIPCHK(hipMalloc(&c->d_rys, coef.size() * sizeof(double)));
@@ -1323,41 +1326,45 @@ int cintx_plan_execute_graph(cintx_plan* plan, double* out_dev) {
     double* out = out_dev ? out_dev : plan->d_out;
     if (plan->graph_exec && plan->graph_out != out) plan_drop_graph(plan);
     if (!plan->graph_exec) {
-      // two chains of bins (fork/join on the plan stream), so that one
-      // bin's tail overlaps the other chain's kernel; the runtime-class bins
-      // share the workspace and stay on one chain
-      hipEvent_t fork = nullptr, join = nullptr;
-      HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-      if (hipEventCreateWithFlags(&join, hipEventDisableTiming) != hipSuccess) {
-        (void)hipEventDestroy(fork);
-        fail(CINTX_STATUS_EXECUTION_FAILED, "hipEventCreate failed");
-      }
+      // kChains chains of bins (fork/join on the plan stream), so that
+      // kernels of different classes share the CUs (the one-lane kernels
+      // leave registers or LDS idle that another class uses) and one bin's
+      // tail overlaps the others; the runtime-class bins share the workspace
+      // and stay on the plan stream's chain
+      hipEvent_t ev[kChains] = {};  // ev[0]: fork, ev[1..]: joins
+      for (int c = 0; c < kChains; ++c)
+        if (hipEventCreateWithFlags(&ev[c], hipEventDisableTiming) != hipSuccess) {
+          for (int d = 0; d < c; ++d) (void)hipEventDestroy(ev[d]);
+          fail(CINTX_STATUS_EXECUTION_FAILED, "hipEventCreate failed");
+        }
+      auto drop_events = [&] {
+        for (int c = 0; c < kChains; ++c) (void)hipEventDestroy(ev[c]);
+      };
       if (hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        (void)hipEventDestroy(fork);
-        (void)hipEventDestroy(join);
+        drop_events();
         fail(CINTX_STATUS_EXECUTION_FAILED, "hipStreamBeginCapture failed");
       }
       try {
-        HIPCHK(hipEventRecord(fork, C.stream));
-        HIPCHK(hipStreamWaitEvent(C.branch, fork, 0));
+        HIPCHK(hipEventRecord(ev[0], C.stream));
+        for (int c = 0; c < kChains - 1; ++c) HIPCHK(hipStreamWaitEvent(C.branch[c], ev[0], 0));
         size_t n_other = 0;
         for (size_t i = 0; i < plan->bins.size(); ++i) {
-          const bool side = !plan->bins[i].generic && (n_other++ % 2 == 1);
-          plan_launch_bin(plan, i, out, false, side ? C.branch : C.stream);
+          const int chain = plan->bins[i].generic ? 0 : (int)(n_other++ % kChains);
+          plan_launch_bin(plan, i, out, false, chain ? C.branch[chain - 1] : C.stream);
         }
-        HIPCHK(hipEventRecord(join, C.branch));
-        HIPCHK(hipStreamWaitEvent(C.stream, join, 0));
+        for (int c = 0; c < kChains - 1; ++c) {
+          HIPCHK(hipEventRecord(ev[c + 1], C.branch[c]));
+          HIPCHK(hipStreamWaitEvent(C.stream, ev[c + 1], 0));
+        }
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(C.stream, &g);
         if (g) (void)hipGraphDestroy(g);
-        (void)hipEventDestroy(fork);
-        (void)hipEventDestroy(join);
+        drop_events();
         throw;
       }
       const hipError_t ec = hipStreamEndCapture(C.stream, &plan->graph);
-      (void)hipEventDestroy(fork);
-      (void)hipEventDestroy(join);
+      drop_events();
       HIPCHK(ec);
       HIPCHK(hipGraphInstantiate(&plan->graph_exec, plan->graph, nullptr, nullptr, 0));
       plan->graph_out = out;
