@@ -194,7 +194,7 @@ static void build_c2s_table(std::vector<double>& tab) {
 // ------------------------------------------------------------------
 // Device context (one per process; current device at first use).
 // ------------------------------------------------------------------
-constexpr int kChains = 4;  // parallel chains of a captured plan graph
+constexpr int kChains = 8;  // parallel chains of a captured plan graph
 
 struct DevCtx {
   int device = -1;
