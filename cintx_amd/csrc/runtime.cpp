@@ -199,7 +199,6 @@ constexpr int kChains = 8;  // parallel chains of a captured plan graph
 struct DevCtx {
   int device = -1;
   hipStream_t stream = nullptr;
-  hipStream_t side = nullptr;  // J/K digestion beside the class kernels
   hipStream_t branch[kChains - 1] = {};  // further chains of a captured plan graph
   RysTable rys{};
   double* d_rys = nullptr;
@@ -223,7 +222,6 @@ static DevCtx& ctx() {
   DevCtx* c = new DevCtx();
   HIPCHK(hipGetDevice(&c->device));
   HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
   for (int b = 0; b < kChains - 1; ++b)
     HIPCHK(hipStreamCreateWithFlags(&c->branch[b], hipStreamNonBlocking));
   std::vector<double> coef, large;
@@ -2005,35 +2003,46 @@ int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
                           C.stream));
     // D is taken as symmetric: (D + D^T) / 2 (the digest relies on it)
     HIPCHK(launch_jk_symmetrize(dD.p, nao, 0.5, C.stream));
-    // the class kernels run on the plan stream, and each bin's digest on the
-    // side stream as soon as that bin's block is written, so the
-    // atomic-bound digestion overlaps the compute-bound kernels of the next
-    // bins (disjoint output regions; only the side stream touches J and K)
-    const size_t nb = plan->bins.size();
-    std::vector<hipEvent_t> done(nb, nullptr);
+    // the bins spread over the plan graph's chains of streams, each bin's
+    // digest right after its class kernel on the same stream: kernels of
+    // different classes and the atomic-bound digests overlap (disjoint output
+    // regions; J and K take atomics only).  Runtime-class bins share the
+    // workspace and stay on the plan stream.
+    hipEvent_t ev[kChains] = {};  // ev[0]: fork, ev[1..]: joins
+    // (declared after the device buffers, so on an early exit it waits for
+    // every chain before those are freed)
     struct EvGuard {
-      std::vector<hipEvent_t>& e;
+      hipEvent_t* e;
+      DevCtx& c;
       ~EvGuard() {
-        for (hipEvent_t v : e)
-          if (v) (void)hipEventDestroy(v);
+        for (int b = 0; b < kChains - 1; ++b) (void)hipStreamSynchronize(c.branch[b]);
+        (void)hipStreamSynchronize(c.stream);
+        for (int b = 0; b < kChains; ++b)
+          if (e[b]) (void)hipEventDestroy(e[b]);
       }
-    } evguard{done};
-    for (size_t i = 0; i < nb; ++i)
-      HIPCHK(hipEventCreateWithFlags(&done[i], hipEventDisableTiming));
-    for (size_t i = 0; i < nb; ++i) {
-      plan_launch_bin(plan, i, plan->d_out, false);
-      HIPCHK(hipEventRecord(done[i], C.stream));
+    } evguard{ev, C};
+    for (int c = 0; c < kChains; ++c)
+      HIPCHK(hipEventCreateWithFlags(&ev[c], hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ev[0], C.stream));
+    for (int c = 0; c < kChains - 1; ++c) HIPCHK(hipStreamWaitEvent(C.branch[c], ev[0], 0));
+    size_t n_other = 0;
+    for (size_t i = 0; i < plan->bins.size(); ++i) {
       const Bin& b = plan->bins[i];
-      HIPCHK(hipStreamWaitEvent(C.side, done[i], 0));
+      const int chain = b.generic ? 0 : (int)(n_other++ % kChains);
+      hipStream_t st = chain ? C.branch[chain - 1] : C.stream;
+      plan_launch_bin(plan, i, plan->d_out, false, st);
       HIPCHK(launch_jk_digest(plan->d_out, b.out_base, b.block, b.d_q, (int64_t)b.q.size(), dsh.p,
-                              dao.p, dl.p, dD.p, dJ.p, dK.p, nao, C.side));
+                              dao.p, dl.p, dD.p, dJ.p, dK.p, nao, st));
+    }
+    for (int c = 0; c < kChains - 1; ++c) {
+      HIPCHK(hipEventRecord(ev[c + 1], C.branch[c]));
+      HIPCHK(hipStreamWaitEvent(C.stream, ev[c + 1], 0));
     }
     // the digest accumulates one triangle's worth: J = A + A^T, K = B + B^T
-    HIPCHK(launch_jk_symmetrize(dJ.p, nao, 1.0, C.side));
-    HIPCHK(launch_jk_symmetrize(dK.p, nao, 1.0, C.side));
-    HIPCHK(hipMemcpyAsync(J, dJ.p, nn * 8, hipMemcpyDeviceToHost, C.side));
-    HIPCHK(hipMemcpyAsync(K, dK.p, nn * 8, hipMemcpyDeviceToHost, C.side));
-    HIPCHK(hipStreamSynchronize(C.side));
+    HIPCHK(launch_jk_symmetrize(dJ.p, nao, 1.0, C.stream));
+    HIPCHK(launch_jk_symmetrize(dK.p, nao, 1.0, C.stream));
+    HIPCHK(hipMemcpyAsync(J, dJ.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
+    HIPCHK(hipMemcpyAsync(K, dK.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
     HIPCHK(hipStreamSynchronize(C.stream));
   });
 }
