@@ -466,7 +466,9 @@ struct cintx_plan {
   char* blob = nullptr;
   size_t blob_bytes = 0;
   double* d_out = nullptr;
-  double* d_scratch = nullptr;  // generic-kernel workspace (shared by its bins)
+  double* d_scratch = nullptr;  // generic-kernel workspace: one slice per chain
+  int64_t scratch_stride = 0;   // doubles per chain's slice
+  int scratch_chains = 1;       // chains that may run runtime-class bins
   unsigned long long* d_counters = nullptr;  // 2 per bin
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0;
@@ -662,7 +664,14 @@ static void plan_upload(cintx_plan* P) {
       scratch = std::max(scratch, grid * generic_scratch_per_block(b.l, kFam[P->family].deriv));
     }
   size_t o_out = take((size_t)P->total * 8);
-  size_t o_scr = take((size_t)scratch * 8);
+  // one workspace per graph chain (so runtime-class bins of different chains
+  // run concurrently), fewer when that would pass 8 GiB
+  P->scratch_chains = 1;
+  while (P->scratch_chains < kChains &&
+         (size_t)scratch * 8 * (size_t)(2 * P->scratch_chains) <= ((size_t)8 << 30))
+    P->scratch_chains *= 2;
+  P->scratch_stride = scratch;
+  size_t o_scr = take((size_t)scratch * 8 * (size_t)P->scratch_chains);
   size_t host_bytes = o_out;  // everything before the output is uploaded
   P->blob_bytes = cur;
   HIPCHK(hipMalloc(&P->blob, P->blob_bytes));
@@ -768,7 +777,7 @@ static void plan_check_kernels(cintx_plan* P) {
 }
 
 static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count,
-                            hipStream_t st = nullptr) {
+                            hipStream_t st = nullptr, int chain = 0) {
   DevCtx& C = ctx();
   if (!st) st = C.stream;
   const FamilyInfo& f = kFam[P->family];
@@ -817,7 +826,7 @@ static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count,
     a.nq = std::min(chunk, (int64_t)b.q.size() - s);
     hipError_t e;
     if (b.generic) {
-      a.scratch = P->d_scratch;
+      a.scratch = P->d_scratch + (size_t)chain * (size_t)P->scratch_stride;
       int grid = (int)std::min<int64_t>(a.nq, kGenericGrid);
       e = launch_generic(a, b.l, f.deriv, grid, st);
     } else {
@@ -1327,8 +1336,8 @@ int cintx_plan_execute_graph(cintx_plan* plan, double* out_dev) {
       // kChains chains of bins (fork/join on the plan stream), so that
       // kernels of different classes share the CUs (the one-lane kernels
       // leave registers or LDS idle that another class uses) and one bin's
-      // tail overlaps the others; the runtime-class bins share the workspace
-      // and stay on the plan stream's chain
+      // tail overlaps the others; runtime-class bins use their chain's
+      // workspace slice (only the first scratch_chains chains have one)
       hipEvent_t ev[kChains] = {};  // ev[0]: fork, ev[1..]: joins
       for (int c = 0; c < kChains; ++c)
         if (hipEventCreateWithFlags(&ev[c], hipEventDisableTiming) != hipSuccess) {
@@ -1345,10 +1354,11 @@ int cintx_plan_execute_graph(cintx_plan* plan, double* out_dev) {
       try {
         HIPCHK(hipEventRecord(ev[0], C.stream));
         for (int c = 0; c < kChains - 1; ++c) HIPCHK(hipStreamWaitEvent(C.branch[c], ev[0], 0));
-        size_t n_other = 0;
+        size_t n_other = 0, n_gen = 0;
         for (size_t i = 0; i < plan->bins.size(); ++i) {
-          const int chain = plan->bins[i].generic ? 0 : (int)(n_other++ % kChains);
-          plan_launch_bin(plan, i, out, false, chain ? C.branch[chain - 1] : C.stream);
+          const int chain = plan->bins[i].generic ? (int)(n_gen++ % plan->scratch_chains)
+                                                  : (int)(n_other++ % kChains);
+          plan_launch_bin(plan, i, out, false, chain ? C.branch[chain - 1] : C.stream, chain);
         }
         for (int c = 0; c < kChains - 1; ++c) {
           HIPCHK(hipEventRecord(ev[c + 1], C.branch[c]));
@@ -2006,8 +2016,8 @@ int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
     // the bins spread over the plan graph's chains of streams, each bin's
     // digest right after its class kernel on the same stream: kernels of
     // different classes and the atomic-bound digests overlap (disjoint output
-    // regions; J and K take atomics only).  Runtime-class bins share the
-    // workspace and stay on the plan stream.
+    // regions; J and K take atomics only).  Runtime-class bins use their
+    // chain's workspace slice.
     hipEvent_t ev[kChains] = {};  // ev[0]: fork, ev[1..]: joins
     // (declared after the device buffers, so on an early exit it waits for
     // every chain before those are freed)
@@ -2025,12 +2035,13 @@ int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
       HIPCHK(hipEventCreateWithFlags(&ev[c], hipEventDisableTiming));
     HIPCHK(hipEventRecord(ev[0], C.stream));
     for (int c = 0; c < kChains - 1; ++c) HIPCHK(hipStreamWaitEvent(C.branch[c], ev[0], 0));
-    size_t n_other = 0;
+    size_t n_other = 0, n_gen = 0;
     for (size_t i = 0; i < plan->bins.size(); ++i) {
       const Bin& b = plan->bins[i];
-      const int chain = b.generic ? 0 : (int)(n_other++ % kChains);
+      const int chain = b.generic ? (int)(n_gen++ % plan->scratch_chains)
+                                  : (int)(n_other++ % kChains);
       hipStream_t st = chain ? C.branch[chain - 1] : C.stream;
-      plan_launch_bin(plan, i, plan->d_out, false, st);
+      plan_launch_bin(plan, i, plan->d_out, false, st, chain);
       HIPCHK(launch_jk_digest(plan->d_out, b.out_base, b.block, b.d_q, (int64_t)b.q.size(), dsh.p,
                               dao.p, dl.p, dD.p, dJ.p, dK.p, nao, st));
     }
