@@ -242,6 +242,33 @@ def test_graph_replay_equals_eager(oracle):
     p.close()
 
 
+def test_graph_chains_with_runtime_class_bins(oracle):
+    """The captured graph runs the bins as concurrent chains and gives the
+    runtime-class bins of different chains their own workspace slices: many
+    f/g classes replayed several times must equal the one-bin-at-a-time
+    execution bit for bit, and the oracle."""
+    mol = M.config(5)
+    rng = np.random.default_rng(21)
+    hi = [s for s in range(mol.nbas) if mol.bas[s][1] >= 3]
+    tuples = [(int(hi[i]), int(hi[j]), int(rng.integers(0, mol.nbas)), int(rng.integers(0, mol.nbas)))
+              for i, j in rng.integers(0, len(hi), (120, 2))]
+    say(f"plan int2e_sph cfg5 ntuples={len(tuples)} (graph chains)")
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, tuples)
+    assert sum(b["generic"] for b in p.bin_stats()) >= 8
+    p.execute()
+    p.sync()
+    eager = p.output()
+    for rep in range(3):
+        p.execute(graph=True)
+        p.sync()
+        assert np.array_equal(eager, p.output()), rep
+        p.execute()
+        p.sync()
+    ref, offs, sizes = oracle_blocks(oracle, "int2e_sph", tuples, mol)
+    assert_parity(eager, ref)
+    p.close()
+
+
 def test_screened_plan_cfg2(oracle):
     mol = M.config(2)
     say("screened plan cfg2")
