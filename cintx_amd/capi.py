@@ -59,7 +59,7 @@ EXPORTS = [
     "cintx_plan_copy_output", "cintx_plan_copy_tuples", "cintx_plan_execute_graph", "cintx_plan_last_time_ms",
     "cintx_plan_work", "cintx_plan_execute_timed", "cintx_plan_jk", "cintx_plan_bin_stats",
     "cintx_plan_autotune", "cintx_plan_set_variant", "cintx_plan_bin_variants",
-    "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
+    "cintx_plan_fill_output", "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
 ]
 
 
@@ -121,6 +121,7 @@ def lib():
         L.cintx_plan_execute.argtypes = [vp, DP]
         L.cintx_plan_execute_graph.argtypes = [vp, DP]
         L.cintx_plan_sync.argtypes = [vp]
+        L.cintx_plan_fill_output.argtypes = [vp, ctypes.c_double]
         L.cintx_plan_copy_output.argtypes = [vp, DP]
         L.cintx_plan_copy_tuples.argtypes = [vp, I64P, ctypes.c_int64, DP, I64P]
         L.cintx_plan_last_time_ms.argtypes = [vp, DP]
@@ -302,6 +303,10 @@ class Plan:
 
     def sync(self):
         _check(lib().cintx_plan_sync(self.h))
+
+    def fill_output(self, value):
+        """fill the plan's output buffer with `value` (asynchronous)"""
+        _check(lib().cintx_plan_fill_output(self.h, float(value)))
 
     def last_ms(self):
         v = ctypes.c_double()

@@ -5,7 +5,7 @@
 //                 the caller's basis are split into one virtual shell per
 //                 contraction column, which is exact by linearity.
 //   pairs       : one record per shell pair (i, j) that a plan needs.
-//   prim pairs  : structure-of-arrays over all primitive pairs of all pairs,
+//   prim pairs  : one 64-byte record per primitive pair of all pairs,
 //                 contiguous per pair, ordered jp-major/ip-minor like
 //                 CINTset_pairdata (libcint-master/src/optimizer.c:317-341).
 //   quartets    : (bra pair, ket pair) int32 tuples, binned by class.
@@ -40,17 +40,12 @@ struct RysTable {
 
 // Per-launch arguments of a class kernel.
 struct EriArgs {
-  // primitive-pair SoA
-  const double* aij;
-  const double* px;
-  const double* py;
-  const double* pz;
-  const double* kfac;   // exp(-a_i a_j/a_ij |R_ij|^2) * c_i * c_j
-  const double* ai;     // exponent of the first shell (nabla_i)
-  const double* cce;    // screening key without the l-dependent term
-  const double* hinv;   // 0.5 / a_ij
-  // the same per primitive pair as one 64-byte record (a kernel lane reads
-  // one cache line per primitive pair): aij, Px, Py, Pz, kfac, hinv, cce, ai
+  // per primitive pair one 64-byte record (a lane reads one cache line per
+  // primitive pair): a_ij, w_i = a_i/a_ij, w_j = a_j/a_ij, 0,
+  // kfac = exp(-a_i a_j/a_ij |R_ij|^2) c_i c_j, 0.5/a_ij, the screening key
+  // cce (without the l-dependent term) and a_i (nabla_i).  The product
+  // centre is P = R_i + w_j (R_j - R_i); kernels use only the differences
+  // P - R_i = w_j (R_j - R_i) and P - R_j = -w_i (R_j - R_i).
   const double* prim;
   // per pair
   const int* pair_off;  // first prim pair of the pair

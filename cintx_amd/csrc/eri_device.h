@@ -466,11 +466,17 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
   const double rk[3] = {gk[0], gk[1], gk[2]};
   const double rl[3] = {gk[3], gk[4], gk[5]};
   const double logd_ij = gb[6], logd_kl = gk[6];
-  double rij[3], rkl[3];
+  // coordinate differences: every position the recurrences need is formed
+  // from these and the primitive weights (P - R_i = w_j R_ji), never from
+  // an absolute product centre
+  double rij[3], rkl[3], rji[3], rlk[3], rik[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    rij[d] = IBASE ? ri[d] - rj[d] : rj[d] - ri[d];
-    rkl[d] = KBASE ? rk[d] - rl[d] : rl[d] - rk[d];
+    rji[d] = rj[d] - ri[d];
+    rlk[d] = rl[d] - rk[d];
+    rik[d] = ri[d] - rk[d];
+    rij[d] = IBASE ? -rji[d] : rji[d];
+    rkl[d] = KBASE ? -rlk[d] : rlk[d];
   }
   unsigned int nsurv = 0;
 
@@ -531,7 +537,13 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
     const int kidx = koff + (kin ? kp : 0);
     const double* pk = a.prim + 8 * (size_t)kidx;
     const double akl = pk[0];
-    const double Qv[3] = {pk[1], pk[2], pk[3]};
+    const double wk = pk[1], wl = pk[2];
+    double qk[3], qb[3];  // Q - R_k, Q - R(ket base)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      qk[d] = wl * rlk[d];
+      qb[d] = KBASE ? qk[d] : -wk * rlk[d];
+    }
     const double kkl = pk[4];
     const double hkl = pk[5];
     const double ccekl = a.three_center ? 0.0 : pk[6] - a.lkl_ceil * logd_kl;
@@ -549,14 +561,15 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
       PrimTuple p;
       p.aij = pi[0];
       p.akl = akl;
-      const double Pv[3] = {pi[1], pi[2], pi[3]};
+      const double wi = pi[1], wj = pi[2];
       double rr = 0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        p.dx[d] = Pv[d] - Qv[d];
+        const double pr = wj * rji[d];  // P - R_i
+        p.dx[d] = (rik[d] + pr) - qk[d];
         rr += p.dx[d] * p.dx[d];
-        p.prb[d] = Pv[d] - (IBASE ? ri[d] : rj[d]);
-        p.qrk[d] = Qv[d] - (KBASE ? rk[d] : rl[d]);
+        p.prb[d] = IBASE ? pr : -wi * rji[d];
+        p.qrk[d] = qb[d];
       }
       p.inv_sum = 1.0 / (p.aij + akl);
       p.hij = pi[5];
@@ -802,13 +815,14 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
         double* sc = sps + pp * NSC;
         if (ok) {
           const double aij = pi[0], akl = pk[0];
-          const double Pv[3] = {pi[1], pi[2], pi[3]};
-          const double Qv[3] = {pk[1], pk[2], pk[3]};
-          double dx[3], rr = 0;
+          double dx[3], pb[3], qb[3], rr = 0;
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            dx[d] = Pv[d] - Qv[d];
+            const double pr = pi[2] * rji[d], qr = pk[2] * rlk[d];  // P - R_i, Q - R_k
+            dx[d] = (rik[d] + pr) - qr;
             rr += dx[d] * dx[d];
+            pb[d] = IBASE ? pr : -pi[1] * rji[d];
+            qb[d] = KBASE ? qr : -pk[1] * rlk[d];
           }
           const double inv_sum = 1.0 / (aij + akl);
           const double hij = pi[5], hkl = pk[5];
@@ -829,8 +843,8 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
               sc[5 + d] = dx[d];
-              sc[8 + d] = Pv[d] - (IBASE ? ri[d] : rj[d]);
-              sc[11 + d] = Qv[d] - (KBASE ? rk[d] : rl[d]);
+              sc[8 + d] = pb[d];
+              sc[11 + d] = qb[d];
             }
             sc[14] = pi[7];
             ++nsurv;

@@ -69,10 +69,10 @@ __device__ void generic_vrr(double* __restrict__ g, const GenShape& S, int axis,
 }
 
 __device__ void generic_bra(double* __restrict__ g, const GenShape& S, int axis, int r, int m,
-                            double drb) {
+                            double drb, bool ibase) {
   const int NR = S.nr;
   const int NMAX = S.lic + S.l[1];
-  const int A = S.ibase ? S.lic : S.l[1], B = S.ibase ? S.l[1] : S.lic;
+  const int A = ibase ? S.lic : S.l[1], B = ibase ? S.l[1] : S.lic;
   const int DJ = S.dj, BLK = S.dk * S.dl;
   double* ga = g + (size_t)axis * S.g4 * NR + r + (size_t)m * NR;
   double t[kChainMax];
@@ -89,17 +89,17 @@ __device__ void generic_bra(double* __restrict__ g, const GenShape& S, int axis,
 #pragma unroll
     for (int a = 0; a < kChainMax; ++a) {
       if (a > A) break;
-      const int i = S.ibase ? a : b, j = S.ibase ? b : a;
+      const int i = ibase ? a : b, j = ibase ? b : a;
       ga[(size_t)(i * DJ + j) * BLK * NR] = t[a];
     }
   }
 }
 
 __device__ void generic_ket(double* __restrict__ g, const GenShape& S, int axis, int r, int ij,
-                            double drk) {
+                            double drk, bool kbase) {
   const int NR = S.nr;
   const int MMAX = S.l[2] + S.l[3];
-  const int C = S.kbase ? S.l[2] : S.l[3], D = S.kbase ? S.l[3] : S.l[2];
+  const int C = kbase ? S.l[2] : S.l[3], D = kbase ? S.l[3] : S.l[2];
   const int DL = S.dl, BLK = S.dk * S.dl;
   double* blk = g + (size_t)axis * S.g4 * NR + r + (size_t)ij * BLK * NR;
   double t[kChainMax];
@@ -116,7 +116,7 @@ __device__ void generic_ket(double* __restrict__ g, const GenShape& S, int axis,
 #pragma unroll
     for (int c = 0; c < kChainMax; ++c) {
       if (c > C) break;
-      const int k = S.kbase ? c : d, l = S.kbase ? d : c;
+      const int k = kbase ? c : d, l = kbase ? d : c;
       blk[(k * DL + l) * NR] = t[c];
     }
   }
@@ -170,14 +170,13 @@ __global__ void __launch_bounds__(256, (DERIV ? 1 : 2)) eri_generic_kernel(EriAr
     const int boff = a.pair_off[pq.x], koff = a.pair_off[pq.y];
     const double* gb = a.pair_geo + 8 * (size_t)pq.x;
     const double* gk = a.pair_geo + 8 * (size_t)pq.y;
-    double ri[3], rj[3], rk[3], rl[3], rij[3], rkl[3];
+    // coordinate differences (positions are formed from these and the
+    // primitive weights, never from an absolute product centre)
+    double rji[3], rlk[3], rik[3];
     for (int d = 0; d < 3; ++d) {
-      ri[d] = gb[d];
-      rj[d] = gb[3 + d];
-      rk[d] = gk[d];
-      rl[d] = gk[3 + d];
-      rij[d] = S.ibase ? ri[d] - rj[d] : rj[d] - ri[d];
-      rkl[d] = S.kbase ? rk[d] - rl[d] : rl[d] - rk[d];
+      rji[d] = gb[3 + d] - gb[d];
+      rlk[d] = gk[3 + d] - gk[d];
+      rik[d] = gb[d] - gk[d];
     }
     if (t == 0) cnt[0] = 0;
     const double logd_ij = gb[6], logd_kl = gk[6];
@@ -219,10 +218,21 @@ __global__ void __launch_bounds__(256, (DERIV ? 1 : 2)) eri_generic_kernel(EriAr
         const int kidx = koff + kp;
         const double* pk = a.prim + 8 * (size_t)kidx;
         const double akl = pk[0];
-        const double Qv[3] = {pk[1], pk[2], pk[3]};
         const double ccekl = a.three_center ? 0.0 : pk[6] - a.lkl_ceil * logd_kl;
         if (ccekl > a.expcutoff) continue;  // uniform across the workgroup
         const double eijcut = a.expcutoff - ccekl;
+        // transfer base of the ket: libcint's (lk > ll) unless both shells
+        // carry angular momentum, then the centre of the larger exponent
+        // (Q lies near it, so the VRR terms stay small and the transfer
+        // adds instead of cancelling; FP64 results of (f|g)-type quartets
+        // with distant centres otherwise lose up to 7 digits)
+        const bool kb = (S.l[2] > 0 && S.l[3] > 0) ? (pk[7] >= akl - pk[7]) : (S.kbase != 0);
+        double rkl[3], qk[3], qb[3];  // HRR step, Q - R_k, Q - R(ket base)
+        for (int d = 0; d < 3; ++d) {
+          rkl[d] = kb ? -rlk[d] : rlk[d];
+          qk[d] = pk[2] * rlk[d];
+          qb[d] = kb ? qk[d] : -pk[1] * rlk[d];
+        }
         for (int ip = 0; ip < np_ij; ++ip) {
           const int iidx = boff + ip;
           const double* pi = a.prim + 8 * (size_t)iidx;
@@ -230,13 +240,17 @@ __global__ void __launch_bounds__(256, (DERIV ? 1 : 2)) eri_generic_kernel(EriAr
           // (libcint zeroes the prefactor of a bra pair with cce >= expcutoff)
           if (cceij > eijcut || cceij >= a.expcutoff) continue;
           const double aij = pi[0];
-          const double Pv[3] = {pi[1], pi[2], pi[3]};
-          double dx[3], prb[3], qrk[3], rr = 0;
+          // the bra's transfer base chosen like the ket's (uniform across
+          // the workgroup: one tuple, one primitive tuple at a time)
+          const bool ib = (S.lic > 0 && S.l[1] > 0) ? (pi[7] >= aij - pi[7]) : (S.ibase != 0);
+          double dx[3], prb[3], qrk[3], rij[3], rr = 0;
           for (int d = 0; d < 3; ++d) {
-            dx[d] = Pv[d] - Qv[d];
+            const double pr = pi[2] * rji[d];  // P - R_i
+            dx[d] = (rik[d] + pr) - qk[d];
             rr += dx[d] * dx[d];
-            prb[d] = Pv[d] - (S.ibase ? ri[d] : rj[d]);
-            qrk[d] = Qv[d] - (S.kbase ? rk[d] : rl[d]);
+            prb[d] = ib ? pr : -pi[1] * rji[d];
+            qrk[d] = qb[d];
+            rij[d] = ib ? -rji[d] : rji[d];
           }
           const double inv_sum = 1.0 / (aij + akl);
           const double hij = pi[5], hkl = pk[5];
@@ -262,14 +276,14 @@ __global__ void __launch_bounds__(256, (DERIV ? 1 : 2)) eri_generic_kernel(EriAr
           for (int c = t; c < bra_tasks; c += T) {
             const int r = c % NR, rest = c / NR;
             const int m = rest % NM, axis = rest / NM;
-            generic_bra(g, S, axis, r, m, rij[axis]);
+            generic_bra(g, S, axis, r, m, rij[axis], ib);
           }
           __syncthreads();
           if (ket_tasks) {  // (with min(lk, ll) = 0 the slots are already (k, l))
             for (int c = t; c < ket_tasks; c += T) {
               const int r = c % NR, rest = c / NR;
               const int ij = rest % IJ, axis = rest / IJ;
-              generic_ket(g, S, axis, r, ij, rkl[axis]);
+              generic_ket(g, S, axis, r, ij, rkl[axis], kb);
             }
             __syncthreads();
           }

@@ -40,7 +40,7 @@ hipError_t mlp_eval_device(const float* prm, const RysTable& tab, int n, const d
                            int64_t nx, int steps, int mode, double* u, double* w, int* status,
                            double* perr, hipStream_t st);
 int mlp_nparam();
-hipError_t launch_copy_probe(const double* src, double* dst, int64_t n, hipStream_t s);
+hipError_t launch_fill(double* dst, int64_t n, double v, hipStream_t s);
 hipError_t launch_jk_symmetrize(double* A, int n, double scale, hipStream_t s);
 hipError_t launch_jk_digest(const double* eri, int64_t out_base, int64_t block, const int2* q,
                             int64_t nq, const int2* pair_sh, const int* ao_off, const int* shl_l,
@@ -318,10 +318,14 @@ static HostBasis ingest(const int* atm, int natm, const int* bas, int nbas, cons
 // ------------------------------------------------------------------
 // Primitive-pair data (CINTset_pairdata, optimizer.c:288-342; screening
 // keys cint2e.c:93-123, 199-236).  cce stored without the l-dependent
-// l_ij*log(|R_ij|+1) term, which the kernel subtracts per class.
+// l_ij*log(|R_ij|+1) term, which the kernel subtracts per class.  The
+// product centre is kept as the weights w_i = a_i/a_ij, w_j = a_j/a_ij
+// instead of P itself: the kernels form P - R_i = w_j (R_j - R_i) and
+// P - Q from coordinate differences, so no absolute coordinate enters a
+// difference (one-centre terms vanish exactly, as they do analytically).
 // ------------------------------------------------------------------
 struct PairSet {
-  std::vector<double> aij, px, py, pz, kfac, ai, cce, hinv;
+  std::vector<double> aij, wi, wj, kfac, ai, cce, hinv;
   std::vector<int> off, np;
   std::vector<int2> shells;  // original shells (a, b) of each pair (b = -1: pseudo)
   std::vector<double> geo;
@@ -362,12 +366,10 @@ struct PairSet {
         double ci = B.coefs[A.off + ip], cj = B.coefs[Bs.off + jp];
         double inv = 1.0 / (a_i + a_j);
         double eij = rr * a_i * a_j * inv;
-        double wj = a_j * inv;
         aij.push_back(a_i + a_j);
         hinv.push_back(0.5 * inv);
-        px.push_back(A.r[0] + wj * (Bs.r[0] - A.r[0]));
-        py.push_back(A.r[1] + wj * (Bs.r[1] - A.r[1]));
-        pz.push_back(A.r[2] + wj * (Bs.r[2] - A.r[2]));
+        wi.push_back(a_i * inv);
+        wj.push_back(a_j * inv);
         kfac.push_back(std::exp(-eij) * ci * cj);
         ai.push_back(a_i);
         cce.push_back(eij - log_rr0 - std::log(std::fabs(ci)) - std::log(std::fabs(cj)));
@@ -400,9 +402,8 @@ struct PairSet {
     for (int kp = 0; kp < K.nprim; ++kp) {
       aij.push_back(B.exps[K.off + kp]);
       hinv.push_back(0.5 / B.exps[K.off + kp]);
-      px.push_back(K.r[0]);
-      py.push_back(K.r[1]);
-      pz.push_back(K.r[2]);
+      wi.push_back(1.0);
+      wj.push_back(0.0);
       kfac.push_back(B.coefs[K.off + kp]);
       ai.push_back(B.exps[K.off + kp]);
       cce.push_back(0.0);
@@ -476,9 +477,7 @@ struct cintx_plan {
   hipGraphExec_t graph_exec = nullptr;
   double* graph_out = nullptr;
   // device pointers of the pair SoA
-  double *d_aij = nullptr, *d_px = nullptr, *d_py = nullptr, *d_pz = nullptr, *d_kfac = nullptr,
-         *d_ai = nullptr, *d_cce = nullptr, *d_geo = nullptr, *d_hinv = nullptr,
-         *d_rec = nullptr;
+  double *d_geo = nullptr, *d_rec = nullptr;
   int* d_poff = nullptr;
   int* d_pnp = nullptr;
 };
@@ -650,9 +649,7 @@ static void plan_upload(cintx_plan* P) {
     cur = align256(cur + bytes);
     return o;
   };
-  size_t o_aij = take(npp * 8), o_px = take(npp * 8), o_py = take(npp * 8), o_pz = take(npp * 8),
-         o_k = take(npp * 8), o_ai = take(npp * 8), o_cce = take(npp * 8), o_h = take(npp * 8),
-         o_rec = take(npp * 64);
+  size_t o_rec = take(npp * 64);
   size_t o_poff = take(npair * 4), o_pnp = take(npair * 4), o_geo = take(npair * 64);
   std::vector<size_t> o_q(P->bins.size());
   for (size_t i = 0; i < P->bins.size(); ++i) o_q[i] = take(P->bins[i].q.size() * sizeof(int2));
@@ -679,22 +676,14 @@ static void plan_upload(cintx_plan* P) {
   auto put = [&](size_t o, const void* src, size_t bytes) {
     if (bytes) std::memcpy(h.data() + o, src, bytes);
   };
-  put(o_aij, S.aij.data(), npp * 8);
-  put(o_px, S.px.data(), npp * 8);
-  put(o_py, S.py.data(), npp * 8);
-  put(o_pz, S.pz.data(), npp * 8);
-  put(o_k, S.kfac.data(), npp * 8);
-  put(o_ai, S.ai.data(), npp * 8);
-  put(o_cce, S.cce.data(), npp * 8);
-  put(o_h, S.hinv.data(), npp * 8);
   {
     std::vector<double> rec(npp * 8);
     for (size_t k = 0; k < npp; ++k) {
       double* r = rec.data() + 8 * k;
       r[0] = S.aij[k];
-      r[1] = S.px[k];
-      r[2] = S.py[k];
-      r[3] = S.pz[k];
+      r[1] = S.wi[k];
+      r[2] = S.wj[k];
+      r[3] = 0.0;
       r[4] = S.kfac[k];
       r[5] = S.hinv[k];
       r[6] = S.cce[k];
@@ -712,14 +701,6 @@ static void plan_upload(cintx_plan* P) {
   }
   HIPCHK(hipMemcpyAsync(P->blob, h.data(), host_bytes, hipMemcpyHostToDevice, C.stream));
   HIPCHK(hipStreamSynchronize(C.stream));
-  P->d_aij = (double*)(P->blob + o_aij);
-  P->d_px = (double*)(P->blob + o_px);
-  P->d_py = (double*)(P->blob + o_py);
-  P->d_pz = (double*)(P->blob + o_pz);
-  P->d_kfac = (double*)(P->blob + o_k);
-  P->d_ai = (double*)(P->blob + o_ai);
-  P->d_cce = (double*)(P->blob + o_cce);
-  P->d_hinv = (double*)(P->blob + o_h);
   P->d_rec = (double*)(P->blob + o_rec);
   P->d_poff = (int*)(P->blob + o_poff);
   P->d_pnp = (int*)(P->blob + o_pnp);
@@ -787,14 +768,6 @@ static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count,
   EriArgs a;
   // (generic bins run the runtime-class kernel below)
   std::memset(&a, 0, sizeof a);
-  a.aij = P->d_aij;
-  a.px = P->d_px;
-  a.py = P->d_py;
-  a.pz = P->d_pz;
-  a.kfac = P->d_kfac;
-  a.ai = P->d_ai;
-  a.cce = P->d_cce;
-  a.hinv = P->d_hinv;
   a.prim = P->d_rec;
   a.pair_off = P->d_poff;
   a.pair_np = P->d_pnp;
@@ -2058,25 +2031,11 @@ int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
   });
 }
 
-int cintx_copy_probe(int64_t nelem, int reps, double* ms) {
+int cintx_plan_fill_output(cintx_plan* plan, double value) {
   return guarded([&] {
-    if (nelem <= 0 || reps <= 0) fail(CINTX_STATUS_INVALID_INPUT, "nelem and reps must be positive");
+    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
     DevCtx& C = ctx();
-    DevArr<double> a((size_t)nelem), b((size_t)nelem);
-    HIPCHK(hipMemsetAsync(a.p, 0, (size_t)nelem * 8, C.stream));
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(launch_copy_probe(a.p, b.p, nelem, C.stream));  // warm-up
-    HIPCHK(hipEventRecord(e0, C.stream));
-    for (int r = 0; r < reps; ++r) HIPCHK(launch_copy_probe(a.p, b.p, nelem, C.stream));
-    HIPCHK(hipEventRecord(e1, C.stream));
-    HIPCHK(hipStreamSynchronize(C.stream));
-    float t = 0;
-    HIPCHK(hipEventElapsedTime(&t, e0, e1));
-    if (ms) *ms = t / reps;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    HIPCHK(launch_fill(plan->d_out, plan->total, value, C.stream));
   });
 }
 

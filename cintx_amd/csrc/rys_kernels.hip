@@ -31,17 +31,19 @@ hipError_t launch_rys_eval(const RysTable& tab, int n, const double* x, int64_t 
 
 namespace cx {
 
-// Plain streaming copy (one read, one write per element): the calibration
-// reference for hardware byte counters (FETCH_SIZE / WRITE_SIZE).
-__global__ void copy_probe_kernel(const double* __restrict__ src, double* __restrict__ dst,
-                                  int64_t n) {
+// Fill n doubles with one value (a kernel, not hipMemsetAsync, so that it
+// can sit in a captured graph and write any bit pattern, NaN included).
+__global__ void fill_kernel(double* __restrict__ dst, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+    dst[i] = v;
 }
 
-hipError_t launch_copy_probe(const double* src, double* dst, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(copy_probe_kernel, dim3(2048), dim3(256), 0, s, src, dst, n);
+hipError_t launch_fill(double* dst, int64_t n, double v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t want = (n + 255) / 256;
+  const unsigned grid = (unsigned)(want < 4096 ? want : 4096);
+  hipLaunchKernelGGL(fill_kernel, dim3(grid), dim3(256), 0, s, dst, n, v);
   return hipGetLastError();
 }
 

@@ -269,9 +269,11 @@ int cintx_rys_mlp_eval(const float *params, int n, const double *x, int64_t nx,
                        int newton_steps, int init_mode, double *u, double *w, int *status,
                        double *polish_err, double *ms);
 
-/* A plain device copy of nelem doubles, reps times (average ms per copy):
- * the reference kernel for calibrating hardware byte counters. */
-int cintx_copy_probe(int64_t nelem, int reps, double *ms);
+/* Fill the plan's own output buffer with `value` on the plan's stream
+ * (asynchronous; a kernel, so it may be captured in a graph).  No counterpart
+ * in the reference: lets a caller poison the buffer (NaN) before a replay to
+ * see that every block is rewritten. */
+int cintx_plan_fill_output(cintx_plan *plan, double value);
 
 /* Device memory helpers for callers without a runtime of their own. */
 int cintx_device_alloc(double **ptr, int64_t nelem);

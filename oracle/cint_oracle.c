@@ -25,6 +25,16 @@
  * The memory layout of the intermediate g tensor is this file's own
  * ([axis][root][i][j][k][l]); only the arithmetic and the output layout follow
  * libcint.
+ *
+ * Two builds of this one source: the default build computes in FP64 like
+ * libcint (the oracle proper); with ORC_EXT every value after the inputs
+ * (pair data, Rys rule, recurrences, gout, contraction, cart -> sph) is an
+ * 80-bit long double (64-bit significand), the extended judge of FP64
+ * rounding: the same algorithm with 2^11 times smaller rounding, so its
+ * results stand for the exact integrals of the FP64 inputs at the 1e-14
+ * level even where cancellation in the recurrences costs FP64 evaluations
+ * several digits.  The screening decisions (cce keys) stay FP64 in both, so
+ * both builds sum the same primitive tuples.
  */
 #include <math.h>
 #include <float.h>
@@ -34,6 +44,18 @@
 
 #ifdef _OPENMP
 #include <omp.h>
+#endif
+
+#ifdef ORC_EXT
+typedef long double real;
+#define RSQRT sqrtl
+#define REXP expl
+#define RPI 3.14159265358979323846264338327950288L
+#else
+typedef double real;
+#define RSQRT sqrt
+#define REXP exp
+#define RPI M_PI
 #endif
 
 #define EXPCUTOFF 60.0        /* cint_config.h.in: EXPCUTOFF */
@@ -211,9 +233,9 @@ static void tridiag_ql(int n, long double *d, long double *e, long double *z0)
     }
 }
 
-int orc_rys_roots_m(int n, double xd, double *u, double *w, int nodes)
+/* The rule in long double: s_i = t_i^2 ascending and w_i. */
+static void rys_rule_ld(int n, long double x, long double *sout, long double *wout, int nodes)
 {
-    long double x = xd;
     /* the measure beyond t_max carries < e^{-(100+4n)} of any moment used */
     long double T = 100.0L + 4.0L * n;
     long double tmax = 1.0L;
@@ -272,10 +294,25 @@ int orc_rys_roots_m(int n, double xd, double *u, double *w, int nodes)
         idx[j + 1] = v;
     }
     for (int i = 0; i < n; i++) {
-        long double si = d[idx[i]];
-        long double wi = mu0 * z0[idx[i]] * z0[idx[i]];
-        u[i] = (double)(si / (1.0L - si));
-        w[i] = (double)wi;
+        sout[i] = d[idx[i]];
+        wout[i] = mu0 * z0[idx[i]] * z0[idx[i]];
+    }
+}
+
+/* The rule at full long-double precision (s = t^2 ascending, w): the roots
+ * the extended build's recurrences use. */
+void orc_rys_rule_ld(int n, double x, long double *s, long double *w)
+{
+    rys_rule_ld(n, x, s, w, 0);
+}
+
+int orc_rys_roots_m(int n, double x, double *u, double *w, int nodes)
+{
+    long double s[ORC_MAX_ROOTS], wl[ORC_MAX_ROOTS];
+    rys_rule_ld(n, x, s, wl, nodes);
+    for (int i = 0; i < n; i++) {
+        u[i] = (double)(s[i] / (1.0L - s[i]));
+        w[i] = (double)wl[i];
     }
     return 0;
 }
@@ -283,6 +320,17 @@ int orc_rys_roots_m(int n, double xd, double *u, double *w, int nodes)
 int orc_rys_roots(int n, double x, double *u, double *w)
 {
     return orc_rys_roots_m(n, x, u, w, 0);
+}
+
+/* The rule as the recurrences use it (u = s/(1-s), w) in the build's type. */
+static void rys_roots_real(int n, real x, real *u, real *w)
+{
+    long double s[ORC_MAX_ROOTS], wl[ORC_MAX_ROOTS];
+    rys_rule_ld(n, x, s, wl, 0);
+    for (int i = 0; i < n; i++) {
+        u[i] = (real)(s[i] / (1.0L - s[i]));
+        w[i] = (real)wl[i];
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -295,7 +343,7 @@ int orc_rys_roots(int n, double x, double *u, double *w)
  * real solid harmonics (Helgaker, Jorgensen, Olsen, "Molecular
  * Electronic-Structure Theory", eq. 6.4.47-6.4.50) times sqrt((2l+1)/4pi). */
 /* ------------------------------------------------------------------ */
-static double c2s_tab[LMAX_ORC + 1][2 * LMAX_ORC + 1][NCART_MAX];
+static real c2s_tab[LMAX_ORC + 1][2 * LMAX_ORC + 1][NCART_MAX];
 static volatile int c2s_ready = 0;
 
 static long double binoml(int n, int k)
@@ -347,7 +395,7 @@ static void build_c2s(void)
                         int py = 2 * uu + v2;
                         if (px < 0) continue;
                         int col = cart_index(l, px, py);
-                        c2s_tab[l][m + l][col] += (double)(ynorm * nlm * c);
+                        c2s_tab[l][m + l][col] += (real)(ynorm * nlm * c);
                     }
                 }
             }
@@ -373,7 +421,7 @@ double orc_c2s_coef(int l, int m, int c)
 {
     ensure_c2s();
     if (l < 0 || l > LMAX_ORC || m < 0 || m >= nsph(l) || c < 0 || c >= ncart(l)) return 0;
-    return c2s_tab[l][m][c];
+    return (double)c2s_tab[l][m][c];
 }
 
 /* ------------------------------------------------------------------ */
@@ -395,10 +443,15 @@ static void get_shl(Shl *s, int sh, const int *atm, const int *bas, const double
     s->c = env + b[6];
 }
 
-static double common_fac_sp(int l) /* g1e.c:566-573 */
+static real common_fac_sp(int l) /* g1e.c:566-573 */
 {
+#ifdef ORC_EXT
+    if (l == 0) return 0.5L / RSQRT(RPI);          /* 1/(2 sqrt(pi)) */
+    if (l == 1) return RSQRT(3.0L / (4.0L * RPI)); /* sqrt(3/(4 pi)) */
+#else
     if (l == 0) return 0.282094791773878143;
     if (l == 1) return 0.488602511902919921;
+#endif
     return 1.0;
 }
 
@@ -446,9 +499,9 @@ static inline size_t gidx(const GDesc *d, int ax, int r, int i, int j, int k, in
 /* H(a, b+1) = H(a+1, b) + dr * H(a, b): a runs over the base centre's powers,
  * b over the other centre's (g2e.c:428-690).  src[a] for a = 0..A+B with
  * stride sstride; dst[a*(B+1)+b]. */
-static void hrr_1d(double *dst, const double *src, int sstride, int A, int B, double dr)
+static void hrr_1d(real *dst, const real *src, int sstride, int A, int B, real dr)
 {
-    double tmp[(2 * LMAX_ORC + 2) * (LMAX_ORC + 2)];
+    real tmp[(2 * LMAX_ORC + 2) * (LMAX_ORC + 2)];
     int W = A + B + 1;
     for (int a = 0; a < W; a++) tmp[a * (B + 1) + 0] = src[a * sstride];
     for (int b = 0; b < B; b++)
@@ -458,39 +511,53 @@ static void hrr_1d(double *dst, const double *src, int sstride, int A, int B, do
         for (int b = 0; b <= B; b++) dst[a * (B + 1) + b] = tmp[a * (B + 1) + b];
 }
 
-static void build_g(double *G, const GDesc *d, double aij, double akl, const double *P,
-                    const double *Q, double fac)
+/* ai, aj, ak, al: the primitive exponents.  The FP64 build transfers from
+ * libcint's base centres (the higher l, g2e.c:102-150); the extended build
+ * (the judge) transfers from the centre of the larger exponent of each pair
+ * whose shells both carry angular momentum -- the product centre lies near
+ * it, so the transfer adds terms instead of cancelling them, and the judge
+ * stays exact to ~1e-18 relative on quartets where libcint's direction loses
+ * several digits. */
+static void build_g(real *G, const GDesc *d, real aij, real akl, const real *P,
+                    const real *Q, real fac, double ai, double aj, double ak, double al)
 {
+    int ibase = d->ibase, kbase = d->kbase;
+#ifdef ORC_EXT
+    if (d->li > 0 && d->lj > 0) ibase = ai >= aj;
+    if (d->lk > 0 && d->ll > 0) kbase = ak >= al;
+#else
+    (void)ai; (void)aj; (void)ak; (void)al;
+#endif
     const int nr = d->nroots;
     const int nmax = d->li + d->lj, mmax = d->lk + d->ll;
-    double dx[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
-    double rr = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
-    double a1 = aij * akl;
-    double a0 = a1 / (aij + akl);
-    double fac1 = sqrt(a0 / (a1 * a1 * a1)) * fac;
-    double x = a0 * rr;
-    double u[ORC_MAX_ROOTS], w[ORC_MAX_ROOTS];
-    orc_rys_roots(nr, x, u, w);
-    const double *Rb = d->ibase ? d->ri : d->rj;
-    const double *Rq = d->kbase ? d->rk : d->rl;
-    double g2[(2 * LMAX_ORC + 2) * (2 * LMAX_ORC + 2)];
-    double h1[(LMAX_ORC + 1) * (LMAX_ORC + 1) * (2 * LMAX_ORC + 2)];
+    real dx[3] = {P[0] - Q[0], P[1] - Q[1], P[2] - Q[2]};
+    real rr = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+    real a1 = aij * akl;
+    real a0 = a1 / (aij + akl);
+    real fac1 = RSQRT(a0 / (a1 * a1 * a1)) * fac;
+    real x = a0 * rr;
+    real u[ORC_MAX_ROOTS], w[ORC_MAX_ROOTS];
+    rys_roots_real(nr, x, u, w);
+    const double *Rb = ibase ? d->ri : d->rj;
+    const double *Rq = kbase ? d->rk : d->rl;
+    real g2[(2 * LMAX_ORC + 2) * (2 * LMAX_ORC + 2)];
+    real h1[(LMAX_ORC + 1) * (LMAX_ORC + 1) * (2 * LMAX_ORC + 2)];
     const int W2 = mmax + 1;
 
     for (int r = 0; r < nr; r++) {
         /* g2e.c:4523-4543 */
-        double u2 = a0 * u[r];
-        double tmp4 = .5 / (u2 * (aij + akl) + a1);
-        double tmp5 = u2 * tmp4;
-        double tmp1 = 2. * tmp5;
-        double tmp2 = tmp1 * akl;
-        double tmp3 = tmp1 * aij;
-        double b00 = tmp5;
-        double b10 = tmp5 + tmp4 * akl;
-        double b01 = tmp5 + tmp4 * aij;
+        real u2 = a0 * u[r];
+        real tmp4 = .5 / (u2 * (aij + akl) + a1);
+        real tmp5 = u2 * tmp4;
+        real tmp1 = 2. * tmp5;
+        real tmp2 = tmp1 * akl;
+        real tmp3 = tmp1 * aij;
+        real b00 = tmp5;
+        real b10 = tmp5 + tmp4 * akl;
+        real b01 = tmp5 + tmp4 * aij;
         for (int ax = 0; ax < 3; ax++) {
-            double c00 = (P[ax] - Rb[ax]) - tmp2 * dx[ax];
-            double c0p = (Q[ax] - Rq[ax]) + tmp3 * dx[ax];
+            real c00 = (P[ax] - Rb[ax]) - tmp2 * dx[ax];
+            real c0p = (Q[ax] - Rq[ax]) + tmp3 * dx[ax];
             /* 2D VRR (g2e.c:272-411), g2[n*W2 + m] */
             g2[0] = (ax == 2) ? w[r] * fac1 : 1.0;
             if (nmax > 0) g2[1 * W2] = c00 * g2[0];
@@ -498,39 +565,39 @@ static void build_g(double *G, const GDesc *d, double aij, double akl, const dou
                 g2[(n + 1) * W2] = c00 * g2[n * W2] + n * b10 * g2[(n - 1) * W2];
             for (int m = 0; m < mmax; m++) {
                 for (int n = 0; n <= nmax; n++) {
-                    double v = c0p * g2[n * W2 + m];
+                    real v = c0p * g2[n * W2 + m];
                     if (m > 0) v += m * b01 * g2[n * W2 + m - 1];
                     if (n > 0) v += n * b00 * g2[(n - 1) * W2 + m];
                     g2[n * W2 + m + 1] = v;
                 }
             }
             /* bra HRR: for every ket power m, transfer from the bra base */
-            int A = d->ibase ? d->li : d->lj;
-            int B = d->ibase ? d->lj : d->li;
-            double drb = d->ibase ? d->ri[ax] - d->rj[ax] : d->rj[ax] - d->ri[ax];
+            int A = ibase ? d->li : d->lj;
+            int B = ibase ? d->lj : d->li;
+            real drb = ibase ? (real)d->ri[ax] - d->rj[ax] : (real)d->rj[ax] - d->ri[ax];
             /* h1[(i*(lj+1) + j)*W2 + m] */
             for (int m = 0; m <= mmax; m++) {
-                double hb[(LMAX_ORC + 1) * (LMAX_ORC + 1)];
+                real hb[(LMAX_ORC + 1) * (LMAX_ORC + 1)];
                 hrr_1d(hb, g2 + m, W2, A, B, drb);
                 for (int a = 0; a <= A; a++)
                     for (int b = 0; b <= B; b++) {
-                        int i = d->ibase ? a : b;
-                        int j = d->ibase ? b : a;
+                        int i = ibase ? a : b;
+                        int j = ibase ? b : a;
                         h1[(i * (d->lj + 1) + j) * W2 + m] = hb[a * (B + 1) + b];
                     }
             }
             /* ket HRR */
-            int C = d->kbase ? d->lk : d->ll;
-            int D = d->kbase ? d->ll : d->lk;
-            double drk = d->kbase ? d->rk[ax] - d->rl[ax] : d->rl[ax] - d->rk[ax];
+            int C = kbase ? d->lk : d->ll;
+            int D = kbase ? d->ll : d->lk;
+            real drk = kbase ? (real)d->rk[ax] - d->rl[ax] : (real)d->rl[ax] - d->rk[ax];
             for (int i = 0; i <= d->li; i++)
                 for (int j = 0; j <= d->lj; j++) {
-                    double hk[(LMAX_ORC + 1) * (LMAX_ORC + 1)];
+                    real hk[(LMAX_ORC + 1) * (LMAX_ORC + 1)];
                     hrr_1d(hk, h1 + (i * (d->lj + 1) + j) * W2, 1, C, D, drk);
                     for (int c = 0; c <= C; c++)
                         for (int e = 0; e <= D; e++) {
-                            int k = d->kbase ? c : e;
-                            int l = d->kbase ? e : c;
+                            int k = kbase ? c : e;
+                            int l = kbase ? e : c;
                             G[gidx(d, ax, r, i, j, k, l)] = hk[c * (D + 1) + e];
                         }
                 }
@@ -551,9 +618,9 @@ typedef struct {
 } Family;
 
 typedef struct {
-    double cce;
-    double e;
-    double P[3];
+    double cce; /* screening key (FP64 in both builds) */
+    real e;
+    real P[3];
 } PairPrim;
 
 /* CINTset_pairdata (optimizer.c:288-342); returns 1 if all pairs screened. */
@@ -579,9 +646,15 @@ static int set_pairdata(PairPrim *pd, const Shl *si, const Shl *sj, int li_ceil,
             pd[n].cce = cce;
             if (cce < expcutoff) {
                 empty = 0;
-                double wj = sj->a[jp] * ainv;
-                for (int d = 0; d < 3; d++) pd[n].P[d] = si->r[d] + wj * (sj->r[d] - si->r[d]);
-                pd[n].e = exp(-eij);
+                real ainvr = (real)1 / ((real)si->a[ip] + sj->a[jp]);
+                real wj = sj->a[jp] * ainvr;
+                real rrr = 0;
+                for (int d = 0; d < 3; d++) {
+                    real dr = (real)sj->r[d] - si->r[d];
+                    rrr += dr * dr;
+                    pd[n].P[d] = si->r[d] + wj * dr;
+                }
+                pd[n].e = REXP(-(rrr * si->a[ip] * sj->a[jp] * ainvr));
             } else {
                 pd[n].P[0] = pd[n].P[1] = pd[n].P[2] = 1e18;
                 pd[n].e = 0;
@@ -592,7 +665,7 @@ static int set_pairdata(PairPrim *pd, const Shl *si, const Shl *sj, int li_ceil,
 
 /* Apply per-axis transforms: src[l][k][j][i] (i fastest) with sizes
  * (ci,cj,ck,cl) -> dst with sizes (si,sj,sk,sl) using T_x[s][c]. */
-static void transform4(double *dst, const double *src, const int lv[4], int spherical, double *work)
+static void transform4(real *dst, const real *src, const int lv[4], int spherical, real *work)
 {
     int cin[4], cout[4];
     for (int a = 0; a < 4; a++) {
@@ -601,16 +674,16 @@ static void transform4(double *dst, const double *src, const int lv[4], int sphe
     }
     if (!spherical) {
         size_t n = (size_t)cin[0] * cin[1] * cin[2] * cin[3];
-        memcpy(dst, src, n * sizeof(double));
+        memcpy(dst, src, n * sizeof(real));
         return;
     }
     ensure_c2s();
     size_t nall = (size_t)cin[0] * cin[1] * cin[2] * cin[3];
-    double *bufA = work, *bufB = work + nall;
-    const double *cur = src;
+    real *bufA = work, *bufB = work + nall;
+    const real *cur = src;
     int dim[4] = {cin[0], cin[1], cin[2], cin[3]};
     for (int a = 0; a < 4; a++) {
-        double *nxt = (a == 3) ? dst : ((a % 2 == 0) ? bufA : bufB);
+        real *nxt = (a == 3) ? dst : ((a % 2 == 0) ? bufA : bufB);
         size_t inner = 1, outer = 1;
         for (int b = 0; b < a; b++) inner *= dim[b];
         for (int b = a + 1; b < 4; b++) outer *= dim[b];
@@ -618,9 +691,9 @@ static void transform4(double *dst, const double *src, const int lv[4], int sphe
         for (size_t o = 0; o < outer; o++)
             for (int s = 0; s < ns; s++)
                 for (size_t in = 0; in < inner; in++) {
-                    double acc = 0;
+                    real acc = 0;
                     for (int c = 0; c < nc; c++) {
-                        double t = c2s_tab[lv[a]][s][c];
+                        real t = c2s_tab[lv[a]][s][c];
                         if (t != 0.0) acc += t * cur[(o * nc + c) * inner + in];
                     }
                     nxt[(o * ns + s) * inner + in] = acc;
@@ -631,7 +704,7 @@ static void transform4(double *dst, const double *src, const int lv[4], int sphe
 }
 
 /* g of one axis with the exponent of slot `slot` moved by `delta` */
-static double gshift(const double *G, const GDesc *gd, int axis, int r, const int *p, int slot,
+static real gshift(const real *G, const GDesc *gd, int axis, int r, const int *p, int slot,
                      int delta)
 {
     int q[4] = {p[0], p[1], p[2], p[3]};
@@ -702,7 +775,7 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
     gd.rk = s[2].r;
     gd.rl = s[3].r;
 
-    double common = (M_PI * M_PI * M_PI) * 2 / sqrt(M_PI) * common_fac_sp(lv[0]) *
+    real common = (RPI * RPI * RPI) * 2 / RSQRT(RPI) * common_fac_sp(lv[0]) *
                     common_fac_sp(lv[2]);
     if (!fam->two_center) common *= common_fac_sp(lv[1]); /* g2c2e.c:44-45: i and k only */
     if (!fam->three_center) common *= common_fac_sp(lv[3]);
@@ -721,11 +794,11 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
     size_t gsz = (size_t)3 * gd.nroots * (gd.li + 1) * (gd.lj + 1) * (gd.lk + 1) * (gd.ll + 1);
 
     PairPrim *pdij = malloc(sizeof(PairPrim) * s[0].nprim * s[1].nprim);
-    double *G = malloc(sizeof(double) * gsz);
-    double *gout = malloc(sizeof(double) * nfall * ncomp);
-    double *gctr = calloc(nfall * nctr_all * ncomp, sizeof(double));
-    double *work = malloc(sizeof(double) * 2 * nfall);
-    double *blk = malloc(sizeof(double) * nfall);
+    real *G = malloc(sizeof(real) * gsz);
+    real *gout = malloc(sizeof(real) * nfall * ncomp);
+    real *gctr = calloc(nfall * nctr_all * ncomp, sizeof(real));
+    real *work = malloc(sizeof(real) * 2 * nfall);
+    real *blk = malloc(sizeof(real) * nfall);
 
     int empty = set_pairdata(pdij, &s[0], &s[1], gd.li, gd.lj, expcutoff);
     if (fam->two_center) {
@@ -743,7 +816,11 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
         int ix[4][NCART_MAX], iy[4][NCART_MAX], iz[4][NCART_MAX];
         for (int a = 0; a < 4; a++) cart_comp(lv[a], ix[a], iy[a], iz[a]);
         double rr_kl = 0;
-        for (int d = 0; d < 3; d++) rr_kl += (s[2].r[d] - s[3].r[d]) * (s[2].r[d] - s[3].r[d]);
+        real rr_klr = 0;
+        for (int d = 0; d < 3; d++) {
+            rr_kl += (s[2].r[d] - s[3].r[d]) * (s[2].r[d] - s[3].r[d]);
+            rr_klr += ((real)s[2].r[d] - s[3].r[d]) * ((real)s[2].r[d] - s[3].r[d]);
+        }
         double lck[MAXPRIM], lcl[MAXPRIM];
         log_max_coeff(lck, &s[2]);
         log_max_coeff(lcl, &s[3]);
@@ -758,7 +835,9 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
             for (int kp = 0; kp < s[2].nprim; kp++) {
                 double ak = s[2].a[kp], al = s[3].a[lp];
                 double akl = ak + al;
-                double Q[3], ekl_exp, eijcutoff;
+                real aklr = (real)ak + al;
+                real Q[3], ekl_exp;
+                double eijcutoff;
                 if (fam->three_center) {
                     for (int d = 0; d < 3; d++) Q[d] = s[2].r[d];
                     ekl_exp = 1.0;
@@ -767,21 +846,21 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
                     double ekl = rr_kl * ak * al / akl;
                     double ccekl = ekl - log_rr_kl - lck[kp] - lcl[lp];
                     if (ccekl > expcutoff) continue;
-                    for (int d = 0; d < 3; d++) Q[d] = (ak * s[2].r[d] + al * s[3].r[d]) / akl;
+                    for (int d = 0; d < 3; d++) Q[d] = (ak * (real)s[2].r[d] + al * (real)s[3].r[d]) / aklr;
                     eijcutoff = expcutoff - ccekl;
-                    ekl_exp = exp(-ekl);
+                    ekl_exp = REXP(-(rr_klr * ak * al / aklr));
                 }
                 for (int jp = 0; jp < s[1].nprim; jp++)
                     for (int ip = 0; ip < s[0].nprim; ip++) {
                         const PairPrim *pp = &pdij[jp * s[0].nprim + ip];
                         if (pp->cce > eijcutoff) continue;
                         double ai = s[0].a[ip], aj = s[1].a[jp];
-                        double fac = common * pp->e * ekl_exp;
-                        build_g(G, &gd, ai + aj, akl, pp->P, Q, fac);
+                        real fac = common * pp->e * ekl_exp;
+                        build_g(G, &gd, (real)ai + aj, aklr, pp->P, Q, fac, ai, aj, ak, al);
                         not0 = 1;
                         /* gout over Cartesian components, element order
                          * (i fastest, j, k, l) */
-                        double ai2 = -2.0 * ai;
+                        real ai2 = -2.0 * (real)ai;
                         size_t n = 0;
                         for (int fl = 0; fl < nf[3]; fl++)
                             for (int fk = 0; fk < nf[2]; fk++)
@@ -791,7 +870,7 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
                                         int py[4] = {iy[0][fi], iy[1][fj], iy[2][fk], iy[3][fl]};
                                         int pz[4] = {iz[0][fi], iz[1][fj], iz[2][fk], iz[3][fl]};
                                         if (!fam->deriv_i && !fam->deriv_k) {
-                                            double acc = 0;
+                                            real acc = 0;
                                             for (int r = 0; r < gd.nroots; r++)
                                                 acc += G[gidx(&gd, 0, r, px[0], px[1], px[2], px[3])] *
                                                        G[gidx(&gd, 1, r, py[0], py[1], py[2], py[3])] *
@@ -802,15 +881,15 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
                                              * (g2e.c:4574-4620); nabla_k the same on
                                              * the k index with ak (g2e.c:4675-4720) */
                                             int ds = fam->deriv_i ? 0 : 2;
-                                            double a2 = fam->deriv_i ? ai2 : -2.0 * ak;
-                                            double sx = 0, sy = 0, sz = 0;
+                                            real a2 = fam->deriv_i ? ai2 : -2.0 * (real)ak;
+                                            real sx = 0, sy = 0, sz = 0;
                                             for (int r = 0; r < gd.nroots; r++) {
-                                                double gx = G[gidx(&gd, 0, r, px[0], px[1], px[2], px[3])];
-                                                double gy = G[gidx(&gd, 1, r, py[0], py[1], py[2], py[3])];
-                                                double gz = G[gidx(&gd, 2, r, pz[0], pz[1], pz[2], pz[3])];
-                                                double fx = a2 * gshift(G, &gd, 0, r, px, ds, 1);
-                                                double fy = a2 * gshift(G, &gd, 1, r, py, ds, 1);
-                                                double fz = a2 * gshift(G, &gd, 2, r, pz, ds, 1);
+                                                real gx = G[gidx(&gd, 0, r, px[0], px[1], px[2], px[3])];
+                                                real gy = G[gidx(&gd, 1, r, py[0], py[1], py[2], py[3])];
+                                                real gz = G[gidx(&gd, 2, r, pz[0], pz[1], pz[2], pz[3])];
+                                                real fx = a2 * gshift(G, &gd, 0, r, px, ds, 1);
+                                                real fy = a2 * gshift(G, &gd, 1, r, py, ds, 1);
+                                                real fz = a2 * gshift(G, &gd, 2, r, pz, ds, 1);
                                                 if (px[ds] > 0) fx += px[ds] * gshift(G, &gd, 0, r, px, ds, -1);
                                                 if (py[ds] > 0) fy += py[ds] * gshift(G, &gd, 1, r, py, ds, -1);
                                                 if (pz[ds] > 0) fz += pz[ds] * gshift(G, &gd, 2, r, pz, ds, -1);
@@ -829,14 +908,14 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
                             for (int kc = 0; kc < nctr[2]; kc++)
                                 for (int jc = 0; jc < nctr[1]; jc++)
                                     for (int ic = 0; ic < nctr[0]; ic++, blkc++) {
-                                        double c = s[0].c[ic * s[0].nprim + ip] *
+                                        real c = (real)s[0].c[ic * s[0].nprim + ip] *
                                                    s[1].c[jc * s[1].nprim + jp] *
                                                    s[2].c[kc * s[2].nprim + kp] *
                                                    s[3].c[lc * s[3].nprim + lp];
                                         if (c == 0.0) continue;
                                         for (int cp = 0; cp < ncomp; cp++) {
-                                            double *dst = gctr + (cp * nctr_all + blkc) * nfall;
-                                            const double *src = gout + cp * nfall;
+                                            real *dst = gctr + (cp * nctr_all + blkc) * nfall;
+                                            const real *src = gout + cp * nfall;
                                             for (size_t f = 0; f < nfall; f++) dst[f] += c * src[f];
                                         }
                                     }
@@ -869,7 +948,7 @@ static int tuple_driver(const Family *fam, double *out, const int *dims, const i
                                     for (int mi = 0; mi < ns[0]; mi++, q++) {
                                         size_t I = ic * ns[0] + mi, J = jc * ns[1] + mj;
                                         size_t K = kc * ns[2] + mk, L = lc * ns[3] + ml;
-                                        o[I + dd[0] * (J + dd[1] * (K + dd[2] * L))] = blk[q];
+                                        o[I + dd[0] * (J + dd[1] * (K + dd[2] * L))] = (double)blk[q];
                                     }
                     }
     }

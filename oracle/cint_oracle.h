@@ -7,6 +7,11 @@
  * __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker;
  * the product library (libcintx_hip.so) never links, loads or calls it.
  *
+ * Built from one source as liboracle_cint.so (FP64, libcint's rounding),
+ * liboracle_cint_fma.so (the same with fused multiply-add) and
+ * liboracle_cint_ext.so (long double with well-conditioned transfers: the
+ * judge of FP64 rounding; see cint_oracle.c).
+ *
  * Every entry point keeps libcint's raw signature (out, dims, shls, atm, natm,
  * bas, nbas, env, opt, cache) with an `orc_` prefix so it can sit in the same
  * process as the product library without symbol clashes.
@@ -33,6 +38,10 @@ void orc_boys(int mmax, double x, double *f);
 int orc_rys_roots(int nroots, double x, double *u, double *w);
 /* Same, with the number of discretisation nodes forced (convergence tests). */
 int orc_rys_roots_m(int nroots, double x, double *u, double *w, int nodes);
+
+/* The same rule unrounded: s = t^2 = u/(1+u) ascending and w, in long double
+ * (80-bit on x86-64). */
+void orc_rys_rule_ld(int nroots, double x, long double *s, long double *w);
 
 /* Cartesian->real-spherical coefficient C[m][c] for angular momentum l
  * (cart2sph.c:21 table; p is the identity, cart2sph.c:33-43). */

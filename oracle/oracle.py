@@ -1,4 +1,7 @@
-"""ctypes binding of the CPU oracle (oracle/liboracle_cint.so).
+"""ctypes binding of the CPU oracle (oracle/liboracle_cint.so) and of its
+extended-precision build (oracle/liboracle_cint_ext.so: the same algorithm
+in 80-bit long double, the judge of FP64 rounding on ill-conditioned
+quartets; pass ext=True).
 
 TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py as the checker.  The product package never
@@ -12,6 +15,11 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_cint.so")
+EXT_PATH = os.path.join(HERE, "liboracle_cint_ext.so")
+FMA_PATH = os.path.join(HERE, "liboracle_cint_fma.so")
+# build selector of lib()/eval_*(): False the FP64 oracle, True the long
+# double judge, "fma" the FP64 oracle with fused multiply-add contraction
+_PATHS = {False: LIB_PATH, True: EXT_PATH, "fma": FMA_PATH}
 
 API_INT2E_SPH = 0
 API_INT2E_IP1_SPH = 1
@@ -19,24 +27,26 @@ API_INT3C2E_SPH = 2
 API_INT3C2E_IP1_SPH = 3
 API_INT2E_CART = 4
 
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def lib(ext=False):
+    path = _PATHS[ext]
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         dp = ctypes.POINTER(ctypes.c_double)
         ip = ctypes.POINTER(ctypes.c_int)
         L.orc_boys.argtypes = [ctypes.c_int, ctypes.c_double, dp]
         L.orc_rys_roots.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp]
         L.orc_rys_roots_m.argtypes = [ctypes.c_int, ctypes.c_double, dp, dp, ctypes.c_int]
+        ldp = ctypes.POINTER(ctypes.c_longdouble)
+        L.orc_rys_rule_ld.argtypes = [ctypes.c_int, ctypes.c_double, ldp, ldp]
         L.orc_c2s_coef.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_c2s_coef.restype = ctypes.c_double
         sig = [dp, ip, ip, ip, ctypes.c_int, ip, ctypes.c_int, dp, ctypes.c_void_p, dp]
@@ -51,8 +61,8 @@ def lib():
         L.orc_eval_list.restype = ctypes.c_long
         L.orc_work_model.argtypes = [ctypes.c_int, ip, ip, ctypes.c_int, ip, ctypes.c_int,
                                      dp, dp]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _dp(a):
@@ -77,6 +87,15 @@ def rys_roots(n, x, nodes=0):
     else:
         lib().orc_rys_roots(n, float(x), _dp(u), _dp(w))
     return u, w
+
+
+def rys_rule_ld(n, x):
+    """(s, w) of the n-point rule in long double (numpy longdouble arrays)."""
+    s = np.zeros(n, dtype=np.longdouble)
+    w = np.zeros(n, dtype=np.longdouble)
+    ldp = ctypes.POINTER(ctypes.c_longdouble)
+    lib().orc_rys_rule_ld(n, float(x), s.ctypes.data_as(ldp), w.ctypes.data_as(ldp))
+    return s, w
 
 
 def c2s_matrix(l):
@@ -117,7 +136,7 @@ def tuple_dims(name, bas, shls):
     return [shell_dim(bas, s, cart) for s in shls[:nsh]]
 
 
-def eval_tuple(name, shls, atm, bas, env, dims=None, out=None):
+def eval_tuple(name, shls, atm, bas, env, dims=None, out=None, ext=False):
     """One libcint-signature call; returns (out array (F-order block), not0)."""
     atm = np.ascontiguousarray(atm, dtype=np.int32)
     bas = np.ascontiguousarray(bas, dtype=np.int32)
@@ -130,13 +149,13 @@ def eval_tuple(name, shls, atm, bas, env, dims=None, out=None):
     dims_arr = None
     if dims is not None:
         dims_arr = np.ascontiguousarray(dims, dtype=np.int32)
-    fn = getattr(lib(), _FN[name])
+    fn = getattr(lib(ext), _FN[name])
     not0 = fn(_dp(out), _ip(dims_arr) if dims_arr is not None else None, _ip(sh), _ip(atm),
               len(atm), _ip(bas), len(bas), _dp(env), None, None)
     return out, not0
 
 
-def eval_list(name, shls_list, atm, bas, env, nthreads=0):
+def eval_list(name, shls_list, atm, bas, env, nthreads=0, ext=False):
     """Evaluate many tuples (OpenMP); returns (flat out, offsets)."""
     atm = np.ascontiguousarray(atm, dtype=np.int32)
     bas = np.ascontiguousarray(bas, dtype=np.int32)
@@ -150,7 +169,7 @@ def eval_list(name, shls_list, atm, bas, env, nthreads=0):
     if len(sl):
         offsets[1:] = np.cumsum(sizes)[:-1]
     out = np.zeros(int(sizes.sum()))
-    lib().orc_eval_list(_API[name], len(sl), _ip(sl),
+    lib(ext).orc_eval_list(_API[name], len(sl), _ip(sl),
                         offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), _dp(out),
                         _ip(atm), len(atm), _ip(bas), len(bas), _dp(env), nthreads)
     return out, offsets, sizes
@@ -167,7 +186,7 @@ def work_model(name, shls, atm, bas, env):
     return c
 
 
-def full_tensor_int2e(atm, bas, env, nthreads=0):
+def full_tensor_int2e(atm, bas, env, nthreads=0, ext=False):
     """Full (nao,nao,nao,nao) int2e_sph tensor, libcint F-order per block."""
     bas = np.asarray(bas)
     nbas = len(bas)
@@ -176,7 +195,7 @@ def full_tensor_int2e(atm, bas, env, nthreads=0):
     nao = int(offs[-1])
     shls = [(i, j, k, l) for l in range(nbas) for k in range(nbas)
             for j in range(nbas) for i in range(nbas)]
-    out, offsets, sizes = eval_list("int2e_sph", shls, atm, bas, env, nthreads)
+    out, offsets, sizes = eval_list("int2e_sph", shls, atm, bas, env, nthreads, ext=ext)
     eri = np.zeros((nao, nao, nao, nao))
     for q, (i, j, k, l) in enumerate(shls):
         blk = out[offsets[q]:offsets[q] + sizes[q]].reshape(

@@ -1,7 +1,13 @@
 """GPU parity: libcintx_hip.so (HIP kernels on the MI355X) against the oracle
 on the same seeded inputs.  FP64 bar: the reference's own element criterion
 |got - ref| <= 1e-12 + 1e-12 |ref| (UNIFIED_ATOL/UNIFIED_RTOL and the pass rule
-of crates/cintx-oracle/src/compare.rs:21-22, 256-259)."""
+of crates/cintx-oracle/src/compare.rs:21-22, 256-259) for every family,
+derivatives included, with `ref` from the oracle's long-double build (the
+judge: the exact integrals of the FP64 inputs, pinned in
+test_oracle_judge.py).  The oracle's FP64 build restates libcint's rounding
+as well as its algorithm, and that rounding alone breaks the criterion on
+ill-conditioned quartets (up to 1e-10 on (g d|g g) of the s..g molecule, ~1e-11
+on cfg-1 nabla integrals), so it is not the reference value here."""
 import ctypes
 import sys
 
@@ -14,18 +20,11 @@ from cintx_amd import molecule as M
 pytestmark = pytest.mark.gpu
 ATOL = 1e-12
 RTOL = 1e-12
-# nabla_i integrals at cfg-1 exponents (up to 5000) cancel terms of size
-# ~2a g; two FP64 builds of the oracle that differ only in FMA contraction
-# already break the element rule there, so the derivative families use the
-# rule plus the measured FP64 floor (test_library_cpu.deriv_tolerance).
-from test_library_cpu import IP2_FLOOR, deriv_tolerance
 
 
-def assert_parity(got, ref, block_scale=None, floor=3e-11):
+def assert_parity(got, ref):
     err = np.abs(got - ref)
     tol = ATOL + RTOL * np.abs(ref)
-    if block_scale is not None:
-        tol = deriv_tolerance(ref, block_scale, floor)
     bad = err > tol
     say(f"  max abs err {err.max():.3e}  max |ref| {np.abs(ref).max():.3e}  "
         f"max rel(>1e-6) {np.max(err[np.abs(ref) > 1e-6] / np.abs(ref[np.abs(ref) > 1e-6]), initial=0):.3e}")
@@ -38,7 +37,8 @@ def say(msg):
 
 
 def oracle_blocks(O, name, tuples, mol):
-    out, offs, sizes = O.eval_list(name, tuples, mol.atm, mol.bas, mol.env)
+    """The judge's blocks (long-double oracle) of the tuples, libcint layout."""
+    out, offs, sizes = O.eval_list(name, tuples, mol.atm, mol.bas, mol.env, ext=True)
     return out, offs, sizes
 
 
@@ -53,11 +53,7 @@ def check_plan(O, name, tuples, mol, variant=None):
     ref, offs, sizes = oracle_blocks(O, name, tuples, mol)
     t, o = p.tuples()
     assert np.array_equal(o, offs)
-    scale = None
-    if "_ip" in name:
-        scale = np.repeat([np.abs(ref[a:a + n]).max() if n else 0.0
-                           for a, n in zip(offs, sizes)], sizes)
-    assert_parity(got, ref, scale, IP2_FLOOR if "_ip2" in name else 3e-11)
+    assert_parity(got, ref)
     p.close()
     return got
 
@@ -146,11 +142,11 @@ def test_cart_families(oracle):
         sh = [int(low[i]) for i in rng.integers(0, len(low), 4)]
         dims = [d + 1 for d in capi.natural_dims("int2e_cart", sh, gc.bas)]
         got, not0 = capi.raw_call("int2e_cart", sh, gc.atm, gc.bas, gc.env, dims)
-        ref, rnot0 = oracle.eval_tuple("int2e_cart", sh, gc.atm, gc.bas, gc.env, dims=dims)
+        ref, rnot0 = oracle.eval_tuple("int2e_cart", sh, gc.atm, gc.bas, gc.env, dims=dims, ext=True)
         assert not0 == rnot0
         assert_parity(got, ref)
         got, not0 = capi.raw_call("int3c2e_cart", sh[:3], gc.atm, gc.bas, gc.env)
-        ref, rnot0 = oracle.eval_tuple("int3c2e_cart", sh[:3], gc.atm, gc.bas, gc.env)
+        ref, rnot0 = oracle.eval_tuple("int3c2e_cart", sh[:3], gc.atm, gc.bas, gc.env, ext=True)
         assert not0 == rnot0
         assert_parity(got, ref)
     with pytest.raises(capi.CintxError):
@@ -178,13 +174,14 @@ def test_raw_signature_dims_and_general_contraction(oracle):
     for _ in range(25):
         shls = [int(low[i]) for i in rng.integers(0, len(low), 4)]
         got, not0 = capi.int2e_sph(shls, mol.atm, mol.bas, mol.env)
-        ref, rnot0 = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env)
+        ref, rnot0 = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env, ext=True)
         assert not0 == rnot0
         assert_parity(got, ref)
         d0 = capi.natural_dims("int2e_sph", shls, mol.bas)
         dims = [d + 1 for d in d0]
         gd, _ = capi.int2e_sph(shls, mol.atm, mol.bas, mol.env, dims=dims)
-        rd, _ = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env, dims=dims)
+        rd, _ = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env, dims=dims,
+                                  ext=True)
         assert_parity(gd, rd)
     shls = [int(low[0]), int(low[1]), int(low[2])]
     for name in ("int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph",
@@ -192,15 +189,15 @@ def test_raw_signature_dims_and_general_contraction(oracle):
         sh = shls + [int(low[3])] if name.startswith("int2e") else shls
         sh = sh[:capi.arity(name)]
         got, not0 = capi.raw_call(name, sh, mol.atm, mol.bas, mol.env)
-        ref, rnot0 = oracle.eval_tuple(name, sh, mol.atm, mol.bas, mol.env)
+        ref, rnot0 = oracle.eval_tuple(name, sh, mol.atm, mol.bas, mol.env, ext=True)
         assert not0 == rnot0
-        assert_parity(got, ref, np.abs(ref).max() if "_ip" in name else None,
-                      IP2_FLOOR if "_ip2" in name else 3e-11)
+        assert_parity(got, ref)
     for _ in range(10):
         sh = [int(low[i]) for i in rng.integers(0, len(low), 2)]
         dims = [d + 2 for d in capi.natural_dims("int2c2e_sph", sh, mol.bas)]
         gd, not0 = capi.int2c2e_sph(sh, mol.atm, mol.bas, mol.env, dims=dims)
-        rd, rnot0 = oracle.eval_tuple("int2c2e_sph", sh, mol.atm, mol.bas, mol.env, dims=dims)
+        rd, rnot0 = oracle.eval_tuple("int2c2e_sph", sh, mol.atm, mol.bas, mol.env, dims=dims,
+                                       ext=True)
         assert not0 == rnot0
         assert_parity(gd, rd)
 
@@ -211,14 +208,14 @@ def test_cintrs_eval_on_device(oracle):
     req = int(np.prod(capi.natural_dims("int2e_sph", shls, mol.bas)))
     out = np.zeros(req)
     s = capi.eval_raw(capi.INT2E_SPH, out, None, shls, mol.atm, mol.bas, mol.env)
-    ref, not0 = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env)
+    ref, not0 = oracle.eval_tuple("int2e_sph", shls, mol.atm, mol.bas, mol.env, ext=True)
     assert s["not0"] == not0 and s["bytes_written"] == req * 8
     assert_parity(out, ref)
     # api 9 (int2e_cart): Cartesian extents
     req = int(np.prod(capi.natural_dims("int2e_cart", shls, mol.bas)))
     out = np.zeros(req)
     s = capi.eval_raw(capi.INT2E_CART, out, None, shls, mol.atm, mol.bas, mol.env)
-    ref, not0 = oracle.eval_tuple("int2e_cart", shls, mol.atm, mol.bas, mol.env)
+    ref, not0 = oracle.eval_tuple("int2e_cart", shls, mol.atm, mol.bas, mol.env, ext=True)
     assert s["not0"] == not0 and s["bytes_written"] == req * 8
     assert_parity(out, ref)
 
@@ -232,13 +229,18 @@ def test_graph_replay_equals_eager(oracle):
     p.execute()
     p.sync()
     eager = p.output()
+    assert not np.isnan(eager).any()
     for rep in range(4):
+        # poison the buffer first: a bin the graph left out would stay NaN
+        p.fill_output(np.nan)
         p.execute(graph=True)
         p.sync()
         again = p.output()
         assert np.array_equal(eager, again), rep
+        p.fill_output(np.nan)
         p.execute()
         p.sync()
+        assert np.array_equal(eager, p.output()), rep
     p.close()
 
 
@@ -258,10 +260,13 @@ def test_graph_chains_with_runtime_class_bins(oracle):
     p.execute()
     p.sync()
     eager = p.output()
+    assert not np.isnan(eager).any()
     for rep in range(3):
+        p.fill_output(np.nan)
         p.execute(graph=True)
         p.sync()
         assert np.array_equal(eager, p.output()), rep
+        p.fill_output(np.nan)
         p.execute()
         p.sync()
     ref, offs, sizes = oracle_blocks(oracle, "int2e_sph", tuples, mol)
@@ -279,14 +284,110 @@ def test_screened_plan_cfg2(oracle):
     t, offs = p.tuples()
     say(f"  tuples {p.ntuples}")
     assert p.ntuples > 1000
-    # unique-quartet invariant: i>=j, k>=l and bra pair id >= ket pair id
     rng = np.random.default_rng(7)
     pick = rng.choice(p.ntuples, size=min(2000, p.ntuples), replace=False)
     sub = [tuple(int(v) for v in t[q]) for q in pick]
-    ref, roffs, sizes = oracle.eval_list("int2e_sph", sub, mol.atm, mol.bas, mol.env)
+    ref, roffs, sizes = oracle.eval_list("int2e_sph", sub, mol.atm, mol.bas, mol.env, ext=True)
     blk = np.concatenate([got[offs[q]:offs[q] + sizes[k]] for k, q in enumerate(pick)])
     assert_parity(blk, ref)
     p.close()
+
+
+def test_screened_enumeration_complete_and_unique(oracle):
+    """The screened plan holds each unordered quartet of shells at most once
+    (i >= j, k >= l, ij >= kl in the pair order), and exactly the quartets
+    whose Schwarz product sqrt(max|(ij|ij)|) sqrt(max|(kl|kl)|) reaches the
+    threshold, with bra pairs restricted to every stride-th pair: the set is
+    recomputed here from the judge's diagonal integrals (pairs within 1e-6
+    relative of the threshold, where the device's and the judge's rounding
+    may decide differently, are left out of the comparison)."""
+    mol = M.config(2)
+    thr, stride = 1e-12, 37
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, screened=(thr, stride))
+    t, _ = p.tuples()
+    p.close()
+    n = mol.nbas
+    ls = mol.bas[:, 1]
+    pid = {}
+    diag = []
+    for i in range(n):
+        for j in range(i + 1):
+            pid[(i, j)] = len(diag)
+            diag.append((i, j, i, j))
+    ref, offs, sizes = oracle.eval_list("int2e_sph", diag, mol.atm, mol.bas, mol.env, ext=True)
+    q = np.sqrt([np.abs(ref[a:a + k]).max() for a, k in zip(offs, sizes)])
+    qmax = q.max()
+
+    def key(a, b):
+        return pid[(max(a, b), min(a, b))]
+
+    got = {}
+    for a, b, c, d in t:
+        ka, kb = key(a, b), key(c, d)
+        assert ls[a] >= ls[b] and ls[c] >= ls[d]
+        assert ka >= kb
+        k = (ka, kb)
+        assert k not in got, k
+        got[k] = True
+    say(f"  screened cfg2 stride {stride}: {len(got)} unique quartets")
+    live = np.where(q * qmax >= thr)[0]
+    border = 0
+    want = set()
+    for ka in live:
+        if ka % stride:
+            continue
+        for kb in live[live <= ka]:
+            v = q[ka] * q[kb]
+            if abs(v - thr) <= 1e-6 * thr:
+                border += 1
+                got.pop((int(ka), int(kb)), None)
+                continue
+            if v >= thr:
+                want.add((int(ka), int(kb)))
+    assert want == set(got), (len(want - set(got)), len(set(got) - want), border)
+
+
+def test_high_l_against_judge_sg_sample(oracle):
+    """The f/g classes of the 32-atom s..g molecule (runtime-class kernel):
+    the benchmark's own sample (300 tuples, seed 5, of the screened plan with
+    bra stride 64, and of its every-8th-tuple int2e_ip1_sph plan), the
+    (g d|g g) tuple on which libcint's FP64 route loses 1e-10, and 1500
+    random tuples with an f or g shell, all at the element rule against the
+    judge."""
+    mol = M.config(5)
+    p5 = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, screened=(1e-12, 64))
+    t5, _ = p5.tuples()
+    p5.execute()
+    p5.sync()
+    rng = np.random.default_rng(5)
+    pick = np.sort(rng.choice(len(t5), size=min(300, len(t5)), replace=False))
+    got, _, _ = p5.output_tuples(pick)
+    ref, _, _ = oracle.eval_list("int2e_sph", [tuple(int(v) for v in t5[q]) for q in pick],
+                                 mol.atm, mol.bas, mol.env, ext=True)
+    say(f"plan int2e_sph s..g screened sample {len(pick)} of {len(t5)}")
+    assert_parity(got, ref)
+    p5.close()
+    t8 = np.array(t5[::8], dtype=np.int32)
+    pi = capi.Plan("int2e_ip1_sph", mol.atm, mol.bas, mol.env, t8)
+    pi.execute()
+    pi.sync()
+    rng = np.random.default_rng(5)
+    pick = np.sort(rng.choice(len(t8), size=min(300, len(t8)), replace=False))
+    got, _, _ = pi.output_tuples(pick)
+    ref, _, _ = oracle.eval_list("int2e_ip1_sph", [tuple(int(v) for v in t8[q]) for q in pick],
+                                 mol.atm, mol.bas, mol.env, ext=True)
+    say(f"plan int2e_ip1_sph s..g sample {len(pick)}")
+    assert_parity(got, ref)
+    pi.close()
+    ls = mol.bas[:, 1]
+    rng = np.random.default_rng(23)
+    hl = [(215, 140, 79, 71)]
+    while len(hl) < 1500:
+        sh = tuple(int(s) for s in rng.integers(0, mol.nbas, 4))
+        if max(ls[list(sh)]) >= 3:
+            hl.append(sh)
+    check_plan(oracle, "int2e_sph", hl, mol)
+    check_plan(oracle, "int2e_ip1_sph", hl[:600], mol)
 
 
 def test_high_l_generic_kernel_cfg5(oracle):
@@ -320,7 +421,7 @@ def test_jk_build_cfg1(oracle):
     # J/K from the screened unique-quartet plan (screen off) against an
     # einsum over the oracle's full cfg-1 tensor
     mol = M.config(1)
-    eri = oracle.full_tensor_int2e(mol.atm, mol.bas, mol.env)
+    eri = oracle.full_tensor_int2e(mol.atm, mol.bas, mol.env, ext=True)
     nao = eri.shape[0]
     rng = np.random.default_rng(11)
     D = rng.normal(size=(nao, nao)) / nao
@@ -409,7 +510,7 @@ def test_mixed_plan_error_bound_cfg2(oracle):
                                         replace=False))
     got, _, _ = p.output_tuples(pick)
     ref, _, _ = oracle.eval_list("int2e_sph", [tuple(int(v) for v in t[q]) for q in pick],
-                                 mol.atm, mol.bas, mol.env)
+                                 mol.atm, mol.bas, mol.env, ext=True)
     err = float(np.abs(got - ref).max())
     say(f"  mixed cfg2: {p.ntuples_fp32}/{p.ntuples} tuples in FP32, max abs err {err:.3e}")
     assert err <= 1e-10
@@ -430,6 +531,7 @@ def test_autotune_keeps_results(oracle):
     say("  variants " + " ".join(f"{b['variant']}/{b['nvariants']}:T{b['lanes']}P{b['prims']}"
                                  for b in info))
     assert all(0 <= b["variant"] < b["nvariants"] for b in info)
+    p.fill_output(np.nan)
     p.execute(graph=True)
     p.sync()
     after = p.output()

@@ -123,72 +123,55 @@ def test_generated_molecules_have_the_quoted_sizes():
     assert m4.nao(range(m4.aux_start, m4.nbas)) == 4544
 
 
-def deriv_tolerance(ref, block_scale, floor=3e-11):
-    """Parity rule for nabla integrals: the reference's element rule
-    (crates/cintx-oracle/src/compare.rs:256-259) plus the FP64 rounding floor
-    of the algorithm, 1e-13 of the tuple's largest element and `floor`
-    absolute: 3e-11 for nabla_i (test_fp64_floor_of_derivative_integrals),
-    IP2_FLOOR for nabla_k (test_fp64_floor_of_nabla_k_routes)."""
-    return 1e-12 + 1e-12 * np.abs(ref) + 1e-13 * block_scale + floor
-
-
-# the device evaluates (ij|nabla k l) as the nabla_i integral of the
-# bra-ket swapped quartet; the oracle follows libcint's nabla_k recurrence
-IP2_FLOOR = 1e-10
-
-
-def test_fp64_floor_of_derivative_integrals(oracle):
-    """Two FP64 builds of the oracle that differ only in FMA contraction
-    disagree on nabla_i integrals of the cfg-1 molecule (exponents up to 5000,
-    terms 2a*g that cancel, e.g. on one centre) by up to ~1e-11 absolute: the
-    reference's element rule cannot separate two correct FP64 evaluations
-    there, the relaxed rule used for the device parity tests can."""
+def test_fp64_routes_are_no_judge_for_cfg1_derivatives(oracle):
+    """libcint's FP64 algorithm (the oracle) and the same source built with
+    fused multiply-add both break the reference's element rule against the
+    long-double judge on nabla_i integrals of the cfg-1 molecule (exponents
+    up to 5000: terms 2a g that cancel, absolute coordinates rounded into
+    the product centres), so the device parity tests take the judge, not an
+    FP64 build, as the reference value."""
     import subprocess
     from oracle import oracle as O
     subprocess.check_call(["make", "-s", "-C", O.HERE])
     mol = M.config(1)
     rng = np.random.default_rng(3)
     tuples = [tuple(int(s) for s in rng.integers(0, mol.nbas, 4)) for _ in range(2000)]
-    ref, offs, sizes = O.eval_list("int2e_ip1_sph", tuples, mol.atm, mol.bas, mol.env)
-    saved = O._lib
-    try:
-        O._lib = None
-        O.LIB_PATH = os.path.join(O.HERE, "liboracle_cint_fma.so")
-        got, _, _ = O.eval_list("int2e_ip1_sph", tuples, mol.atm, mol.bas, mol.env)
-    finally:
-        O.LIB_PATH = os.path.join(O.HERE, "liboracle_cint.so")
-        O._lib = saved
-    err = np.abs(got - ref)
-    strict = 1e-12 + 1e-12 * np.abs(ref)
-    assert (err > strict).sum() > 0
-    scale = np.repeat([np.abs(ref[a:a + n]).max() if n else 0.0 for a, n in zip(offs, sizes)],
-                      sizes)
-    assert np.all(err <= deriv_tolerance(ref, scale))
+    judge, _, _ = O.eval_list("int2e_ip1_sph", tuples, mol.atm, mol.bas, mol.env, ext=True)
+    strict = 1e-12 + 1e-12 * np.abs(judge)
+    for build in (False, "fma"):
+        got, _, _ = O.eval_list("int2e_ip1_sph", tuples, mol.atm, mol.bas, mol.env, ext=build)
+        err = np.abs(got - judge)
+        assert (err > strict).sum() > 0, build
+        assert err.max() < 1e-10, build
 
 
-def test_fp64_floor_of_nabla_k_routes(oracle):
+def test_judge_nabla_k_equals_nabla_i_of_the_swapped_quartet(oracle):
     """(ij|nabla k l) through libcint's nabla_k recurrence and through the
-    nabla_i recurrence of (k l|i j) are both correct FP64 evaluations; on
-    one-centre quartets of the cfg-1 molecule, whose exact value is 0 and
-    whose computed value is the rounding residue of 2a*g terms with a up to
-    5000, they differ by up to ~4e-11, beyond the nabla_i floor but well
-    inside IP2_FLOOR."""
+    nabla_i recurrence of (k l|i j) (the device's route) agree in the
+    long-double judge to 1e-13 on the cfg-1 molecule, where the two FP64
+    routes differ by up to ~4e-11 (one-centre quartets whose exact value is
+    0 and whose FP64 value is the rounding residue of 2a*g terms)."""
     mol = M.config(1)
     rng = np.random.default_rng(13)
     tuples = [tuple(int(s) for s in rng.integers(0, mol.nbas, 4)) for _ in range(2000)]
-    ref, offs, sizes = oracle.eval_list("int2e_ip2_sph", tuples, mol.atm, mol.bas, mol.env)
     swapped = [(k, l, i, j) for (i, j, k, l) in tuples]
-    alt, offs2, sizes2 = oracle.eval_list("int2e_ip1_sph", swapped, mol.atm, mol.bas, mol.env)
-    got = np.zeros_like(ref)
-    for t, sh in enumerate(tuples):
-        d = oracle.tuple_dims("int2e_ip2_sph", mol.bas, sh)
-        b = alt[offs2[t]:offs2[t] + sizes2[t]].reshape([3, d[1], d[0], d[3], d[2]])
-        got[offs[t]:offs[t] + sizes[t]] = b.transpose(0, 3, 4, 1, 2).ravel()
-    err = np.abs(got - ref)
-    scale = np.repeat([np.abs(ref[a:a + n]).max() if n else 0.0 for a, n in zip(offs, sizes)],
-                      sizes)
-    assert (err > deriv_tolerance(ref, scale)).sum() > 0
-    assert np.all(err <= deriv_tolerance(ref, scale, IP2_FLOOR))
+
+    def both(ext):
+        ref, offs, sizes = oracle.eval_list("int2e_ip2_sph", tuples, mol.atm, mol.bas, mol.env,
+                                            ext=ext)
+        alt, offs2, sizes2 = oracle.eval_list("int2e_ip1_sph", swapped, mol.atm, mol.bas,
+                                              mol.env, ext=ext)
+        got = np.zeros_like(ref)
+        for t, sh in enumerate(tuples):
+            d = oracle.tuple_dims("int2e_ip2_sph", mol.bas, sh)
+            b = alt[offs2[t]:offs2[t] + sizes2[t]].reshape([3, d[1], d[0], d[3], d[2]])
+            got[offs[t]:offs[t] + sizes[t]] = b.transpose(0, 3, 4, 1, 2).ravel()
+        return np.abs(got - ref), ref
+
+    err, ref = both(True)
+    assert np.all(err <= 1e-13 + 1e-13 * np.abs(ref)), err.max()
+    err64, ref64 = both(False)
+    assert err64.max() > 1e-12
 
 
 def _cart(l):

@@ -72,23 +72,6 @@ def test_int3c2e_cart_equals_int2e_cart_with_unit_function(oracle):
         assert_close(three[0], four[0][..., 0] / 0.282094791773878143, atol=1e-13)
 
 
-def test_conditioning_floor_of_high_l_quartets(oracle):
-    """On (g d|g g) quartets of the s..g molecule with four distant centres the
-    Rys/HRR evaluation loses digits to cancellation: the oracle and the
-    independent McMurchie-Davidson route (both FP64) differ by ~1e-10 on
-    elements of size ~1e-5, so two correct FP64 evaluations of such a block
-    cannot be held to 1e-12 of each other (the device result of this tuple
-    lies between the two)."""
-    mol = M.config(5)
-    shls = [215, 140, 79, 71]
-    assert [int(mol.bas[s][1]) for s in shls] == [4, 2, 4, 4]
-    got, _ = block(oracle, "int2e_sph", shls, mol.atm, mol.bas, mol.env)
-    ref = md_eri.int2e_sph_md(shls, mol.atm, mol.bas, mol.env)
-    err = np.abs(got[0] - ref).max()
-    assert 1e-11 < err < 1e-9, err
-    assert np.abs(ref).max() < 1e-3
-
-
 def test_against_mcmurchie_davidson_general_contraction(oracle):
     mol = M.general_contraction_h2()
     rng = np.random.default_rng(5)
