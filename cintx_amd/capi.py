@@ -60,6 +60,7 @@ EXPORTS = [
     "cintx_plan_work", "cintx_plan_execute_timed", "cintx_plan_jk", "cintx_plan_bin_stats",
     "cintx_plan_autotune", "cintx_plan_set_variant", "cintx_plan_bin_variants",
     "cintx_plan_fill_output", "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
+    "cintx_device_upload", "cintx_device_download",
 ]
 
 
@@ -133,6 +134,10 @@ def lib():
         L.cintx_plan_set_variant.argtypes = [vp, ctypes.c_int]
         L.cintx_plan_bin_variants.argtypes = [vp, ctypes.c_int64, IP, IP, IP, IP, IP]
         L.cintx_device_count.argtypes = [IP]
+        L.cintx_device_alloc.argtypes = [ctypes.POINTER(DP), ctypes.c_int64]
+        L.cintx_device_free.argtypes = [DP]
+        L.cintx_device_upload.argtypes = [DP, DP, ctypes.c_int64]
+        L.cintx_device_download.argtypes = [DP, DP, ctypes.c_int64]
         _lib = L
     return _lib
 
@@ -401,3 +406,38 @@ def device_count():
     n = ctypes.c_int()
     _check(lib().cintx_device_count(ctypes.byref(n)))
     return n.value
+
+
+class DeviceArray:
+    """A device buffer of doubles owned by the library (cintx_device_alloc),
+    for callers without a GPU runtime of their own."""
+
+    def __init__(self, nelem):
+        self.nelem = int(nelem)
+        self.ptr = DP()
+        _check(lib().cintx_device_alloc(ctypes.byref(self.ptr), self.nelem))
+
+    @property
+    def address(self):
+        return ctypes.cast(self.ptr, ctypes.c_void_p).value
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=np.float64).ravel()
+        assert host.size == self.nelem
+        _check(lib().cintx_device_upload(self.ptr, _dp(host), self.nelem))
+
+    def download(self):
+        out = np.zeros(self.nelem)
+        _check(lib().cintx_device_download(_dp(out), self.ptr, self.nelem))
+        return out
+
+    def free(self):
+        if self.ptr:
+            _check(lib().cintx_device_free(self.ptr))
+            self.ptr = DP()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -153,6 +153,23 @@ __global__ void __launch_bounds__(256) jk_symmetrize_kernel(double* __restrict__
   A[(size_t)q * n + p] = s;
 }
 
+// dst <- (src + src^T) / 2 (out of place)
+__global__ void __launch_bounds__(256) jk_sym_copy_kernel(const double* __restrict__ src,
+                                                          double* __restrict__ dst, int n) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * n) return;
+  const int p = (int)(e / n), q = (int)(e % n);
+  dst[e] = 0.5 * (src[(size_t)p * n + q] + src[(size_t)q * n + p]);
+}
+
+hipError_t launch_jk_sym_copy(const double* src, double* dst, int n, hipStream_t s) {
+  const int64_t nn = (int64_t)n * n;
+  if (nn == 0) return hipSuccess;
+  hipLaunchKernelGGL(jk_sym_copy_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, src,
+                     dst, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_jk_symmetrize(double* A, int n, double scale, hipStream_t s) {
   const int64_t nn = (int64_t)n * n;
   if (nn == 0) return hipSuccess;

@@ -42,6 +42,7 @@ hipError_t mlp_eval_device(const float* prm, const RysTable& tab, int n, const d
 int mlp_nparam();
 hipError_t launch_fill(double* dst, int64_t n, double v, hipStream_t s);
 hipError_t launch_jk_symmetrize(double* A, int n, double scale, hipStream_t s);
+hipError_t launch_jk_sym_copy(const double* src, double* dst, int n, hipStream_t s);
 hipError_t launch_jk_digest(const double* eri, int64_t out_base, int64_t block, const int2* q,
                             int64_t nq, const int2* pair_sh, const int* ao_off, const int* shl_l,
                             const double* D, double* J, double* K, int nao, hipStream_t s);
@@ -471,7 +472,14 @@ struct cintx_plan {
   int64_t scratch_stride = 0;   // doubles per chain's slice
   int scratch_chains = 1;       // chains that may run runtime-class bins
   unsigned long long* d_counters = nullptr;  // 2 per bin
+  // J/K digestion tables: AO offset and l of every shell (spherical,
+  // segmented shells), original shells of every pair
+  int* d_ao_off = nullptr;
+  int* d_shl_l = nullptr;
+  int2* d_pair_sh = nullptr;
+  int nao = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;  // ev0/ev1 bracket an execute whose time sync may read
   double last_ms = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -654,6 +662,9 @@ static void plan_upload(cintx_plan* P) {
   std::vector<size_t> o_q(P->bins.size());
   for (size_t i = 0; i < P->bins.size(); ++i) o_q[i] = take(P->bins[i].q.size() * sizeof(int2));
   size_t o_cnt = take(P->bins.size() * 2 * 8 + 8);
+  const int nbas = P->basis.nbas;
+  size_t o_ao = take((size_t)nbas * 4), o_l = take((size_t)nbas * 4),
+         o_psh = take(npair * sizeof(int2));
   int64_t scratch = 0;
   for (auto& b : P->bins)
     if (b.generic) {
@@ -694,6 +705,17 @@ static void plan_upload(cintx_plan* P) {
   put(o_poff, S.off.data(), npair * 4);
   put(o_pnp, S.np.data(), npair * 4);
   put(o_geo, S.geo.data(), npair * 64);
+  {
+    std::vector<int> ao(nbas);
+    P->nao = 0;
+    for (int sh = 0; sh < nbas; ++sh) {
+      ao[sh] = P->nao;
+      P->nao += nsph(P->basis.l[sh]) * P->basis.nctr[sh];
+    }
+    put(o_ao, ao.data(), (size_t)nbas * 4);
+    put(o_l, P->basis.l.data(), (size_t)nbas * 4);
+    put(o_psh, S.shells.data(), npair * sizeof(int2));
+  }
   for (size_t i = 0; i < P->bins.size(); ++i) {
     Bin& b = P->bins[i];
     put(o_q[i], b.q.data(), b.q.size() * sizeof(int2));
@@ -706,6 +728,9 @@ static void plan_upload(cintx_plan* P) {
   P->d_pnp = (int*)(P->blob + o_pnp);
   P->d_geo = (double*)(P->blob + o_geo);
   P->d_counters = (unsigned long long*)(P->blob + o_cnt);
+  P->d_ao_off = (int*)(P->blob + o_ao);
+  P->d_shl_l = (int*)(P->blob + o_l);
+  P->d_pair_sh = (int2*)(P->blob + o_psh);
   P->d_out = (double*)(P->blob + o_out);
   P->d_scratch = scratch ? (double*)(P->blob + o_scr) : nullptr;
   HIPCHK(hipEventCreate(&P->ev0));
@@ -790,14 +815,20 @@ static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count,
   a.rys = C.rys;
   a.c2s = C.d_c2s;
   a.counters = count ? P->d_counters + 2 * i : nullptr;
-  // grid dimension limit: launch in chunks of at most 2^28 tuples
-  const int64_t chunk = (int64_t)1 << 28;
+  // a dispatch holds at most 2^32 - 1 work-items per grid dimension: launch
+  // in chunks of at most 2^31 lanes (2^28 tuples for the runtime-class
+  // kernel, whose grid is capped anyway)
+  const int64_t chunk = b.generic ? ((int64_t)1 << 28)
+                                  : (((int64_t)1 << 31) / std::max(1, k.lanes_per_tuple));
   for (int64_t s = 0; s < (int64_t)b.q.size(); s += chunk) {
     a.quartets = b.d_q + s;
     a.out_base = b.out_base + s * b.block;
     a.block = b.block;
     a.nq = std::min(chunk, (int64_t)b.q.size() - s);
     hipError_t e;
+    // the launch wrappers read the thread's last HIP error after launching;
+    // drop any error an earlier, deliberately unchecked call left behind
+    (void)hipGetLastError();
     if (b.generic) {
       a.scratch = P->d_scratch + (size_t)chain * (size_t)P->scratch_stride;
       int grid = (int)std::min<int64_t>(a.nq, kGenericGrid);
@@ -806,7 +837,14 @@ static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count,
       e = k.fn(a, st);
     }
     if (e != hipSuccess)
-      fail(CINTX_STATUS_EXECUTION_FAILED, std::string("kernel launch: ") + hipGetErrorString(e));
+      fail(CINTX_STATUS_EXECUTION_FAILED,
+           "kernel launch of class (" + std::to_string(b.l[0]) + std::to_string(b.l[1]) + "|" +
+               std::to_string(b.l[2]) + std::to_string(b.l[3]) + ") " +
+               (b.generic ? std::string("runtime-class kernel")
+                          : "variant " + std::to_string(b.variant) + " (" +
+                                std::to_string(k.lanes_per_tuple) + " lanes, " +
+                                std::to_string(k.tuples_per_block) + " tuples per workgroup)") +
+               ", " + std::to_string(a.nq) + " tuples: " + hipGetErrorString(e));
   }
 }
 
@@ -1296,6 +1334,7 @@ int cintx_plan_execute(cintx_plan* plan, double* out_dev) {
     HIPCHK(hipEventRecord(plan->ev0, C.stream));
     plan_launch_all(plan, out_dev ? out_dev : plan->d_out, false);
     HIPCHK(hipEventRecord(plan->ev1, C.stream));
+    plan->timed = true;
   });
 }
 
@@ -1353,6 +1392,7 @@ int cintx_plan_execute_graph(cintx_plan* plan, double* out_dev) {
     HIPCHK(hipEventRecord(plan->ev0, C.stream));
     HIPCHK(hipGraphLaunch(plan->graph_exec, C.stream));
     HIPCHK(hipEventRecord(plan->ev1, C.stream));
+    plan->timed = true;
   });
 }
 
@@ -1361,8 +1401,15 @@ int cintx_plan_sync(cintx_plan* plan) {
     if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
     DevCtx& C = ctx();
     HIPCHK(hipStreamSynchronize(C.stream));
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, plan->ev0, plan->ev1) == hipSuccess) plan->last_ms = ms;
+    // the time of the last execute, once (events never recorded would make
+    // hipEventElapsedTime fail, and that error would stay the thread's last
+    // HIP error and be reported by the next kernel launch)
+    if (plan->timed) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, plan->ev0, plan->ev1));
+      plan->last_ms = ms;
+      plan->timed = false;
+    }
   });
 }
 
@@ -1953,44 +2000,67 @@ int cintx_rys_mlp_eval(const float* params, int n, const double* x, int64_t nx, 
   });
 }
 
-// J/K build from an int2e_sph plan over unique (8-fold) quartets of
-// segmented shells; D, J, K are nao x nao row-major on the host.
+// J/K from an int2e_sph plan over unique (8-fold) quartets of segmented
+// shells: checks shared by both entry points.
+static void jk_check(const cintx_plan* plan, const void* D, const void* J, const void* K) {
+  if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
+  if (!D || !J || !K) fail(CINTX_STATUS_NULL_POINTER, "null argument");
+  if (plan->family != 0) fail(CINTX_STATUS_UNSUPPORTED_API, "J/K needs an int2e_sph plan");
+  const HostBasis& B = plan->basis;
+  for (int sh = 0; sh < B.nbas; ++sh)
+    if (B.nctr[sh] != 1) fail(CINTX_STATUS_UNSUPPORTED_API, "J/K needs segmented shells");
+}
+
+// Enqueue the J/K build on the plan stream: dJ and dK are cleared with a
+// fill kernel (graph-safe), every bin's class kernel is followed by its
+// digest on the same stream, and J = A + A^T, K = B + B^T at the end.  Dsym
+// is the symmetric density in device memory.  With chains, the bins are
+// spread over the context's chains of streams (fork/join through `ev`, the
+// caller's kChains events); without, everything runs on the plan stream.
+static void jk_enqueue(cintx_plan* plan, const double* Dsym, double* dJ, double* dK,
+                       hipEvent_t* ev) {
+  DevCtx& C = ctx();
+  const int nao = plan->nao;
+  const int64_t nn = (int64_t)nao * nao;
+  HIPCHK(launch_fill(dJ, nn, 0.0, C.stream));
+  HIPCHK(launch_fill(dK, nn, 0.0, C.stream));
+  const int nchain = ev ? kChains : 1;
+  if (ev) {
+    HIPCHK(hipEventRecord(ev[0], C.stream));
+    for (int c = 0; c < kChains - 1; ++c) HIPCHK(hipStreamWaitEvent(C.branch[c], ev[0], 0));
+  }
+  // kernels of different classes and the atomic-bound digests overlap
+  // across chains (disjoint ERI output regions; J and K take atomics only);
+  // runtime-class bins use their chain's workspace slice
+  size_t n_other = 0, n_gen = 0;
+  for (size_t i = 0; i < plan->bins.size(); ++i) {
+    const Bin& b = plan->bins[i];
+    const int chain = b.generic ? (int)(n_gen++ % std::min(plan->scratch_chains, nchain))
+                                : (int)(n_other++ % nchain);
+    hipStream_t st = chain ? C.branch[chain - 1] : C.stream;
+    plan_launch_bin(plan, i, plan->d_out, false, st, chain);
+    HIPCHK(launch_jk_digest(plan->d_out, b.out_base, b.block, b.d_q, (int64_t)b.q.size(),
+                            plan->d_pair_sh, plan->d_ao_off, plan->d_shl_l, Dsym, dJ, dK, nao,
+                            st));
+  }
+  if (ev) {
+    for (int c = 0; c < kChains - 1; ++c) {
+      HIPCHK(hipEventRecord(ev[c + 1], C.branch[c]));
+      HIPCHK(hipStreamWaitEvent(C.stream, ev[c + 1], 0));
+    }
+  }
+  // the digest accumulates one triangle's worth: J = A + A^T, K = B + B^T
+  HIPCHK(launch_jk_symmetrize(dJ, nao, 1.0, C.stream));
+  HIPCHK(launch_jk_symmetrize(dK, nao, 1.0, C.stream));
+}
+
+// J/K build; D, J, K are nao x nao row-major on the host.
 int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
   return guarded([&] {
-    if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
-    if (!D || !J || !K) fail(CINTX_STATUS_NULL_POINTER, "null argument");
-    if (plan->family != 0) fail(CINTX_STATUS_UNSUPPORTED_API, "J/K needs an int2e_sph plan");
-    const HostBasis& B = plan->basis;
-    for (int sh = 0; sh < B.nbas; ++sh)
-      if (B.nctr[sh] != 1) fail(CINTX_STATUS_UNSUPPORTED_API, "J/K needs segmented shells");
+    jk_check(plan, D, J, K);
     DevCtx& C = ctx();
-    std::vector<int> ao_off(B.nbas), shl_l(B.l);
-    int nao = 0;
-    for (int sh = 0; sh < B.nbas; ++sh) {
-      ao_off[sh] = nao;
-      nao += nsph(B.l[sh]);
-    }
-    const size_t nn = (size_t)nao * nao;
+    const size_t nn = (size_t)plan->nao * plan->nao;
     DevArr<double> dD(nn), dJ(nn), dK(nn);
-    DevArr<int> dao(B.nbas), dl(B.nbas);
-    DevArr<int2> dsh(plan->pairs.shells.size());
-    // everything on the plan's (non-blocking) stream, so the zeroing and the
-    // uploads are ordered before the digest kernels
-    HIPCHK(hipMemcpyAsync(dD.p, D, nn * 8, hipMemcpyHostToDevice, C.stream));
-    HIPCHK(hipMemsetAsync(dJ.p, 0, nn * 8, C.stream));
-    HIPCHK(hipMemsetAsync(dK.p, 0, nn * 8, C.stream));
-    HIPCHK(hipMemcpyAsync(dao.p, ao_off.data(), B.nbas * 4, hipMemcpyHostToDevice, C.stream));
-    HIPCHK(hipMemcpyAsync(dl.p, shl_l.data(), B.nbas * 4, hipMemcpyHostToDevice, C.stream));
-    HIPCHK(hipMemcpyAsync(dsh.p, plan->pairs.shells.data(),
-                          plan->pairs.shells.size() * sizeof(int2), hipMemcpyHostToDevice,
-                          C.stream));
-    // D is taken as symmetric: (D + D^T) / 2 (the digest relies on it)
-    HIPCHK(launch_jk_symmetrize(dD.p, nao, 0.5, C.stream));
-    // the bins spread over the plan graph's chains of streams, each bin's
-    // digest right after its class kernel on the same stream: kernels of
-    // different classes and the atomic-bound digests overlap (disjoint output
-    // regions; J and K take atomics only).  Runtime-class bins use their
-    // chain's workspace slice.
     hipEvent_t ev[kChains] = {};  // ev[0]: fork, ev[1..]: joins
     // (declared after the device buffers, so on an early exit it waits for
     // every chain before those are freed)
@@ -2006,25 +2076,10 @@ int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
     } evguard{ev, C};
     for (int c = 0; c < kChains; ++c)
       HIPCHK(hipEventCreateWithFlags(&ev[c], hipEventDisableTiming));
-    HIPCHK(hipEventRecord(ev[0], C.stream));
-    for (int c = 0; c < kChains - 1; ++c) HIPCHK(hipStreamWaitEvent(C.branch[c], ev[0], 0));
-    size_t n_other = 0, n_gen = 0;
-    for (size_t i = 0; i < plan->bins.size(); ++i) {
-      const Bin& b = plan->bins[i];
-      const int chain = b.generic ? (int)(n_gen++ % plan->scratch_chains)
-                                  : (int)(n_other++ % kChains);
-      hipStream_t st = chain ? C.branch[chain - 1] : C.stream;
-      plan_launch_bin(plan, i, plan->d_out, false, st, chain);
-      HIPCHK(launch_jk_digest(plan->d_out, b.out_base, b.block, b.d_q, (int64_t)b.q.size(), dsh.p,
-                              dao.p, dl.p, dD.p, dJ.p, dK.p, nao, st));
-    }
-    for (int c = 0; c < kChains - 1; ++c) {
-      HIPCHK(hipEventRecord(ev[c + 1], C.branch[c]));
-      HIPCHK(hipStreamWaitEvent(C.stream, ev[c + 1], 0));
-    }
-    // the digest accumulates one triangle's worth: J = A + A^T, K = B + B^T
-    HIPCHK(launch_jk_symmetrize(dJ.p, nao, 1.0, C.stream));
-    HIPCHK(launch_jk_symmetrize(dK.p, nao, 1.0, C.stream));
+    HIPCHK(hipMemcpyAsync(dD.p, D, nn * 8, hipMemcpyHostToDevice, C.stream));
+    // D is taken as symmetric: (D + D^T) / 2 (the digest relies on it)
+    HIPCHK(launch_jk_symmetrize(dD.p, plan->nao, 0.5, C.stream));
+    jk_enqueue(plan, dD.p, dJ.p, dK.p, ev);
     HIPCHK(hipMemcpyAsync(J, dJ.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
     HIPCHK(hipMemcpyAsync(K, dK.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
     HIPCHK(hipStreamSynchronize(C.stream));
@@ -2049,6 +2104,24 @@ int cintx_device_alloc(double** ptr, int64_t nelem) {
 }
 int cintx_device_free(double* ptr) {
   return guarded([&] { HIPCHK(hipFree(ptr)); });
+}
+int cintx_device_upload(double* dst_dev, const double* src_host, int64_t nelem) {
+  return guarded([&] {
+    if (nelem < 0) fail(CINTX_STATUS_INVALID_INPUT, "negative size");
+    if (nelem == 0) return;
+    if (!dst_dev || !src_host) fail(CINTX_STATUS_NULL_POINTER, "null argument");
+    ctx();
+    HIPCHK(hipMemcpy(dst_dev, src_host, (size_t)nelem * 8, hipMemcpyHostToDevice));
+  });
+}
+int cintx_device_download(double* dst_host, const double* src_dev, int64_t nelem) {
+  return guarded([&] {
+    if (nelem < 0) fail(CINTX_STATUS_INVALID_INPUT, "negative size");
+    if (nelem == 0) return;
+    if (!dst_host || !src_dev) fail(CINTX_STATUS_NULL_POINTER, "null argument");
+    ctx();
+    HIPCHK(hipMemcpy(dst_host, src_dev, (size_t)nelem * 8, hipMemcpyDeviceToHost));
+  });
 }
 int cintx_device_count(int* n) {
   return guarded([&] {

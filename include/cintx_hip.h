@@ -278,6 +278,9 @@ int cintx_plan_fill_output(cintx_plan *plan, double value);
 /* Device memory helpers for callers without a runtime of their own. */
 int cintx_device_alloc(double **ptr, int64_t nelem);
 int cintx_device_free(double *ptr);
+/* synchronous copies of nelem doubles host -> device and device -> host */
+int cintx_device_upload(double *dst_dev, const double *src_host, int64_t nelem);
+int cintx_device_download(double *dst_host, const double *src_dev, int64_t nelem);
 int cintx_device_count(int *n);
 
 #ifdef __cplusplus
