@@ -59,7 +59,7 @@ EXPORTS = [
     "cintx_plan_copy_output", "cintx_plan_copy_tuples", "cintx_plan_execute_graph", "cintx_plan_last_time_ms",
     "cintx_plan_work", "cintx_plan_execute_timed", "cintx_plan_jk", "cintx_plan_bin_stats",
     "cintx_plan_autotune", "cintx_plan_set_variant", "cintx_plan_bin_variants",
-    "cintx_plan_fill_output", "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
+    "cintx_plan_fill_output", "cintx_plan_jk_noalloc", "cintx_device_alloc", "cintx_device_free", "cintx_device_count",
     "cintx_device_upload", "cintx_device_download",
 ]
 
@@ -123,6 +123,7 @@ def lib():
         L.cintx_plan_execute_graph.argtypes = [vp, DP]
         L.cintx_plan_sync.argtypes = [vp]
         L.cintx_plan_fill_output.argtypes = [vp, ctypes.c_double]
+        L.cintx_plan_jk_noalloc.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.cintx_plan_copy_output.argtypes = [vp, DP]
         L.cintx_plan_copy_tuples.argtypes = [vp, I64P, ctypes.c_int64, DP, I64P]
         L.cintx_plan_last_time_ms.argtypes = [vp, DP]
@@ -357,6 +358,12 @@ class Plan:
         K = np.zeros_like(D)
         _check(lib().cintx_plan_jk(self.h, _dp(D), _dp(J), _dp(K)))
         return J, K
+
+    def jk_noalloc(self, D_ptr, J_ptr, K_ptr):
+        """J/K into caller-owned device buffers (raw device addresses), no
+        allocation; asynchronous on the plan stream (sync() waits)."""
+        _check(lib().cintx_plan_jk_noalloc(self.h, ctypes.c_void_p(D_ptr), ctypes.c_void_p(J_ptr),
+                                           ctypes.c_void_p(K_ptr)))
 
     def output(self):
         out = np.zeros(self.nelem)

@@ -13,6 +13,7 @@
 // one ([l][k][j][i], i fastest).
 #include <hip/hip_runtime.h>
 
+#include "../../include/cintx_hip.h"
 #include "common.h"
 
 namespace cx {
@@ -169,6 +170,20 @@ hipError_t launch_jk_sym_copy(const double* src, double* dst, int n, hipStream_t
                      dst, n);
   return hipGetLastError();
 }
+
+// Workspace of the allocation-free J/K entry point (cintx_plan_jk_noalloc):
+// the symmetrised density lives here instead of in a per-call allocation.
+// Each device holds its own copy from library load until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// (jk_scratch_address), never through the symbol itself.
+constexpr size_t kJkScratchBytes = CINTX_JK_NOALLOC_MAX_BYTES;
+static __device__ __align__(256) double jk_scratch[kJkScratchBytes / sizeof(double)];
+
+hipError_t jk_scratch_address(double** ptr) {
+  return hipGetSymbolAddress((void**)ptr, HIP_SYMBOL(jk_scratch));
+}
+
+size_t jk_scratch_bytes() { return kJkScratchBytes; }
 
 hipError_t launch_jk_symmetrize(double* A, int n, double scale, hipStream_t s) {
   const int64_t nn = (int64_t)n * n;

@@ -43,6 +43,8 @@ int mlp_nparam();
 hipError_t launch_fill(double* dst, int64_t n, double v, hipStream_t s);
 hipError_t launch_jk_symmetrize(double* A, int n, double scale, hipStream_t s);
 hipError_t launch_jk_sym_copy(const double* src, double* dst, int n, hipStream_t s);
+hipError_t jk_scratch_address(double** ptr);
+size_t jk_scratch_bytes();
 hipError_t launch_jk_digest(const double* eri, int64_t out_base, int64_t block, const int2* q,
                             int64_t nq, const int2* pair_sh, const int* ao_off, const int* shl_l,
                             const double* D, double* J, double* K, int nao, hipStream_t s);
@@ -206,6 +208,7 @@ struct DevCtx {
   double* d_large = nullptr;
   double* d_c2s = nullptr;
   Registry* reg = nullptr;
+  double* d_jk_scratch = nullptr;  // the static scratch of cintx_plan_jk_noalloc
   // autotuned class-kernel variant per (deriv, form, class); -1: untuned
   signed char tuned[2][kForms][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1];
 };
@@ -238,6 +241,8 @@ static DevCtx& ctx() {
   build_c2s_table(c2s);
   HIPCHK(hipMalloc(&c->d_c2s, c2s.size() * sizeof(double)));
   HIPCHK(hipMemcpy(c->d_c2s, c2s.data(), c2s.size() * sizeof(double), hipMemcpyHostToDevice));
+  // address of the allocation-free J/K's static scratch on this device
+  HIPCHK(jk_scratch_address(&c->d_jk_scratch));
   c->reg = new Registry();
   std::memset(c->reg, 0, sizeof(Registry));
   register_class_kernels(*c->reg);
@@ -2083,6 +2088,25 @@ int cintx_plan_jk(cintx_plan* plan, const double* D, double* J, double* K) {
     HIPCHK(hipMemcpyAsync(J, dJ.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
     HIPCHK(hipMemcpyAsync(K, dK.p, nn * 8, hipMemcpyDeviceToHost, C.stream));
     HIPCHK(hipStreamSynchronize(C.stream));
+  });
+}
+
+// The same J/K build with device-resident D, J, K and no allocation: the
+// symmetrised density goes to the library's static scratch (one per
+// device), everything is enqueued on the plan stream (no chains, so no
+// events are created), and the call returns without waiting.
+int cintx_plan_jk_noalloc(cintx_plan* plan, const double* D_dev, double* J_dev, double* K_dev) {
+  return guarded([&] {
+    jk_check(plan, D_dev, J_dev, K_dev);
+    DevCtx& C = ctx();
+    const size_t need = (size_t)plan->nao * plan->nao * sizeof(double);
+    if (need > jk_scratch_bytes())
+      fail(CINTX_STATUS_MEMORY_LIMIT_EXCEEDED,
+           "the density (" + std::to_string(plan->nao) + " functions, " + std::to_string(need) +
+               " bytes) exceeds the " + std::to_string(jk_scratch_bytes()) +
+               "-byte scratch of the allocation-free J/K");
+    HIPCHK(launch_jk_sym_copy(D_dev, C.d_jk_scratch, plan->nao, C.stream));
+    jk_enqueue(plan, C.d_jk_scratch, J_dev, K_dev, nullptr);
   });
 }
 

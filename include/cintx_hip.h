@@ -245,6 +245,19 @@ int cintx_rys_eval_device(int n, const double *x, int64_t nx, double *u, double 
  * Fock-like consumer of the unique-quartet workload. */
 int cintx_plan_jk(cintx_plan *plan, const double *D, double *J, double *K);
 
+/* The same J/K build for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc);
+ * no counterpart in the reference.  D_dev, J_dev, K_dev are nao x nao
+ * row-major device buffers; the call allocates nothing: the symmetrised D is
+ * kept in a static device scratch of CINTX_JK_NOALLOC_MAX_BYTES per device
+ * (nao^2 * 8 bytes above it: CINTX_STATUS_MEMORY_LIMIT_EXCEEDED), J_dev and
+ * K_dev are cleared by a kernel, and the plan's own ERI buffer and workspace
+ * (allocated when the plan was created) are reused.  Enqueued on the plan's
+ * stream; returns without waiting (cintx_plan_sync waits).  Calls that
+ * overlap on one GPU share the scratch and race. */
+#define CINTX_JK_NOALLOC_MAX_BYTES ((size_t)64 << 20)
+int cintx_plan_jk_noalloc(cintx_plan *plan, const double *D_dev, double *J_dev, double *K_dev);
+
 /* Learned Rys initializer (no counterpart in the reference): a 2x64 tanh
  * MLP mapping (log x, nroots) to initial roots/weights, trained on the device
  * by this library's forward/backward/Adam kernels on rules produced by the
