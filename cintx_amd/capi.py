@@ -186,8 +186,38 @@ def ncomp(family):
     return 3 if "_ip" in family else 1
 
 
-def raw_call(family, shls, atm, bas, env, dims=None):
-    """libcint raw signature: returns (out, not0)."""
+class Optimizer:
+    """A device-resident CINTOpt of one basis (``<family>_optimizer``): the
+    pair data of every shell pair, built once and reused by the raw calls
+    that pass it.  Immutable, so threads may share it."""
+
+    def __init__(self, family, atm, bas, env):
+        self._atm, self._bas, self._env = _arrays(atm, bas, env)
+        name = {"int2e_cart": "int2e", "int3c2e_cart": "int3c2e"}.get(family, family)
+        name = name[:-4] if name.endswith("_sph") else name
+        self.h = ctypes.c_void_p()
+        fn = getattr(lib(), name + "_optimizer")
+        fn.argtypes = [ctypes.POINTER(ctypes.c_void_p), IP, ctypes.c_int, IP, ctypes.c_int, DP]
+        fn(ctypes.byref(self.h), _ip(self._atm), len(self._atm), _ip(self._bas), len(self._bas),
+           _dp(self._env))
+        if not self.h.value:
+            raise CintxError(7, "optimizer construction failed")
+
+    def close(self):
+        if self.h and self.h.value:
+            L = lib()
+            L.CINTdel_optimizer.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+            L.CINTdel_optimizer(ctypes.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def raw_call(family, shls, atm, bas, env, dims=None, opt=None):
+    """libcint raw signature: returns (out, not0).  opt: an Optimizer or None."""
     atm, bas, env = _arrays(atm, bas, env)
     d = natural_dims(family, shls, bas) if dims is None else list(dims)
     out = np.zeros(int(np.prod(d)) * ncomp(family))
@@ -195,7 +225,7 @@ def raw_call(family, shls, atm, bas, env, dims=None):
     dd = None if dims is None else np.ascontiguousarray(dims, dtype=np.int32)
     fn = getattr(lib(), family)
     not0 = fn(_dp(out), _ip(dd), _ip(sh), _ip(atm), len(atm), _ip(bas), len(bas), _dp(env),
-              None, None)
+              None if opt is None else opt.h, None)
     return out, not0
 
 

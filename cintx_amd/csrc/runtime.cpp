@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -213,13 +215,15 @@ struct DevCtx {
   signed char tuned[2][kForms][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1];
 };
 
-static std::mutex g_mu;  // guards context creation and the per-call arena
-static DevCtx* g_ctx = nullptr;
+static std::atomic<DevCtx*> g_ctx{nullptr};
 
+// The process's device context, created on first use on the device current
+// then; later calls read it without a lock.
 static DevCtx& ctx() {
+  if (DevCtx* c = g_ctx.load(std::memory_order_acquire)) return *c;
   static std::mutex init_mu;
   std::lock_guard<std::mutex> lk(init_mu);
-  if (g_ctx) return *g_ctx;
+  if (DevCtx* c = g_ctx.load(std::memory_order_relaxed)) return *c;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     fail(CINTX_STATUS_EXECUTION_FAILED, "no HIP device visible");
@@ -247,7 +251,7 @@ static DevCtx& ctx() {
   std::memset(c->reg, 0, sizeof(Registry));
   register_class_kernels(*c->reg);
   std::memset(c->tuned, -1, sizeof c->tuned);
-  g_ctx = c;
+  g_ctx.store(c, std::memory_order_release);
   return *c;
 }
 
@@ -346,6 +350,14 @@ struct PairSet {
   int* slot(int va, int vb) {  // vb == nv: pseudo pair of va
     if (dense.empty()) return nullptr;
     return &dense[(size_t)va * (nv + 1) + vb];
+  }
+  // id of a pair added before (vb == nv: the pseudo pair of va), -1 if none
+  int find(int va, int vb) const {
+    if (!dense.empty()) return dense[(size_t)va * (nv + 1) + vb];
+    const uint64_t key = vb == nv ? (((uint64_t)0xffffffffu << 32) | (uint32_t)va)
+                                  : (((uint64_t)(uint32_t)va << 32) | (uint32_t)vb);
+    auto it = index.find(key);
+    return it == index.end() ? -1 : it->second;
   }
 
   int add(const HostBasis& B, int va, int vb) {
@@ -677,10 +689,13 @@ static void plan_upload(cintx_plan* P) {
       scratch = std::max(scratch, grid * generic_scratch_per_block(b.l, kFam[P->family].deriv));
     }
   size_t o_out = take((size_t)P->total * 8);
-  // one workspace per graph chain (so runtime-class bins of different chains
-  // run concurrently), fewer when that would pass 8 GiB
+  // one workspace per graph chain that runs runtime-class bins (so those of
+  // different chains run concurrently): as many slices as such bins, up to
+  // the chain count, fewer when that would pass 8 GiB
+  int ngeneric = 0;
+  for (auto& b : P->bins) ngeneric += b.generic ? 1 : 0;
   P->scratch_chains = 1;
-  while (P->scratch_chains < kChains &&
+  while (P->scratch_chains < kChains && 2 * P->scratch_chains <= ngeneric &&
          (size_t)scratch * 8 * (size_t)(2 * P->scratch_chains) <= ((size_t)8 << 30))
     P->scratch_chains *= 2;
   P->scratch_stride = scratch;
@@ -787,70 +802,94 @@ static void plan_check_kernels(cintx_plan* P) {
   }
 }
 
-static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count,
-                            hipStream_t st = nullptr, int chain = 0) {
+// Device pair data a launch reads (a plan's blob or an optimizer's).
+struct PairDev {
+  const double* prim;
+  const int* off;
+  const int* np;
+  const double* geo;
+};
+
+// One class bin's launches: nq tuples d_q[0..nq) of class l (canonical
+// slots) and form prec, blocks of `block` elements written from out_base.
+// Variant `k` of the compiled class kernels, or the runtime-class kernel
+// (generic) with `scratch` of kGenericGrid slices.
+static void launch_bin_kernels(int fam, const int* l, int variant, bool generic,
+                               const KernelInfo& k, const PairDev& pd, double expcutoff,
+                               const int2* d_q, int64_t nq, double* out, int64_t out_base,
+                               int64_t block, double* scratch, unsigned long long* counters,
+                               hipStream_t st) {
   DevCtx& C = ctx();
-  if (!st) st = C.stream;
-  const FamilyInfo& f = kFam[P->family];
+  const FamilyInfo& f = kFam[fam];
   const double common_base = (M_PI * M_PI * M_PI) * 2.0 / std::sqrt(M_PI);
-  Bin& b = P->bins[i];
-  const KernelInfo& k = variant_info(P, b, b.variant);
   EriArgs a;
-  // (generic bins run the runtime-class kernel below)
   std::memset(&a, 0, sizeof a);
-  a.prim = P->d_rec;
-  a.pair_off = P->d_poff;
-  a.pair_np = P->d_pnp;
-  a.pair_geo = P->d_geo;
+  a.prim = pd.prim;
+  a.pair_off = pd.off;
+  a.pair_np = pd.np;
+  a.pair_geo = pd.geo;
   // one CINTcommon_fac_sp per real shell (g2e.c:54-56, g3c2e.c, g2c2e.c:44-45)
-  a.common = common_base * common_fac_sp(b.l[0]) * (f.two ? 1.0 : common_fac_sp(b.l[1])) *
-             common_fac_sp(b.l[2]) * (f.three ? 1.0 : common_fac_sp(b.l[3]));
-  a.expcutoff = P->expcutoff;
-  a.lij_ceil = b.l[0] + b.l[1] + f.deriv;
-  a.lkl_ceil = b.l[2] + b.l[3];
+  a.common = common_base * common_fac_sp(l[0]) * (f.two ? 1.0 : common_fac_sp(l[1])) *
+             common_fac_sp(l[2]) * (f.three ? 1.0 : common_fac_sp(l[3]));
+  a.expcutoff = expcutoff;
+  a.lij_ceil = l[0] + l[1] + f.deriv;
+  a.lkl_ceil = l[2] + l[3];
   a.three_center = f.three ? 1 : 0;
   a.cart = f.cart ? 1 : 0;
-  int ns0 = nfun(f, b.l[0]), ns1 = nfun(f, b.l[1]), ns2 = nfun(f, b.l[2]);
+  int ns0 = nfun(f, l[0]), ns1 = nfun(f, l[1]), ns2 = nfun(f, l[2]);
   a.st[0] = 1;
   a.st[1] = ns0;
   a.st[2] = (int64_t)ns0 * ns1;
   a.st[3] = (int64_t)ns0 * ns1 * ns2;
-  a.st_comp = b.block / f.ncomp;
+  a.st_comp = block / f.ncomp;
   a.out = out;
   a.rys = C.rys;
   a.c2s = C.d_c2s;
-  a.counters = count ? P->d_counters + 2 * i : nullptr;
+  a.counters = counters;
+  a.scratch = scratch;
   // a dispatch holds at most 2^32 - 1 work-items per grid dimension: launch
   // in chunks of at most 2^31 lanes (2^28 tuples for the runtime-class
   // kernel, whose grid is capped anyway)
-  const int64_t chunk = b.generic ? ((int64_t)1 << 28)
-                                  : (((int64_t)1 << 31) / std::max(1, k.lanes_per_tuple));
-  for (int64_t s = 0; s < (int64_t)b.q.size(); s += chunk) {
-    a.quartets = b.d_q + s;
-    a.out_base = b.out_base + s * b.block;
-    a.block = b.block;
-    a.nq = std::min(chunk, (int64_t)b.q.size() - s);
+  const int64_t chunk = generic ? ((int64_t)1 << 28)
+                                : (((int64_t)1 << 31) / std::max(1, k.lanes_per_tuple));
+  for (int64_t s = 0; s < nq; s += chunk) {
+    a.quartets = d_q + s;
+    a.out_base = out_base + s * block;
+    a.block = block;
+    a.nq = std::min(chunk, nq - s);
     hipError_t e;
     // the launch wrappers read the thread's last HIP error after launching;
     // drop any error an earlier, deliberately unchecked call left behind
     (void)hipGetLastError();
-    if (b.generic) {
-      a.scratch = P->d_scratch + (size_t)chain * (size_t)P->scratch_stride;
+    if (generic) {
       int grid = (int)std::min<int64_t>(a.nq, kGenericGrid);
-      e = launch_generic(a, b.l, f.deriv, grid, st);
+      e = launch_generic(a, l, f.deriv, grid, st);
     } else {
       e = k.fn(a, st);
     }
     if (e != hipSuccess)
       fail(CINTX_STATUS_EXECUTION_FAILED,
-           "kernel launch of class (" + std::to_string(b.l[0]) + std::to_string(b.l[1]) + "|" +
-               std::to_string(b.l[2]) + std::to_string(b.l[3]) + ") " +
-               (b.generic ? std::string("runtime-class kernel")
-                          : "variant " + std::to_string(b.variant) + " (" +
-                                std::to_string(k.lanes_per_tuple) + " lanes, " +
-                                std::to_string(k.tuples_per_block) + " tuples per workgroup)") +
+           "kernel launch of class (" + std::to_string(l[0]) + std::to_string(l[1]) + "|" +
+               std::to_string(l[2]) + std::to_string(l[3]) + ") " +
+               (generic ? std::string("runtime-class kernel")
+                        : "variant " + std::to_string(variant) + " (" +
+                              std::to_string(k.lanes_per_tuple) + " lanes, " +
+                              std::to_string(k.tuples_per_block) + " tuples per workgroup)") +
                ", " + std::to_string(a.nq) + " tuples: " + hipGetErrorString(e));
   }
+}
+
+static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count,
+                            hipStream_t st = nullptr, int chain = 0) {
+  DevCtx& C = ctx();
+  if (!st) st = C.stream;
+  Bin& b = P->bins[i];
+  const KernelInfo& k = variant_info(P, b, b.variant);
+  const PairDev pd{P->d_rec, P->d_poff, P->d_pnp, P->d_geo};
+  double* scratch = b.generic ? P->d_scratch + (size_t)chain * (size_t)P->scratch_stride : nullptr;
+  launch_bin_kernels(P->family, b.l, b.variant, b.generic, k, pd, P->expcutoff, b.d_q,
+                     (int64_t)b.q.size(), out, b.out_base, b.block, scratch,
+                     count ? P->d_counters + 2 * i : nullptr, st);
 }
 
 static void plan_launch_all(cintx_plan* P, double* out, bool count,
@@ -995,53 +1034,258 @@ static int cache_size(int fam, const int* shls, const int* bas) {
   }
 }
 
+// ------------------------------------------------------------------
+// Raw (libcint-signature) calls: one shell tuple per call, any number of
+// host threads at once.  Each thread has its own stream and device arena
+// (created on its first call, kept for the process); the calls take no
+// lock.  With an optimizer the pair data come from its device copy (built
+// once per basis, like CINTOpt's pair data, optimizer.c:344-425); without,
+// the call's own pairs are built and uploaded into the arena.
+// ------------------------------------------------------------------
+struct ThreadCtx {
+  hipStream_t stream = nullptr;
+  char* arena = nullptr;
+  size_t cap = 0;
+  std::vector<char> host;  // upload staging
+};
+static std::mutex g_threads_mu;
+static std::vector<ThreadCtx*> g_threads;  // kept for the process
+static thread_local ThreadCtx* t_ctx = nullptr;
+
+static ThreadCtx& thread_ctx() {
+  if (t_ctx) return *t_ctx;
+  DevCtx& C = ctx();
+  HIPCHK(hipSetDevice(C.device));
+  ThreadCtx* T = new ThreadCtx();
+  if (hipStreamCreateWithFlags(&T->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete T;
+    fail(CINTX_STATUS_EXECUTION_FAILED, "hipStreamCreate failed");
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_threads_mu);
+    g_threads.push_back(T);
+  }
+  t_ctx = T;
+  return *T;
+}
+
+// The thread's arena holds at least `bytes` (grows by doubling; the stream
+// is idle between calls, so the old buffer is free to go).
+static char* arena(ThreadCtx& T, size_t bytes) {
+  if (bytes > T.cap) {
+    size_t cap = std::max<size_t>(bytes, std::max<size_t>(2 * T.cap, (size_t)1 << 20));
+    if (T.arena) HIPCHK(hipFree(T.arena));
+    T.arena = nullptr;
+    T.cap = 0;
+    hipError_t e = hipMalloc(&T.arena, cap);
+    if (e != hipSuccess) fail(CINTX_STATUS_ALLOCATION_FAILED, hipGetErrorString(e));
+    T.cap = cap;
+  }
+  return T.arena;
+}
+
+}  // namespace cx
+
+// Device-resident optimizer: the basis, every pair of (virtual) shells in
+// both orders plus the one-Gaussian pseudo pairs, and their 64-byte
+// primitive records, pair offsets/counts and geometry in one device blob.
+struct CINTOpt {
+  int family = 0;
+  int natm = 0, nbas = 0;
+  cx::HostBasis basis;
+  cx::PairSet pairs;
+  char* blob = nullptr;
+  cx::PairDev dev{};
+};
+
+namespace cx {
+
+// Upload a pair set's device form (records, offsets, counts, geometry) to
+// `dst` (device) through the host staging buffer; returns the pointers.
+static size_t pairs_bytes(const PairSet& S) {
+  return align256(S.aij.size() * 64) + align256(S.off.size() * 4) * 2 +
+         align256(S.off.size() * 64);
+}
+
+static PairDev pairs_upload(const PairSet& S, char* dst, std::vector<char>& h, hipStream_t st) {
+  const size_t npp = S.aij.size(), npair = S.off.size();
+  const size_t o_rec = 0, o_off = align256(npp * 64), o_np = o_off + align256(npair * 4),
+               o_geo = o_np + align256(npair * 4), total = o_geo + align256(npair * 64);
+  h.assign(total, 0);
+  double* rec = (double*)(h.data() + o_rec);
+  for (size_t k = 0; k < npp; ++k) {
+    double* r = rec + 8 * k;
+    r[0] = S.aij[k];
+    r[1] = S.wi[k];
+    r[2] = S.wj[k];
+    r[3] = 0.0;
+    r[4] = S.kfac[k];
+    r[5] = S.hinv[k];
+    r[6] = S.cce[k];
+    r[7] = S.ai[k];
+  }
+  if (npair) {
+    std::memcpy(h.data() + o_off, S.off.data(), npair * 4);
+    std::memcpy(h.data() + o_np, S.np.data(), npair * 4);
+    std::memcpy(h.data() + o_geo, S.geo.data(), npair * 64);
+  }
+  HIPCHK(hipMemcpyAsync(dst, h.data(), total, hipMemcpyHostToDevice, st));
+  return PairDev{(const double*)(dst + o_rec), (const int*)(dst + o_off),
+                 (const int*)(dst + o_np), (const double*)(dst + o_geo)};
+}
+
+static CINTOpt* opt_build(int fam, const int* atm, int natm, const int* bas, int nbas,
+                          const double* env) {
+  DevCtx& C = ctx();
+  CINTOpt* O = new CINTOpt();
+  try {
+    O->family = fam;
+    O->natm = natm;
+    O->nbas = nbas;
+    O->basis = ingest(atm, natm, bas, nbas, env, 0);
+    O->pairs.init(O->basis);
+    const int nv = (int)O->basis.v.size();
+    for (int a = 0; a < nv; ++a) {
+      for (int b = 0; b < nv; ++b) O->pairs.add(O->basis, a, b);
+      O->pairs.add_pseudo(O->basis, a);
+    }
+    HIPCHK(hipMalloc(&O->blob, std::max<size_t>(pairs_bytes(O->pairs), 256)));
+    std::vector<char> h;
+    O->dev = pairs_upload(O->pairs, O->blob, h, C.stream);
+    HIPCHK(hipStreamSynchronize(C.stream));
+  } catch (...) {
+    if (O->blob) (void)hipFree(O->blob);
+    delete O;
+    throw;
+  }
+  return O;
+}
+
 // One libcint-signature call.  Returns not0.
-static int eval_one(int fam, double* out, const int* dims, const int* shls, const int* atm,
-                    int natm, const int* bas, int nbas, const double* env, size_t env_len) {
+static int raw_eval(int fam, double* out, const int* dims, const int* shls, const int* atm,
+                    int natm, const int* bas, int nbas, const double* env, size_t env_len,
+                    const CINTOpt* opt) {
   const FamilyInfo& f = kFam[fam];
   if (!shls) fail(CINTX_STATUS_NULL_POINTER, "shls is null");
-  std::lock_guard<std::mutex> lk(g_mu);
+  ThreadCtx& T = thread_ctx();
+  HostBasis local_basis;
+  PairSet local_pairs;
+  const HostBasis* B;
+  if (opt) {
+    if (opt->natm != natm || opt->nbas != nbas)
+      fail(CINTX_STATUS_INVALID_INPUT, "optimizer was built for another basis");
+    B = &opt->basis;
+  } else {
+    local_basis = ingest(atm, natm, bas, nbas, env, env_len);
+    local_pairs.init(local_basis);
+    B = &local_basis;
+  }
+  int l[4] = {0, 0, 0, 0}, shl[4] = {0, 0, 0, 0};
+  for (int a = 0; a < f.arity; ++a) {
+    if (shls[a] < 0 || shls[a] >= B->nbas) fail(CINTX_STATUS_INVALID_INPUT, "shell index out of range");
+    shl[a] = shls[a];
+    l[a] = B->l[shl[a]];
+  }
+  int perm[4];
+  canon_perm(fam, l, perm);
+  int cl[4], cn[4], cs[4];
+  for (int c = 0; c < 4; ++c) {
+    const int a = perm[c];
+    const bool dummy = a >= f.arity;
+    cl[c] = dummy ? 0 : l[a];
+    cn[c] = dummy ? 1 : B->nctr[shl[a]];
+    cs[c] = dummy ? -1 : shl[a];
+  }
+  // virtual tuples (one per contraction column combination)
+  const int nv = (int)B->v.size();
+  auto pair_id = [&](int va, int vb) {  // vb < 0: pseudo pair of va
+    if (opt) return opt->pairs.find(va, vb < 0 ? nv : vb);
+    return vb < 0 ? local_pairs.add_pseudo(*B, va) : local_pairs.add(*B, va, vb);
+  };
+  std::vector<int2> q;
+  std::vector<std::array<int, 4>> cc;
+  for (int c3 = 0; c3 < cn[3]; ++c3)
+    for (int c2 = 0; c2 < cn[2]; ++c2)
+      for (int c1 = 0; c1 < cn[1]; ++c1)
+        for (int c0 = 0; c0 < cn[0]; ++c0) {
+          const int v0 = B->first_v[cs[0]] + c0;
+          const int bra = f.two ? pair_id(v0, -1) : pair_id(v0, B->first_v[cs[1]] + c1);
+          const int ket = f.three ? pair_id(B->first_v[cs[2]] + c2, -1)
+                                  : pair_id(B->first_v[cs[2]] + c2, B->first_v[cs[3]] + c3);
+          if (bra < 0 || ket < 0) fail(CINTX_STATUS_PANIC, "optimizer lacks a pair");
+          q.push_back(make_int2(bra, ket));
+          cc.push_back({c0, c1, c2, c3});
+        }
+  const int64_t nq = (int64_t)q.size();
+  const int64_t block = (int64_t)f.ncomp * nfun(f, cl[0]) * nfun(f, cl[1]) * nfun(f, cl[2]) *
+                        nfun(f, cl[3]);
+  // kernel of the class: the autotuned or default variant, or the
+  // runtime-class kernel
   DevCtx& C = ctx();
-  cintx_plan* P = plan_new(fam, atm, natm, bas, nbas, env, env_len);
-  struct Guard {
-    cintx_plan* p;
-    ~Guard() { plan_free(p); }
-  } guard{P};
-  int tup[4] = {shls[0], shls[1], f.arity >= 3 ? shls[2] : 0, f.arity == 4 ? shls[3] : 0};
-  plan_add_tuples(P, tup, 1, CINTX_PREC_FP64);
-  plan_check_kernels(P);
-  plan_upload(P);
-  HIPCHK(hipMemsetAsync(P->d_counters, 0, P->bins.size() * 16, C.stream));
-  plan_launch_all(P, P->d_out, true);
-  std::vector<double> canon((size_t)P->total), nat((size_t)P->total);
-  HIPCHK(hipMemcpyAsync(canon.data(), P->d_out, (size_t)P->total * 8, hipMemcpyDeviceToHost,
-                        C.stream));
-  std::vector<unsigned long long> cnt(P->bins.size() * 2);
-  HIPCHK(hipMemcpyAsync(cnt.data(), P->d_counters, cnt.size() * 8, hipMemcpyDeviceToHost,
-                        C.stream));
-  HIPCHK(hipStreamSynchronize(C.stream));
-  plan_to_natural(P, canon.data(), nat.data());
-  unsigned long long surv = 0;
-  for (size_t i = 0; i < P->bins.size(); ++i) surv += cnt[2 * i];
-  const TupleRec& tr = P->tuples[0];
+  const KernelInfo* ks = C.reg->k[f.deriv][f.cart ? 2 : 0][cl[0]][cl[1]][cl[2]][cl[3]];
+  const bool generic = ks[0].fn == nullptr;
+  int variant = 0;
+  if (!generic) {
+    const int v = C.tuned[f.deriv][f.cart ? 2 : 0][cl[0]][cl[1]][cl[2]][cl[3]];
+    if (v > 0 && v < kVariants && ks[v].fn) variant = v;
+  }
+  // arena: [pairs (no optimizer)] [tuples] [output] [counters] [scratch]
+  const size_t pbytes = opt ? 0 : pairs_bytes(local_pairs);
+  const size_t o_q = pbytes, o_out = o_q + align256(nq * sizeof(int2));
+  const size_t o_cnt = o_out + align256(nq * block * 8);
+  const size_t o_scr = o_cnt + 256;
+  const size_t scr = generic ? (size_t)std::min<int64_t>(nq, kGenericGrid) *
+                                   generic_scratch_per_block(cl, f.deriv) * 8
+                             : 0;
+  char* A = arena(T, o_scr + scr);
+  const PairDev pd = opt ? opt->dev : pairs_upload(local_pairs, A, T.host, T.stream);
+  HIPCHK(hipMemcpyAsync(A + o_q, q.data(), nq * sizeof(int2), hipMemcpyHostToDevice, T.stream));
+  HIPCHK(hipMemsetAsync(A + o_cnt, 0, 16, T.stream));
+  double* dout = (double*)(A + o_out);
+  unsigned long long* dcnt = (unsigned long long*)(A + o_cnt);
+  launch_bin_kernels(fam, cl, variant, generic, ks[variant], pd, fam_expcutoff(fam, env),
+                     (const int2*)(A + o_q), nq, dout, 0, block,
+                     generic ? (double*)(A + o_scr) : nullptr, dcnt, T.stream);
+  std::vector<double> canon((size_t)(nq * block));
+  unsigned long long cnt[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(canon.data(), dout, canon.size() * 8, hipMemcpyDeviceToHost, T.stream));
+  HIPCHK(hipMemcpyAsync(cnt, dcnt, 16, hipMemcpyDeviceToHost, T.stream));
+  HIPCHK(hipStreamSynchronize(T.stream));
+  // place the canonical blocks into libcint's layout with the caller's dims
   int n[4] = {1, 1, 1, 1};
-  for (int a = 0; a < f.arity; ++a) n[a] = nfun(f, P->basis.l[tr.shl[a]]) * P->basis.nctr[tr.shl[a]];
+  for (int a = 0; a < f.arity; ++a) n[a] = nfun(f, B->l[shl[a]]) * B->nctr[shl[a]];
   int d[4] = {n[0], n[1], n[2], n[3]};
   if (dims)
     for (int a = 0; a < f.arity; ++a) d[a] = dims[a];
   for (int a = 0; a < f.arity; ++a)
     if (d[a] < n[a]) fail(CINTX_STATUS_INVALID_INPUT, "dims smaller than the shell extents");
-  int64_t nout_d = (int64_t)d[0] * d[1] * d[2] * d[3];
-  int64_t nout_n = (int64_t)n[0] * n[1] * n[2] * n[3];
-  for (int comp = 0; comp < f.ncomp; ++comp)
-    for (int l = 0; l < n[3]; ++l)
-      for (int k = 0; k < n[2]; ++k)
-        for (int j = 0; j < n[1]; ++j)
-          for (int i = 0; i < n[0]; ++i)
-            out[comp * nout_d + i + (int64_t)d[0] * (j + (int64_t)d[1] * (k + (int64_t)d[2] * l))] =
-                surv ? nat[comp * nout_n + i + (int64_t)n[0] * (j + (int64_t)n[1] * (k + (int64_t)n[2] * l))]
-                     : 0.0;
-  return surv ? 1 : 0;
+  const int64_t so[4] = {1, d[0], (int64_t)d[0] * d[1], (int64_t)d[0] * d[1] * d[2]};
+  const int64_t nout_d = (int64_t)d[0] * d[1] * d[2] * d[3];
+  const bool not0 = cnt[0] != 0;
+  int ns[4];
+  int64_t st[4];
+  for (int c = 0; c < 4; ++c) {
+    ns[c] = nfun(f, cl[c]);
+    st[c] = perm[c] >= f.arity ? 0 : so[perm[c]];
+  }
+  const int64_t nsall = (int64_t)ns[0] * ns[1] * ns[2] * ns[3];
+  for (int64_t v = 0; v < nq; ++v) {
+    const double* src = canon.data() + v * block;
+    int base[4];
+    for (int c = 0; c < 4; ++c) base[c] = cc[v][c] * ns[c];
+    for (int comp = 0; comp < f.ncomp; ++comp) {
+      const double* sp = src + comp * nsall;
+      double* dp = out + comp * nout_d;
+      int64_t e = 0;
+      for (int m3 = 0; m3 < ns[3]; ++m3)
+        for (int m2 = 0; m2 < ns[2]; ++m2)
+          for (int m1 = 0; m1 < ns[1]; ++m1)
+            for (int m0 = 0; m0 < ns[0]; ++m0, ++e)
+              dp[(base[0] + m0) * st[0] + (base[1] + m1) * st[1] + (base[2] + m2) * st[2] +
+                 (base[3] + m3) * st[3]] = not0 ? sp[e] : 0.0;
+    }
+  }
+  return not0 ? 1 : 0;
 }
 
 static int family_of_api(int api) {
@@ -1080,20 +1324,37 @@ using namespace cx;
 // ------------------------------------------------------------------
 // 1. libcint raw signature
 // ------------------------------------------------------------------
-struct CINTOpt {
-  int family;
-};
-
+// A failed call leaves the block it would have written zeroed (so the
+// caller's buffer is defined) and returns 0; with CINTX_STRICT=1 in the
+// environment it aborts the process instead, so that a device failure
+// cannot pass for a screened-out block.
 static int raw_entry(int fam, double* out, int* dims, int* shls, int* atm, int natm, int* bas,
-                     int nbas, double* env) {
+                     int nbas, double* env, const CINTOpt* opt) {
   if (!out) {
     if (!shls || !bas) return 0;
     return cache_size(fam, shls, bas);
   }
   int not0 = 0;
-  int st = guarded([&] { not0 = eval_one(fam, out, dims, shls, atm, natm, bas, nbas, env, 0); });
+  int st = guarded(
+      [&] { not0 = raw_eval(fam, out, dims, shls, atm, natm, bas, nbas, env, 0, opt); });
   if (st != CINTX_STATUS_SUCCESS) {
     fprintf(stderr, "libcintx_hip %s: %s\n", kFam[fam].name, t_msg.c_str());
+    const char* strict = getenv("CINTX_STRICT");
+    if (strict && strict[0] == '1') abort();
+    const FamilyInfo& f = kFam[fam];
+    if (shls && bas && nbas > 0) {
+      int64_t n = f.ncomp;
+      bool ok = true;
+      for (int a = 0; a < f.arity; ++a) {
+        if (shls[a] < 0 || shls[a] >= nbas) {
+          ok = false;
+          break;
+        }
+        const int* b = bas + 8 * (size_t)shls[a];
+        n *= dims ? (int64_t)dims[a] : (int64_t)nfun(f, b[1]) * b[3];
+      }
+      if (ok && n > 0) std::memset(out, 0, (size_t)n * sizeof(double));
+    }
     return 0;
   }
   return not0;
@@ -1101,75 +1362,50 @@ static int raw_entry(int fam, double* out, int* dims, int* shls, int* atm, int n
 
 extern "C" {
 
-int int2e_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
-              double* env, CINTOpt* opt, double* cache) {
-  (void)opt;
-  (void)cache;
-  return raw_entry(0, out, dims, shls, atm, natm, bas, nbas, env);
-}
-int int2e_ip1_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
-                  double* env, CINTOpt* opt, double* cache) {
-  (void)opt;
-  (void)cache;
-  return raw_entry(1, out, dims, shls, atm, natm, bas, nbas, env);
-}
-int int3c2e_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
-                double* env, CINTOpt* opt, double* cache) {
-  (void)opt;
-  (void)cache;
-  return raw_entry(2, out, dims, shls, atm, natm, bas, nbas, env);
-}
-int int3c2e_ip1_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
-                    double* env, CINTOpt* opt, double* cache) {
-  (void)opt;
-  (void)cache;
-  return raw_entry(3, out, dims, shls, atm, natm, bas, nbas, env);
-}
+#define CINTX_RAW_ENTRY(name, fam)                                                        \
+  int name(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,     \
+           double* env, CINTOpt* opt, double* cache) {                                    \
+    (void)cache;                                                                          \
+    return raw_entry(fam, out, dims, shls, atm, natm, bas, nbas, env, opt);               \
+  }
+CINTX_RAW_ENTRY(int2e_sph, 0)
+CINTX_RAW_ENTRY(int2e_ip1_sph, 1)
+CINTX_RAW_ENTRY(int3c2e_sph, 2)
+CINTX_RAW_ENTRY(int3c2e_ip1_sph, 3)
+CINTX_RAW_ENTRY(int2c2e_sph, 4)
+CINTX_RAW_ENTRY(int2e_ip2_sph, 5)
+CINTX_RAW_ENTRY(int3c2e_ip2_sph, 6)
+CINTX_RAW_ENTRY(int2e_cart, 7)
+CINTX_RAW_ENTRY(int3c2e_cart, 8)
+#undef CINTX_RAW_ENTRY
 
-int int2c2e_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
-                double* env, CINTOpt* opt, double* cache) {
-  (void)opt;
-  (void)cache;
-  return raw_entry(4, out, dims, shls, atm, natm, bas, nbas, env);
-}
-int int2e_cart(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
-               double* env, CINTOpt* opt, double* cache) {
-  (void)opt;
-  (void)cache;
-  return raw_entry(7, out, dims, shls, atm, natm, bas, nbas, env);
-}
-int int3c2e_cart(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
-                 double* env, CINTOpt* opt, double* cache) {
-  (void)opt;
-  (void)cache;
-  return raw_entry(8, out, dims, shls, atm, natm, bas, nbas, env);
-}
-int int2e_ip2_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
-                  double* env, CINTOpt* opt, double* cache) {
-  (void)opt;
-  (void)cache;
-  return raw_entry(5, out, dims, shls, atm, natm, bas, nbas, env);
-}
-int int3c2e_ip2_sph(double* out, int* dims, int* shls, int* atm, int natm, int* bas, int nbas,
-                    double* env, CINTOpt* opt, double* cache) {
-  (void)opt;
-  (void)cache;
-  return raw_entry(6, out, dims, shls, atm, natm, bas, nbas, env);
-}
-
-static void make_opt(CINTOpt** opt, int fam) {
+// optimizer.c:344-425 builds the pair data of a basis once; so does this,
+// on the device.  A failure leaves *opt NULL (calls then build their pairs
+// per call).
+static void make_opt(CINTOpt** opt, int fam, int* atm, int natm, int* bas, int nbas,
+                     double* env) {
   if (!opt) return;
-  *opt = new CINTOpt{fam};
+  *opt = nullptr;
+  int st = guarded([&] { *opt = opt_build(fam, atm, natm, bas, nbas, env); });
+  if (st != CINTX_STATUS_SUCCESS)
+    fprintf(stderr, "libcintx_hip optimizer: %s\n", t_msg.c_str());
 }
-void int2e_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 0); }
-void int2e_ip1_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 1); }
-void int3c2e_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 2); }
-void int3c2e_ip1_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 3); }
-void int2c2e_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 4); }
-void int2e_ip2_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 5); }
-void int3c2e_ip2_optimizer(CINTOpt** opt, int*, int, int*, int, double*) { make_opt(opt, 6); }
+#define CINTX_OPTIMIZER(name, fam)                                                     \
+  void name(CINTOpt** opt, int* atm, int natm, int* bas, int nbas, double* env) {       \
+    make_opt(opt, fam, atm, natm, bas, nbas, env);                                     \
+  }
+CINTX_OPTIMIZER(int2e_optimizer, 0)
+CINTX_OPTIMIZER(int2e_ip1_optimizer, 1)
+CINTX_OPTIMIZER(int3c2e_optimizer, 2)
+CINTX_OPTIMIZER(int3c2e_ip1_optimizer, 3)
+CINTX_OPTIMIZER(int2c2e_optimizer, 4)
+CINTX_OPTIMIZER(int2e_ip2_optimizer, 5)
+CINTX_OPTIMIZER(int3c2e_ip2_optimizer, 6)
+#undef CINTX_OPTIMIZER
+
 void CINTdel_optimizer(CINTOpt** opt) {
   if (opt && *opt) {
+    if ((*opt)->blob) (void)hipFree((*opt)->blob);
     delete *opt;
     *opt = nullptr;
   }
@@ -1235,7 +1471,6 @@ int cintrs_eval(int api, double* out, size_t out_len, const int* dims, size_t di
                 const int* shls, size_t shls_len, const int* atm, size_t atm_len, const int* bas,
                 size_t bas_len, const double* env, size_t env_len, const void* opt, double* cache,
                 size_t cache_len, CintxEvalSummary* summary_out) {
-  (void)opt;
   int fam = family_of_api(api);
   if (fam < 0) {
     set_error(CINTX_STATUS_UNSUPPORTED_API, "api id " + std::to_string(api) + " is not served");
@@ -1262,8 +1497,8 @@ int cintrs_eval(int api, double* out, size_t out_len, const int* dims, size_t di
     if (cache && cache_len * 8 < ws) fail(CINTX_STATUS_BUFFER_TOO_SMALL, "cache too small");
     // evaluate into a staging buffer first: no partial writes on failure
     std::vector<double> staging((size_t)req);
-    int not0 = eval_one(fam, staging.data(), nullptr, shls, atm, (int)(atm_len / 6), bas,
-                        (int)(bas_len / 8), env, env_len);
+    int not0 = raw_eval(fam, staging.data(), nullptr, shls, atm, (int)(atm_len / 6), bas,
+                        (int)(bas_len / 8), env, env_len, (const CINTOpt*)opt);
     std::memcpy(out, staging.data(), (size_t)req * 8);
     s.not0 = not0;
     s.bytes_written = (size_t)req * 8;

@@ -34,7 +34,16 @@ extern "C" {
 
 /* Opaque optimizer handle; replaces CINTOpt (libcint-master/include/
  * cint.h.in:145-152) and cintx's RawOptimizerHandle
- * (crates/cintx-compat/src/optimizer.rs:15). */
+ * (crates/cintx-compat/src/optimizer.rs:15).  Built once per basis by any
+ * *_optimizer below: the basis and the primitive-pair data of every shell
+ * pair, resident on the device, which every raw call that passes it reuses
+ * (optimizer.c:344-425).  A call without one builds its two pairs itself.
+ *
+ * Threading: any number of host threads may call the raw entry points at
+ * once (sharing an optimizer is fine; it is immutable).  Each thread gets
+ * its own stream and device arena on its first call; calls take no lock.
+ * A call that fails writes zeros into its block and returns 0 (a message
+ * goes to stderr); with CINTX_STRICT=1 in the environment it aborts. */
 typedef struct CINTOpt CINTOpt;
 
 /* libcint-master/src/cint2e.c:1186-1193 */
@@ -129,7 +138,8 @@ int cintrs_query_workspace(int api, const int *dims, size_t dims_len, const int 
                            size_t shls_len, const int *atm, size_t atm_len, const int *bas,
                            size_t bas_len, const double *env, size_t env_len, const void *opt,
                            CintxWorkspaceQuery *query_out);
-/* crates/cintx-capi/src/shim.rs:371-389 */
+/* crates/cintx-capi/src/shim.rs:371-389; opt: NULL or a CINTOpt built for
+ * the same atm/bas (see above) */
 int cintrs_eval(int api, double *out, size_t out_len, const int *dims, size_t dims_len,
                 const int *shls, size_t shls_len, const int *atm, size_t atm_len, const int *bas,
                 size_t bas_len, const double *env, size_t env_len, const void *opt, double *cache,

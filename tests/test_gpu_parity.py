@@ -202,6 +202,68 @@ def test_raw_signature_dims_and_general_contraction(oracle):
         assert_parity(gd, rd)
 
 
+def test_raw_calls_with_optimizers_from_four_threads(oracle):
+    """Four host threads call the libcint-signature entry points at once,
+    sharing one device-resident optimizer per family (the role of CINTOpt's
+    pair data, libcint-master/src/optimizer.c:344-425), with some calls
+    passing none; the calls take no lock and every block equals the judge."""
+    import threading
+    m1 = M.config(1)
+    gc = M.general_contraction_h2()
+    n = m1.nbas
+    rng = np.random.default_rng(31)
+    uniq = [(i, j, k, l) for i in range(n) for j in range(i + 1) for k in range(i + 1)
+            for l in range(k + 1)]
+    opts = {("int2e_sph", 1): capi.Optimizer("int2e_sph", m1.atm, m1.bas, m1.env),
+            ("int2e_ip1_sph", 1): capi.Optimizer("int2e_ip1_sph", m1.atm, m1.bas, m1.env),
+            ("int2e_sph", 0): capi.Optimizer("int2e_sph", gc.atm, gc.bas, gc.env),
+            ("int3c2e_sph", 0): capi.Optimizer("int3c2e_sph", gc.atm, gc.bas, gc.env)}
+    mols = {1: m1, 0: gc}
+    jobs = [("int2e_sph", 1, t, True) for t in uniq]
+    jobs += [("int2e_ip1_sph", 1, tuple(int(v) for v in rng.integers(0, n, 4)), True)
+             for _ in range(800)]
+    jobs += [("int2e_sph", 1, tuple(int(v) for v in rng.integers(0, n, 4)), False)
+             for _ in range(400)]
+    jobs += [("int2e_sph", 0, tuple(int(v) for v in rng.integers(0, gc.nbas, 4)), True)
+             for _ in range(150)]
+    jobs += [("int3c2e_sph", 0, tuple(int(v) for v in rng.integers(0, gc.nbas, 3)), True)
+             for _ in range(150)]
+    order = rng.permutation(len(jobs))
+    jobs = [jobs[i] for i in order]
+    results = [None] * len(jobs)
+    errors = []
+
+    def worker(t):
+        try:
+            for i in range(t, len(jobs), 4):
+                fam, mk, sh, use = jobs[i]
+                mol = mols[mk]
+                results[i] = capi.raw_call(fam, sh, mol.atm, mol.bas, mol.env,
+                                           opt=opts[(fam, mk)] if use else None)
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors[:3]
+    say(f"  {len(jobs)} raw calls from 4 threads")
+    for fam in ("int2e_sph", "int2e_ip1_sph", "int3c2e_sph"):
+        for mk, mol in mols.items():
+            idx = [i for i, j in enumerate(jobs) if j[0] == fam and j[1] == mk]
+            if not idx:
+                continue
+            ref, offs, sizes = oracle.eval_list(fam, [jobs[i][2] for i in idx], mol.atm, mol.bas,
+                                                mol.env, ext=True)
+            got = np.concatenate([results[i][0] for i in idx])
+            say(f"  {fam} on {'cfg1' if mk else 'gc basis'}: {len(idx)} calls")
+            assert_parity(got, ref)
+    for o in opts.values():
+        o.close()
+
+
 def test_cintrs_eval_on_device(oracle):
     mol = M.config(1)
     shls = [5, 3, 4, 8]
