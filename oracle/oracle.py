@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle_cint.so")
 EXT_PATH = os.path.join(HERE, "liboracle_cint_ext.so")
 FMA_PATH = os.path.join(HERE, "liboracle_cint_fma.so")
+CPU_PATH = os.path.join(HERE, "liboracle_cpu.so")
 # build selector of lib()/eval_*(): False the FP64 oracle, True the long
 # double judge, "fma" the FP64 oracle with fused multiply-add contraction
 _PATHS = {False: LIB_PATH, True: EXT_PATH, "fma": FMA_PATH}
@@ -203,3 +204,47 @@ def full_tensor_int2e(atm, bas, env, nthreads=0, ext=False):
         eri[offs[i]:offs[i + 1], offs[j]:offs[j + 1], offs[k]:offs[k + 1],
             offs[l]:offs[l + 1]] = blk
     return eri
+
+
+_cpu = None
+
+
+def cpu_lib():
+    """The CPU baseline (oracle/cpu_simd.cpp: cintx's f64x4 CPU kernel
+    restated), with its own copy of the oracle for its tables."""
+    global _cpu
+    if _cpu is None:
+        if not os.path.exists(CPU_PATH):
+            build()
+        L = ctypes.CDLL(CPU_PATH)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.cpus_int2e_sph.argtypes = [ctypes.c_int, ip, ctypes.POINTER(ctypes.c_long),
+                                     ctypes.POINTER(ctypes.c_double), ip, ctypes.c_int, ip,
+                                     ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.cpus_int2e_sph.restype = ctypes.c_double
+        _cpu = L
+    return _cpu
+
+
+def cpu_baseline_int2e(shls_list, atm, bas, env, nthreads=0, keep=False):
+    """int2e_sph of many quartets on the CPU baseline; returns (out, offsets,
+    sizes) when keep, else the checksum alone."""
+    atm = np.ascontiguousarray(atm, dtype=np.int32)
+    bas = np.ascontiguousarray(bas, dtype=np.int32)
+    env = np.ascontiguousarray(env, dtype=np.float64)
+    sl = np.ascontiguousarray(np.asarray(shls_list, dtype=np.int32).reshape(-1, 4))
+    if keep:
+        sizes = np.array([int(np.prod(tuple_dims("int2e_sph", bas, s))) for s in sl],
+                         dtype=np.int64)
+        offsets = np.zeros(len(sl), dtype=np.int64)
+        if len(sl):
+            offsets[1:] = np.cumsum(sizes)[:-1]
+        out = np.zeros(int(sizes.sum()))
+        chk = cpu_lib().cpus_int2e_sph(len(sl), _ip(sl), offsets.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_long)), _dp(out), _ip(atm), len(atm), _ip(bas), len(bas),
+            _dp(env), nthreads)
+        if np.isnan(chk):
+            raise ValueError("the CPU baseline takes segmented shells with l <= 4")
+        return out, offsets, sizes
+    return cpu_lib().cpus_int2e_sph(len(sl), _ip(sl), None, None, _ip(atm), len(atm), _ip(bas),
+                                    len(bas), _dp(env), nthreads)
