@@ -16,6 +16,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -115,23 +116,31 @@ void ql_first_row(int n, long double* d, long double* e, long double* z0) {
   }
 }
 
+// Gauss-Legendre rules of every order, each built once on first use (shared
+// by the table builder's threads; no per-thread copies)
+const GaussLegendre& gauss_legendre_rule(int m) {
+  static GaussLegendre rules[kGlMax + 1];
+  static std::once_flag built[kGlMax + 1];
+  std::call_once(built[m], [m] { gauss_legendre_01(m, rules[m]); });
+  return rules[m];
+}
+
 }  // namespace
 
 void rys_rule_ld(int n, long double x, long double* s_out, long double* w_out) {
-  static thread_local GaussLegendre gl[kGlMax + 1];
   const long double T = 100.0L + 4.0L * n;
   long double tmax = 1.0L;
   if (x * tmax * tmax > T) tmax = sqrtl(T / x);
   long double xeff = x * tmax * tmax;
   int m = 24 + 2 * n + (int)(1.5L * xeff);
   if (m > kGlMax) m = kGlMax;
-  if (gl[m].m != m) gauss_legendre_01(m, gl[m]);
+  const GaussLegendre& gl = gauss_legendre_rule(m);
   long double s[kGlMax], W[kGlMax], p0[kGlMax], p1[kGlMax];
   long double mu0 = 0;
   for (int k = 0; k < m; ++k) {
-    long double t = gl[m].t[k] * tmax;
+    long double t = gl.t[k] * tmax;
     s[k] = t * t;
-    W[k] = gl[m].w[k] * tmax * expl(-x * s[k]);
+    W[k] = gl.w[k] * tmax * expl(-x * s[k]);
     p0[k] = 0;
     p1[k] = 1;
     mu0 += W[k];

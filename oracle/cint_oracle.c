@@ -125,7 +125,7 @@ void orc_boys(int mmax, double x, double *f)
 #define GL_MAXN 256
 static long double gl_t[GL_MAXN + 1][GL_MAXN];
 static long double gl_w[GL_MAXN + 1][GL_MAXN];
-static volatile int gl_ready[GL_MAXN + 1];
+static int gl_ready[GL_MAXN + 1];
 
 static void gauss_legendre_01(int M, long double *t, long double *w)
 {
@@ -165,13 +165,14 @@ static void gauss_legendre_01(int M, long double *t, long double *w)
 
 static const long double *gl_nodes(int M, const long double **wout)
 {
-    if (!gl_ready[M]) {
+    /* double-checked: the flag is read with acquire and set with release
+     * ordering, so a thread that sees it set also sees the rule */
+    if (!__atomic_load_n(&gl_ready[M], __ATOMIC_ACQUIRE)) {
 #pragma omp critical(orc_gl)
         {
-            if (!gl_ready[M]) {
+            if (!__atomic_load_n(&gl_ready[M], __ATOMIC_RELAXED)) {
                 gauss_legendre_01(M, gl_t[M], gl_w[M]);
-                __sync_synchronize();
-                gl_ready[M] = 1;
+                __atomic_store_n(&gl_ready[M], 1, __ATOMIC_RELEASE);
             }
         }
     }
@@ -344,7 +345,7 @@ static void rys_roots_real(int n, real x, real *u, real *w)
  * Electronic-Structure Theory", eq. 6.4.47-6.4.50) times sqrt((2l+1)/4pi). */
 /* ------------------------------------------------------------------ */
 static real c2s_tab[LMAX_ORC + 1][2 * LMAX_ORC + 1][NCART_MAX];
-static volatile int c2s_ready = 0;
+static int c2s_ready = 0;
 
 static long double binoml(int n, int k)
 {
@@ -405,13 +406,12 @@ static void build_c2s(void)
 
 static void ensure_c2s(void)
 {
-    if (!c2s_ready) {
+    if (!__atomic_load_n(&c2s_ready, __ATOMIC_ACQUIRE)) {
 #pragma omp critical(orc_c2s)
         {
-            if (!c2s_ready) {
+            if (!__atomic_load_n(&c2s_ready, __ATOMIC_RELAXED)) {
                 build_c2s();
-                __sync_synchronize();
-                c2s_ready = 1;
+                __atomic_store_n(&c2s_ready, 1, __ATOMIC_RELEASE);
             }
         }
     }
