@@ -288,11 +288,11 @@ def secondary_configs():
     trip[:, 2] = np.tile(aux, len(ii))
     run("int3c2e_sph_aux_lattice", capi.Plan("int3c2e_sph", m4.atm, m4.bas, m4.env, trip),
         "int3c2e_sph", m4, "triples/s")
-    # an explicit list has no Schwarz bounds, so the mixed plan runs every
-    # triple with the FP32 recurrences: its error is not held to the 1e-10
-    # of the screened int2e mixed path
-    run("int3c2e_sph_aux_lattice_all_fp32",
-        capi.Plan("int3c2e_sph", m4.atm, m4.bas, m4.env, trip, precision=capi.PREC_MIXED),
+    # mixed: a triple takes the FP32 recurrences when its Schwarz bound over
+    # the absolute coefficients is below 1e-9
+    run("int3c2e_sph_aux_lattice_mixed",
+        capi.Plan("int3c2e_sph", m4.atm, m4.bas, m4.env, trip, precision=capi.PREC_MIXED,
+                  mixed_threshold=1e-9),
         "int3c2e_sph", m4, "triples/s")
     m5 = molecule.config(5)
     p5 = capi.Plan("int2e_sph", m5.atm, m5.bas, m5.env, screened=(1e-12, 64))

@@ -555,9 +555,13 @@ static double fam_expcutoff(int fam, const double* env) {
   return c;
 }
 
-static void plan_add_tuples(cintx_plan* P, const int* tuples, int64_t nt, int prec) {
+// Add nt caller tuples (4 ints each) in order; precision `prec` for all, or
+// per tuple from prec_of when that is given.
+static void plan_add_tuples(cintx_plan* P, const int* tuples, int64_t nt, int prec,
+                            const uint8_t* prec_of = nullptr) {
   const FamilyInfo& f = kFam[P->family];
-  if (prec != 0 && f.cart) fail(CINTX_STATUS_INVALID_INPUT, "Cartesian families run in FP64 only");
+  if ((prec != 0 || prec_of) && f.cart)
+    fail(CINTX_STATUS_INVALID_INPUT, "Cartesian families run in FP64 only");
   const HostBasis& B = P->basis;
   std::unordered_map<uint64_t, int> bin_of;
   for (int64_t t = 0; t < nt; ++t) {
@@ -584,15 +588,16 @@ static void plan_add_tuples(cintx_plan* P, const int* tuples, int64_t nt, int pr
     tr.nat_off = P->total;
     tr.nat_size = nat_size(P->family, B, tr.shl);
     P->total += tr.nat_size;
+    const int tprec = prec_of ? prec_of[t] : prec;
     uint64_t key = 0;
     for (int c = 0; c < 4; ++c) key = key * 8 + cl[c];
-    key = key * 2 + prec;
+    key = key * 2 + tprec;
     int bi;
     auto it = bin_of.find(key);
     if (it == bin_of.end()) {
       Bin b;
       for (int c = 0; c < 4; ++c) b.l[c] = cl[c];
-      b.prec = prec;
+      b.prec = tprec;
       b.block = (int64_t)f.ncomp * nfun(f, cl[0]) * nfun(f, cl[1]) * nfun(f, cl[2]) * nfun(f, cl[3]);
       bi = (int)P->bins.size();
       P->bins.push_back(b);
@@ -1521,19 +1526,134 @@ size_t cintrs_copy_last_error_message(char* out, size_t out_len) {
 // ------------------------------------------------------------------
 // 3. Plans
 // ------------------------------------------------------------------
+// sqrt(max |block|) of each diagonal tuple (4 ints each; family 0 for
+// (ij|ij), family 4 for (P|P)), evaluated on the device with the
+// coefficients of env_use.
+static std::vector<double> diag_schwarz(int fam, const int* atm, int natm, const int* bas,
+                                        int nbas, const double* env_use,
+                                        const std::vector<int>& diag) {
+  const int64_t n = (int64_t)diag.size() / 4;
+  std::vector<double> qv((size_t)n, 0.0);
+  if (n == 0) return qv;
+  cintx_plan* D = plan_new(fam, atm, natm, bas, nbas, env_use, 0);
+  struct G {
+    cintx_plan* p;
+    ~G() { plan_free(p); }
+  } g{D};
+  plan_add_tuples(D, diag.data(), n, 0);
+  plan_check_kernels(D);
+  plan_upload(D);
+  DevCtx& C = ctx();
+  plan_launch_all(D, D->d_out, false);
+  std::vector<double> canon((size_t)D->total);
+  HIPCHK(hipMemcpyAsync(canon.data(), D->d_out, (size_t)D->total * 8, hipMemcpyDeviceToHost,
+                        C.stream));
+  HIPCHK(hipStreamSynchronize(C.stream));
+  for (int64_t t = 0; t < n; ++t) {
+    const TupleRec& tr = D->tuples[t];
+    double m = 0;
+    for (int64_t k = 0; k < tr.nvq; ++k) {
+      const VQ& vq = D->vqs[tr.first_vq + k];
+      const Bin& b = D->bins[vq.bin];
+      const double* src = canon.data() + b.out_base + vq.idx * b.block;
+      for (int64_t e = 0; e < b.block; ++e) m = std::max(m, std::fabs(src[e]));
+    }
+    qv[t] = std::sqrt(m);
+  }
+  return qv;
+}
+
+// env with every contraction coefficient replaced by its absolute value
+// (the Schwarz bound then bounds the size of the terms summed, which is what
+// FP32 rounding follows, not the size of their possibly cancelled sum)
+static std::vector<double> abs_coef_env(const int* atm, const int* bas, int nbas,
+                                        const double* env) {
+  size_t len = 0;
+  for (int s = 0; s < nbas; ++s) {
+    const int* b = bas + 8 * s;
+    len = std::max(len, (size_t)b[5] + b[2]);
+    len = std::max(len, (size_t)b[6] + (size_t)b[2] * b[3]);
+    len = std::max(len, (size_t)atm[6 * b[0] + 1] + 3);
+  }
+  std::vector<double> eabs(env, env + len);
+  for (int s = 0; s < nbas; ++s) {
+    const int* b = bas + 8 * s;
+    for (int k = 0; k < b[2] * b[3]; ++k)
+      eabs[(size_t)b[6] + k] = std::fabs(eabs[(size_t)b[6] + k]);
+  }
+  return eabs;
+}
+
+// Precision of each tuple of an explicit int2e_sph / int3c2e_sph list in a
+// mixed plan: FP32 recurrences when the Schwarz bound over the absolute
+// coefficients, sqrt((ij|ij)) sqrt((kl|kl)) or sqrt((ij|ij)) sqrt((P|P)),
+// is below mixed_threshold.
+static std::vector<uint8_t> mixed_selection(int family, const int* atm, int natm,
+                                            const int* bas, int nbas, const double* env,
+                                            const int* tuples, int64_t nt, double thr) {
+  const std::vector<double> eabs = abs_coef_env(atm, bas, nbas, env);
+  std::unordered_map<uint64_t, int> pid, aid;
+  std::vector<int> dpair, daux;
+  auto pair_of = [&](int a, int b) {
+    if (a < b) std::swap(a, b);
+    if (a < 0 || b < 0 || a >= nbas) fail(CINTX_STATUS_INVALID_INPUT, "shell index out of range");
+    const uint64_t key = ((uint64_t)a << 32) | (uint32_t)b;
+    auto it = pid.find(key);
+    if (it != pid.end()) return it->second;
+    const int id = (int)pid.size();
+    pid.emplace(key, id);
+    dpair.insert(dpair.end(), {a, b, a, b});
+    return id;
+  };
+  auto aux_of = [&](int k) {
+    if (k < 0 || k >= nbas) fail(CINTX_STATUS_INVALID_INPUT, "shell index out of range");
+    auto it = aid.find((uint64_t)k);
+    if (it != aid.end()) return it->second;
+    const int id = (int)aid.size();
+    aid.emplace((uint64_t)k, id);
+    daux.insert(daux.end(), {k, k, 0, 0});
+    return id;
+  };
+  std::vector<int2> ref((size_t)nt);
+  for (int64_t t = 0; t < nt; ++t) {
+    const int* sh = tuples + 4 * t;
+    ref[(size_t)t].x = pair_of(sh[0], sh[1]);
+    ref[(size_t)t].y = family == 0 ? pair_of(sh[2], sh[3]) : aux_of(sh[2]);
+  }
+  const std::vector<double> qp = diag_schwarz(0, atm, natm, bas, nbas, eabs.data(), dpair);
+  const std::vector<double> qa = diag_schwarz(4, atm, natm, bas, nbas, eabs.data(), daux);
+  std::vector<uint8_t> prec((size_t)nt);
+  for (int64_t t = 0; t < nt; ++t) {
+    const double b = qp[(size_t)ref[(size_t)t].x] *
+                     (family == 0 ? qp[(size_t)ref[(size_t)t].y] : qa[(size_t)ref[(size_t)t].y]);
+    prec[(size_t)t] = b < thr ? 1 : 0;
+  }
+  return prec;
+}
+
 int cintx_plan_create(cintx_plan** plan, int family, const int* atm, int natm, const int* bas,
                       int nbas, const double* env, const int* tuples, int64_t ntuples,
                       int precision, double mixed_threshold) {
-  (void)mixed_threshold;
   return guarded([&] {
     if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
     if (!tuples && ntuples > 0) fail(CINTX_STATUS_NULL_POINTER, "tuples is null");
     if (precision != CINTX_PREC_FP64 && precision != CINTX_PREC_MIXED)
       fail(CINTX_STATUS_INVALID_INPUT, "unknown precision mode");
+    if (precision == CINTX_PREC_MIXED && family != CINTX_FAMILY_INT2E_SPH &&
+        family != CINTX_FAMILY_INT3C2E_SPH)
+      fail(CINTX_STATUS_INVALID_INPUT, "the mixed mode serves int2e_sph and int3c2e_sph");
     cintx_plan* P = plan_new(family, atm, natm, bas, nbas, env, 0);
     try {
-      plan_add_tuples(P, tuples, ntuples, precision == CINTX_PREC_MIXED ? 1 : 0);
-      P->ntuples_fp32 = precision == CINTX_PREC_MIXED ? (int64_t)P->tuples.size() : 0;
+      if (precision == CINTX_PREC_MIXED) {
+        const std::vector<uint8_t> prec = mixed_selection(family, atm, natm, bas, nbas, env,
+                                                          tuples, ntuples, mixed_threshold);
+        plan_add_tuples(P, tuples, ntuples, 0, prec.data());
+        P->ntuples_fp32 = 0;
+        for (uint8_t v : prec) P->ntuples_fp32 += v;
+      } else {
+        plan_add_tuples(P, tuples, ntuples, 0);
+        P->ntuples_fp32 = 0;
+      }
       plan_check_kernels(P);
       plan_upload(P);
     } catch (...) {

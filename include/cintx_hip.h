@@ -172,9 +172,12 @@ typedef struct cintx_plan cintx_plan;
  * ignored).  Block t of the output holds the tuple's libcint-layout result
  * (natural dims, components leading) at element offset offsets[t]
  * (cintx_plan_offsets).  Replaces cintx-cubecl's per-tuple launch
- * (crates/cintx-cubecl/src/kernels/two_electron.rs:1309-1430).  No Schwarz
- * bound is computed for an explicit list: CINTX_PREC_MIXED runs every tuple
- * of it with the FP32 recurrences and mixed_threshold is ignored. */
+ * (crates/cintx-cubecl/src/kernels/two_electron.rs:1309-1430).
+ * CINTX_PREC_MIXED (int2e_sph and int3c2e_sph only) runs a tuple with the
+ * FP32 recurrences when its Schwarz bound over the absolute contraction
+ * coefficients, sqrt((ij|ij)) sqrt((kl|kl)) or sqrt((ij|ij)) sqrt((P|P)),
+ * is below mixed_threshold (computed on the device at creation); the output
+ * order stays the caller's. */
 int cintx_plan_create(cintx_plan **plan, int family, const int *atm, int natm, const int *bas,
                       int nbas, const double *env, const int *tuples, int64_t ntuples,
                       int precision, double mixed_threshold);

@@ -366,4 +366,74 @@ double cpus_int2e_sph(int nq, const int* shls, const long* offsets, double* out,
   return total;
 }
 
+// J_pq = sum_rs (pq|rs) D_rs and K_pr = sum_qs (pq|rs) D_qs (D: nao x nao
+// row-major, taken as given) from nq unique quartets (each unordered shell
+// quartet once): every distinct image of a quartet under the 8-fold
+// permutational symmetry is added in full.  J, K: nao x nao, overwritten.
+// Returns 0, or -1 on unsupported input.
+int cpus_int2e_jk(int nq, const int* shls, const int* atm, int natm, const int* bas, int nbas,
+                  const double* env, const double* D, double* J, double* K, int nthreads) {
+  (void)natm;
+  std::call_once(g_once, build_tables);
+  for (int s = 0; s < nbas; ++s)
+    if (bas[8 * s + 3] != 1 || bas[8 * s + 1] > kMaxL) return -1;
+  std::vector<int> ao(nbas + 1, 0);
+  for (int s = 0; s < nbas; ++s) ao[s + 1] = ao[s] + 2 * bas[8 * s + 1] + 1;
+  const int nao = ao[nbas];
+  const size_t nn = (size_t)nao * nao;
+  for (size_t e = 0; e < nn; ++e) J[e] = K[e] = 0.0;
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+  {
+    Work W;
+    std::vector<double> blk(6561), jl(nn, 0.0), kl(nn, 0.0);
+#pragma omp for schedule(dynamic, 16)
+    for (int q = 0; q < nq; ++q) {
+      const int* sh = shls + 4 * q;
+      Shell S[4];
+      for (int a = 0; a < 4; ++a) S[a] = get_shell(sh[a], atm, bas, env);
+      quartet(S, blk.data(), W);
+      int n[4], st[4];
+      for (int a = 0; a < 4; ++a) n[a] = 2 * S[a].l + 1;
+      st[0] = 1;
+      st[1] = n[0];
+      st[2] = n[0] * n[1];
+      st[3] = n[0] * n[1] * n[2];
+      // the images (p q|r s) as permutations of the slots (0 1|2 3)
+      static const int perm8[8][4] = {{0, 1, 2, 3}, {1, 0, 2, 3}, {0, 1, 3, 2}, {1, 0, 3, 2},
+                                      {2, 3, 0, 1}, {3, 2, 0, 1}, {2, 3, 1, 0}, {3, 2, 1, 0}};
+      int seen[8][4], nseen = 0;
+      for (int g = 0; g < 8; ++g) {
+        int img[4];
+        for (int a = 0; a < 4; ++a) img[a] = sh[perm8[g][a]];
+        bool dup = false;
+        for (int u = 0; u < nseen && !dup; ++u)
+          dup = seen[u][0] == img[0] && seen[u][1] == img[1] && seen[u][2] == img[2] &&
+                seen[u][3] == img[3];
+        if (dup) continue;
+        for (int a = 0; a < 4; ++a) seen[nseen][a] = img[a];
+        ++nseen;
+        const int* pm = perm8[g];
+        const int np = n[pm[0]], nq_ = n[pm[1]], nr = n[pm[2]], ns = n[pm[3]];
+        const int op = ao[img[0]], oq = ao[img[1]], orr = ao[img[2]], os = ao[img[3]];
+        for (int ms = 0; ms < ns; ++ms)
+          for (int mr = 0; mr < nr; ++mr)
+            for (int mq = 0; mq < nq_; ++mq)
+              for (int mp = 0; mp < np; ++mp) {
+                const double v = blk[mp * st[pm[0]] + mq * st[pm[1]] + mr * st[pm[2]] +
+                                     ms * st[pm[3]]];
+                jl[(size_t)(op + mp) * nao + oq + mq] += v * D[(size_t)(orr + mr) * nao + os + ms];
+                kl[(size_t)(op + mp) * nao + orr + mr] += v * D[(size_t)(oq + mq) * nao + os + ms];
+              }
+      }
+    }
+#pragma omp critical(cpus_jk)
+    for (size_t e = 0; e < nn; ++e) {
+      J[e] += jl[e];
+      K[e] += kl[e];
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -222,6 +222,9 @@ def cpu_lib():
                                      ctypes.POINTER(ctypes.c_double), ip, ctypes.c_int, ip,
                                      ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.cpus_int2e_sph.restype = ctypes.c_double
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.cpus_int2e_jk.argtypes = [ctypes.c_int, ip, ip, ctypes.c_int, ip, ctypes.c_int, dp, dp,
+                                    dp, dp, ctypes.c_int]
         _cpu = L
     return _cpu
 
@@ -248,3 +251,21 @@ def cpu_baseline_int2e(shls_list, atm, bas, env, nthreads=0, keep=False):
         return out, offsets, sizes
     return cpu_lib().cpus_int2e_sph(len(sl), _ip(sl), None, None, _ip(atm), len(atm), _ip(bas),
                                     len(bas), _dp(env), nthreads)
+
+
+def cpu_baseline_jk(shls_list, atm, bas, env, D, nthreads=0):
+    """J and K of a unique-quartet list on the CPU baseline (every distinct
+    8-fold image of each quartet added): an independent check of the device
+    J/K digestion."""
+    atm = np.ascontiguousarray(atm, dtype=np.int32)
+    bas = np.ascontiguousarray(bas, dtype=np.int32)
+    env = np.ascontiguousarray(env, dtype=np.float64)
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    sl = np.ascontiguousarray(np.asarray(shls_list, dtype=np.int32).reshape(-1, 4))
+    J = np.zeros_like(D)
+    K = np.zeros_like(D)
+    st = cpu_lib().cpus_int2e_jk(len(sl), _ip(sl), _ip(atm), len(atm), _ip(bas), len(bas),
+                                 _dp(env), _dp(D), _dp(J), _dp(K), nthreads)
+    if st != 0:
+        raise ValueError("the CPU baseline takes segmented shells with l <= 4")
+    return J, K

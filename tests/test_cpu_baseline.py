@@ -36,3 +36,22 @@ def test_cpu_baseline_refuses_general_contractions(oracle):
     mol = M.general_contraction_h2()
     with pytest.raises(ValueError):
         oracle.cpu_baseline_int2e([(0, 1, 2, 3)], mol.atm, mol.bas, mol.env, keep=True)
+
+
+def test_cpu_jk_equals_the_full_tensor_contraction(oracle):
+    """The CPU J/K from unique quartets (each distinct 8-fold image added)
+    equals the contraction of the judge's full cfg-1 tensor with D."""
+    mol = M.config(1)
+    n = mol.nbas
+    t = [(i, j, k, l) for i in range(n) for j in range(i + 1) for k in range(i + 1)
+         for l in range(k + 1) if i * (i + 1) // 2 + j >= k * (k + 1) // 2 + l]
+    eri = oracle.full_tensor_int2e(mol.atm, mol.bas, mol.env, ext=True)
+    nao = eri.shape[0]
+    rng = np.random.default_rng(11)
+    D = rng.normal(size=(nao, nao)) / nao
+    D = D + D.T
+    Jr = np.einsum("pqrs,rs->pq", eri, D)
+    Kr = np.einsum("pqrs,qs->pr", eri, D)
+    J, K = oracle.cpu_baseline_jk(t, mol.atm, mol.bas, mol.env, D, nthreads=4)
+    assert np.abs(J - Jr).max() <= 1e-11 * np.abs(Jr).max()
+    assert np.abs(K - Kr).max() <= 1e-11 * np.abs(Kr).max()

@@ -579,6 +579,67 @@ def test_mixed_plan_error_bound_cfg2(oracle):
     p.close()
 
 
+def test_mixed_plan_cfg3_256_atoms(oracle):
+    """The 256-atom lattice (the headline molecule) in the mixed mode at the
+    bench's thresholds, every 16384th bra pair: tuples sampled from the FP32
+    and FP64 bins within 1e-10 of the judge, and the plan's J/K against the CPU
+    J/K of the same unique quartets (every distinct 8-fold image added, an
+    independent digestion)."""
+    mol = M.config(3)
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, screened=(1e-12, 16384),
+                  precision=capi.PREC_MIXED, mixed_threshold=1e-9)
+    say(f"  cfg3 mixed: {p.ntuples} quartets, {p.ntuples_fp32} in FP32")
+    assert p.ntuples_fp32 > 0
+    p.execute()
+    p.sync()
+    t, _ = p.tuples()
+    rng = np.random.default_rng(12)
+    first32 = p.ntuples - p.ntuples_fp32
+    pick = np.sort(np.concatenate([rng.choice(first32, size=min(1000, first32), replace=False),
+                                   first32 + rng.choice(p.ntuples_fp32,
+                                                        size=min(1000, p.ntuples_fp32),
+                                                        replace=False)]))
+    got, _, _ = p.output_tuples(pick)
+    ref, _, _ = oracle.eval_list("int2e_sph", [tuple(int(v) for v in t[q]) for q in pick],
+                                 mol.atm, mol.bas, mol.env, ext=True)
+    err = float(np.abs(got - ref).max())
+    say(f"  sampled max abs err vs judge {err:.3e}")
+    assert err <= 1e-10
+    nao = mol.nao()
+    D = rng.normal(size=(nao, nao)) / nao
+    D = D + D.T
+    J, K = p.jk(D)
+    Jc, Kc = oracle.cpu_baseline_jk(t, mol.atm, mol.bas, mol.env, D, nthreads=16)
+    ej, ek = float(np.abs(J - Jc).max()), float(np.abs(K - Kc).max())
+    say(f"  J/K vs CPU: {ej:.3e} {ek:.3e} (|J| {np.abs(Jc).max():.3e})")
+    assert ej <= 1e-10 and ek <= 1e-10
+    p.close()
+
+
+def test_int3c2e_mixed_cfg4(oracle):
+    """int3c2e_sph in the mixed mode on the 64-atom lattice with the
+    6s5p4d3f1g auxiliary basis: a triple takes the FP32 recurrences when
+    sqrt((ij|ij)) sqrt((P|P)) over the absolute coefficients is below 1e-9;
+    every triple within 1e-10 of the judge."""
+    mol = M.config(4)
+    rng = np.random.default_rng(41)
+    a0 = mol.aux_start
+    tuples = [(int(rng.integers(0, a0)), int(rng.integers(0, a0)),
+               int(rng.integers(a0, mol.nbas))) for _ in range(4000)]
+    p = capi.Plan("int3c2e_sph", mol.atm, mol.bas, mol.env, tuples, precision=capi.PREC_MIXED,
+                  mixed_threshold=1e-9)
+    say(f"  int3c2e mixed: {p.ntuples_fp32} of {p.ntuples} triples in FP32")
+    assert 0 < p.ntuples_fp32 < p.ntuples
+    p.execute()
+    p.sync()
+    got = p.output()
+    ref, _, _ = oracle.eval_list("int3c2e_sph", tuples, mol.atm, mol.bas, mol.env, ext=True)
+    err = float(np.abs(got - ref).max())
+    say(f"  max abs err vs judge {err:.3e}")
+    assert err <= 1e-10
+    p.close()
+
+
 def test_autotune_keeps_results(oracle):
     mol = M.config(1)
     n = mol.nbas
