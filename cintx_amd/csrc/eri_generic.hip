@@ -123,23 +123,32 @@ __device__ void generic_ket(double* __restrict__ g, const GenShape& S, int axis,
 }
 
 // GLDS: the g tensor in (dynamic) LDS instead of the global slice.
-// Every lane keeps ACC elements of the current element chunk (t + 256 k)
-// and their g offsets in registers (two workgroups per CU; one for nabla_i,
-// whose three components per element need the larger register budget).  A
-// block larger than one chunk (256 ACC Cartesian elements: (ff|ff), (gg|..))
-// is done chunk by chunk, each chunk a full pass over the primitive tuples
-// with the roots and chains recomputed, so that the accumulators stay in
-// registers instead of taking a global read-modify-write per element and
-// primitive tuple.
-constexpr int kGenAcc = 8;
+// Row-blocked gout: a lane owns RPL rows of the block, a row being the
+// ncart(LI) elements that share their j, k, l components (LI, the canonical
+// i shell's l, is a template parameter so that the row's elements index
+// registers).  Per root the lane loads the LI + 1 (nabla_i: LI + 2) g values
+// of each axis once and forms every element of the row from them: about one
+// LDS read per element instead of three.  The accumulators stay in
+// registers (two workgroups per CU; one for nabla_i, whose three components
+// per element need the larger register budget); a block with more rows than
+// one chunk (256 RPL rows) is done chunk by chunk, each chunk a full pass
+// over the primitive tuples with the roots and chains recomputed.
 constexpr size_t kGenLdsBytes = 150 * 1024;  // dynamic LDS for g (gfx950: 160 KB per CU)
 
-template <bool GLDS, bool DERIV>
+// rows per lane: about 16 accumulators (nabla_i: 45, with twice the registers)
+__host__ __device__ constexpr int gen_rows_per_lane(int li, bool deriv) {
+  return deriv ? cmax(1, 15 / ncart(li)) : cmax(1, 16 / ncart(li));
+}
+
+template <bool GLDS, bool DERIV, int LI>
 __global__ void __launch_bounds__(256, (DERIV ? 1 : 2)) eri_generic_kernel(EriArgs a, GenShape S,
                                                           int64_t per_block) {
   constexpr int NC = DERIV ? 3 : 1;
-  constexpr int ACC = kGenAcc;  // elements per lane
-  constexpr int CHUNK = 256 * ACC;
+  constexpr int NCI = ncart(LI);                      // elements per row
+  constexpr int RPL = gen_rows_per_lane(LI, DERIV);   // rows per lane
+  constexpr int NLD = LI + 1 + (DERIV ? 1 : 0);       // i powers loaded per axis
+  constexpr int ACC = RPL * NCI;                      // elements per lane
+  constexpr int CHUNK = 256 * RPL;                    // rows per chunk
   extern __shared__ double gl[];
   const int t = threadIdx.x;
   const int T = blockDim.x;
@@ -157,13 +166,10 @@ __global__ void __launch_bounds__(256, (DERIV ? 1 : 2)) eri_generic_kernel(EriAr
   const int ket_tasks = (S.l[2] > 0 && S.l[3] > 0) ? 3 * IJ * NR : 0;
   const int NM = S.l[2] + S.l[3] + 1;  // VRR columns
   const int bra_tasks = 3 * NM * NR;
-  // this lane's elements: offsets (in doubles) of their x, y, z g entries;
-  // for nabla_i also the i-1 entries (the element's own when the i power is
-  // 0) and the i powers as multipliers
-  constexpr int ND = DERIV ? ACC : 1;
-  int ox[ACC], oy[ACC], oz[ACC];
-  int lx[ND], ly[ND], lz[ND];
-  double mx[ND], my[ND], mz[ND];
+  // this lane's rows: offsets (in doubles) of their x, y, z g entries with
+  // i power 0
+  const int nrow = nfj * nfk * nfl;
+  int ox[RPL], oy[RPL], oz[RPL];
   double racc[NC * ACC];
   for (int64_t q = blockIdx.x; q < a.nq; q += gridDim.x) {
     const int2 pq = a.quartets[q];
@@ -181,35 +187,25 @@ __global__ void __launch_bounds__(256, (DERIV ? 1 : 2)) eri_generic_kernel(EriAr
     if (t == 0) cnt[0] = 0;
     const double logd_ij = gb[6], logd_kl = gk[6];
     const int np_ij = a.pair_np[pq.x], np_kl = a.pair_np[pq.y];
-    for (int e0 = 0; e0 < S.nf; e0 += CHUNK) {
+    for (int r0 = 0; r0 < nrow; r0 += CHUNK) {
 #pragma unroll
-      for (int k = 0; k < ACC; ++k) {
-        const int e = e0 + t + k * 256;
-        int px = 0, py = 0, pz = 0, ix = 0, iy = 0, iz = 0;
-        if (e < S.nf) {  // elements past the block read entry 0 and are not stored
-          int fi = e % nfi, e1 = e / nfi;
-          int fj = e1 % nfj, e2 = e1 / nfj;
-          int fk = e2 % nfk, fl = e2 / nfk;
+      for (int k = 0; k < RPL; ++k) {
+        const int row = r0 + t + k * 256;
+        int px = 0, py = 0, pz = 0;
+        if (row < nrow) {  // rows past the block read entry 0 and are not stored
+          const int fj = row % nfj, e2 = row / nfj;
+          const int fk = e2 % nfk, fl = e2 / nfk;
           int jx, jy, jz, kx, ky, kz, lx_, ly_, lz_;
-          cart_decode(S.l[0], fi, ix, iy, iz);
           cart_decode(S.l[1], fj, jx, jy, jz);
           cart_decode(S.l[2], fk, kx, ky, kz);
           cart_decode(S.l[3], fl, lx_, ly_, lz_);
-          px = ((ix * S.dj + jx) * S.dk + kx) * S.dl + lx_;
-          py = ((iy * S.dj + jy) * S.dk + ky) * S.dl + ly_;
-          pz = ((iz * S.dj + jz) * S.dk + kz) * S.dl + lz_;
+          px = (jx * S.dk + kx) * S.dl + lx_;
+          py = (jy * S.dk + ky) * S.dl + ly_;
+          pz = (jz * S.dk + kz) * S.dl + lz_;
         }
         ox[k] = px * NR;
         oy[k] = (S.g4 + py) * NR;
         oz[k] = (2 * S.g4 + pz) * NR;
-        if constexpr (DERIV) {
-          lx[k] = ix > 0 ? ox[k] - SI * NR : ox[k];
-          ly[k] = iy > 0 ? oy[k] - SI * NR : oy[k];
-          lz[k] = iz > 0 ? oz[k] - SI * NR : oz[k];
-          mx[k] = ix;
-          my[k] = iy;
-          mz[k] = iz;
-        }
       }
 #pragma unroll
       for (int k = 0; k < NC * ACC; ++k) racc[k] = 0.0;
@@ -265,7 +261,7 @@ __global__ void __launch_bounds__(256, (DERIV ? 1 : 2)) eri_generic_kernel(EriAr
             uw[NR + t] = w * fac1;
             uw[2 * NR + t] = qr;
           }
-          if (t == 0 && e0 == 0) ++cnt[0];
+          if (t == 0 && r0 == 0) ++cnt[0];
           __syncthreads();
           for (int c = t; c < 3 * NR; c += T) {
             const int axis = c / NR, r = c - axis * NR;
@@ -287,55 +283,56 @@ __global__ void __launch_bounds__(256, (DERIV ? 1 : 2)) eri_generic_kernel(EriAr
             }
             __syncthreads();
           }
-          // roots outer, elements inner: the ACC elements' LDS reads of one
-          // root are independent, so they overlap instead of each element's
-          // root loop waiting on its own reads; per element the sum over the
-          // roots is formed first, as before
+          // roots outer, rows inner: a row's g values of one root are loaded
+          // once (the loads of the lane's rows are independent, so they
+          // overlap) and every element of the row is formed from them
           const double ai2 = -2.0 * pi[7];
-          const int up = SI * NR;
-          double s0[ACC], s1[ND], s2[ND];
-#pragma unroll
-          for (int k = 0; k < ACC; ++k) s0[k] = 0.0;
-#pragma unroll
-          for (int k = 0; k < ND; ++k) {
-            s1[k] = 0.0;
-            s2[k] = 0.0;
-          }
+          const int up = SI * NR;  // stride of the i power
           for (int r = 0; r < NR; ++r) {
 #pragma unroll
-            for (int k = 0; k < ACC; ++k) {
-              const double gx = g[ox[k] + r], gy = g[oy[k] + r], gz = g[oz[k] + r];
-              if constexpr (!DERIV) {
-                s0[k] += gx * gy * gz;
-              } else {
-                const double fx = ai2 * g[ox[k] + up + r] + mx[k] * g[lx[k] + r];
-                const double fy = ai2 * g[oy[k] + up + r] + my[k] * g[ly[k] + r];
-                const double fz = ai2 * g[oz[k] + up + r] + mz[k] * g[lz[k] + r];
-                s0[k] += fx * gy * gz;
-                s1[k] += gx * fy * gz;
-                s2[k] += gx * gy * fz;
+            for (int k = 0; k < RPL; ++k) {
+              double xv[NLD], yv[NLD], zv[NLD];
+#pragma unroll
+              for (int i = 0; i < NLD; ++i) {
+                xv[i] = g[ox[k] + i * up + r];
+                yv[i] = g[oy[k] + i * up + r];
+                zv[i] = g[oz[k] + i * up + r];
               }
-            }
-          }
 #pragma unroll
-          for (int k = 0; k < ACC; ++k) racc[k] += s0[k];
-          if constexpr (DERIV) {
-#pragma unroll
-            for (int k = 0; k < ACC; ++k) {
-              racc[ACC + k] += s1[k];
-              racc[2 * ACC + k] += s2[k];
+              for (int fi = 0; fi < NCI; ++fi) {
+                int ix, iy, iz;
+                cart_decode(LI, fi, ix, iy, iz);
+                if constexpr (!DERIV) {
+                  racc[k * NCI + fi] += xv[ix] * yv[iy] * zv[iz];
+                } else {
+                  double fx = ai2 * xv[ix + 1];
+                  double fy = ai2 * yv[iy + 1];
+                  double fz = ai2 * zv[iz + 1];
+                  if (ix > 0) fx += (double)ix * xv[ix - 1];
+                  if (iy > 0) fy += (double)iy * yv[iy - 1];
+                  if (iz > 0) fz += (double)iz * zv[iz - 1];
+                  racc[k * NCI + fi] += fx * yv[iy] * zv[iz];
+                  racc[ACC + k * NCI + fi] += xv[ix] * fy * zv[iz];
+                  racc[2 * ACC + k * NCI + fi] += xv[ix] * yv[iy] * fz;
+                }
+              }
             }
           }
           __syncthreads();
         }
       }
-      // the chunk's results to the slice
+      // the chunk's results to the slice (element fi of row `row` is
+      // fi + NCI row: i fastest)
 #pragma unroll
-      for (int k = 0; k < ACC; ++k) {
-        const int e = e0 + t + k * 256;
+      for (int k = 0; k < RPL; ++k) {
+        const int row = r0 + t + k * 256;
+        if (row < nrow) {
 #pragma unroll
-        for (int comp = 0; comp < NC; ++comp)
-          if (e < S.nf) acc[(size_t)comp * S.nf + e] = racc[comp * ACC + k];
+          for (int comp = 0; comp < NC; ++comp)
+#pragma unroll
+            for (int fi = 0; fi < NCI; ++fi)
+              acc[(size_t)comp * S.nf + (size_t)row * NCI + fi] = racc[comp * ACC + k * NCI + fi];
+        }
       }
     }
     __syncthreads();
@@ -401,14 +398,31 @@ hipError_t launch_generic(const EriArgs& a, const int* l, int deriv, int grid, h
   const size_t gbytes = (size_t)3 * S.g4 * S.nr * sizeof(double);
   const bool glds = gbytes <= kGenLdsBytes;
   const size_t dyn = glds ? gbytes : 0;
-  if (glds && deriv)
-    hipLaunchKernelGGL((eri_generic_kernel<true, true>), dim3(grid), dim3(256), dyn, s, a, S, per);
-  else if (glds)
-    hipLaunchKernelGGL((eri_generic_kernel<true, false>), dim3(grid), dim3(256), dyn, s, a, S, per);
-  else if (deriv)
-    hipLaunchKernelGGL((eri_generic_kernel<false, true>), dim3(grid), dim3(256), 0, s, a, S, per);
-  else
-    hipLaunchKernelGGL((eri_generic_kernel<false, false>), dim3(grid), dim3(256), 0, s, a, S, per);
+  switch (l[0]) {
+#define CX_GEN_LAUNCH(LI)                                                                     \
+  case LI:                                                                                    \
+    if (glds && deriv)                                                                        \
+      hipLaunchKernelGGL((eri_generic_kernel<true, true, LI>), dim3(grid), dim3(256), dyn, s, a, \
+                         S, per);                                                             \
+    else if (glds)                                                                            \
+      hipLaunchKernelGGL((eri_generic_kernel<true, false, LI>), dim3(grid), dim3(256), dyn, s,  \
+                         a, S, per);                                                          \
+    else if (deriv)                                                                           \
+      hipLaunchKernelGGL((eri_generic_kernel<false, true, LI>), dim3(grid), dim3(256), 0, s, a, \
+                         S, per);                                                             \
+    else                                                                                      \
+      hipLaunchKernelGGL((eri_generic_kernel<false, false, LI>), dim3(grid), dim3(256), 0, s,   \
+                         a, S, per);                                                          \
+    break;
+    CX_GEN_LAUNCH(0)
+    CX_GEN_LAUNCH(1)
+    CX_GEN_LAUNCH(2)
+    CX_GEN_LAUNCH(3)
+    CX_GEN_LAUNCH(4)
+#undef CX_GEN_LAUNCH
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
