@@ -224,7 +224,8 @@ def build(jobs=None, verbose=True):
     gen = generate()
     srcs = [os.path.join(CSRC, "runtime.cpp"), os.path.join(CSRC, "rys_table.cpp"),
             os.path.join(CSRC, "rys_kernels.hip"), os.path.join(CSRC, "eri_generic.hip"),
-            os.path.join(CSRC, "rys_mlp.hip"), os.path.join(CSRC, "jk_digest.hip")] + gen
+            os.path.join(CSRC, "rys_mlp.hip"), os.path.join(CSRC, "jk_digest.hip"),
+            os.path.join(CSRC, "spinor.hip")] + gen
     extra = {os.path.join(CSRC, "rys_table.cpp"): ["-fopenmp"]}
     for g in gen:
         if os.path.basename(g).startswith("inst_reg_"):

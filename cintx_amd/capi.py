@@ -29,12 +29,14 @@ INT2E_IP1_SPH = 36
 INT3C2E_SPH = 40
 INT2C2E_SPH = 13
 INT2E_CART = 9
+INT2E_SPINOR = 11
+INT3C2E_SPINOR = 41
 
 FAMILY = {"int2e_sph": 0, "int2e_ip1_sph": 1, "int3c2e_sph": 2, "int3c2e_ip1_sph": 3,
           "int2c2e_sph": 4, "int2e_ip2_sph": 5, "int3c2e_ip2_sph": 6, "int2e_cart": 7,
-          "int3c2e_cart": 8}
+          "int3c2e_cart": 8, "int2e_spinor": 9, "int3c2e_spinor": 10}
 API_OF_FAMILY = {0: INT2E_SPH, 1: INT2E_IP1_SPH, 2: INT3C2E_SPH, 3: INT3C2E_IP1_SPH,
-                 4: INT2C2E_SPH, 7: INT2E_CART}
+                 4: INT2C2E_SPH, 7: INT2E_CART, 9: INT2E_SPINOR, 10: INT3C2E_SPINOR}
 PREC_FP64 = 0
 # launch-shape variants compiled per class (kVariants in csrc/launch.h)
 NVARIANTS = 6
@@ -48,8 +50,8 @@ STATUS = {
 
 EXPORTS = [
     "int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph",
-    "int2e_ip2_sph", "int3c2e_ip2_sph", "int2e_cart", "int3c2e_cart",
-    "int2e_optimizer", "int2e_ip1_optimizer", "int3c2e_optimizer", "int3c2e_ip1_optimizer",
+    "int2e_ip2_sph", "int3c2e_ip2_sph", "int2e_cart", "int3c2e_cart", "int2e_spinor",
+    "int3c2e_spinor", "cintx_spinor_coeffs", "int2e_optimizer", "int2e_ip1_optimizer", "int3c2e_optimizer", "int3c2e_ip1_optimizer",
     "int2c2e_optimizer", "int2e_ip2_optimizer", "int3c2e_ip2_optimizer",
     "CINTdel_optimizer",
     "cintrs_query_workspace", "cintrs_eval", "cintrs_last_error_code",
@@ -96,7 +98,8 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         raw = [DP, IP, IP, IP, ctypes.c_int, IP, ctypes.c_int, DP, ctypes.c_void_p, DP]
         for n in ("int2e_sph", "int2e_ip1_sph", "int3c2e_sph", "int3c2e_ip1_sph", "int2c2e_sph",
-                  "int2e_ip2_sph", "int3c2e_ip2_sph", "int2e_cart", "int3c2e_cart"):
+                  "int2e_ip2_sph", "int3c2e_ip2_sph", "int2e_cart", "int3c2e_cart",
+                  "int2e_spinor", "int3c2e_spinor"):
             getattr(L, n).argtypes = raw
             getattr(L, n).restype = ctypes.c_int
         sz = ctypes.c_size_t
@@ -139,6 +142,8 @@ def lib():
         L.cintx_device_free.argtypes = [DP]
         L.cintx_device_upload.argtypes = [DP, DP, ctypes.c_int64]
         L.cintx_device_download.argtypes = [DP, DP, ctypes.c_int64]
+        L.cintx_spinor_coeffs.argtypes = [ctypes.c_int, ctypes.c_int, DP, DP]
+        L.cintx_spinor_coeffs.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -173,17 +178,42 @@ def arity(family):
     return 2 if family.startswith("int2c2e") else 3 if family.startswith("int3c2e") else 4
 
 
-def shell_functions(family, l):
+def spinor_len(l, kappa):
+    """CINTlen_spinor: 4l+2 (kappa 0), 2l+2 (kappa < 0), 2l (kappa > 0)"""
+    return 4 * l + 2 if kappa == 0 else (2 * l + 2 if kappa < 0 else 2 * l)
+
+
+def shell_functions(family, l, kappa=0, slot=0):
+    if family.endswith("_spinor"):
+        if family.startswith("int3c2e") and slot == 2:
+            return 2 * l + 1
+        return spinor_len(l, kappa)
     return (l + 1) * (l + 2) // 2 if family.endswith("_cart") else 2 * l + 1
 
 
 def natural_dims(family, shls, bas):
-    return [shell_functions(family, int(bas[s][1])) * int(bas[s][3])
-            for s in shls[:arity(family)]]
+    return [shell_functions(family, int(bas[s][1]), int(bas[s][4]), a) * int(bas[s][3])
+            for a, s in enumerate(shls[:arity(family)])]
 
 
 def ncomp(family):
     return 3 if "_ip" in family else 1
+
+
+def is_complex(family):
+    return family.endswith("_spinor")
+
+
+def spinor_coeffs(l, kappa):
+    """The library's Cartesian -> spinor coefficients: complex [row][spin][cart]."""
+    nf = (l + 1) * (l + 2) // 2
+    nd = spinor_len(l, kappa)
+    re = np.zeros(max(1, nd * 2 * nf))
+    im = np.zeros_like(re)
+    n = lib().cintx_spinor_coeffs(l, kappa, _dp(re), _dp(im))
+    if n < 0:
+        raise CintxError(1, f"l = {l} outside 0..4")
+    return (re + 1j * im)[:nd * 2 * nf].reshape(nd, 2, nf)
 
 
 class Optimizer:
@@ -193,7 +223,8 @@ class Optimizer:
 
     def __init__(self, family, atm, bas, env):
         self._atm, self._bas, self._env = _arrays(atm, bas, env)
-        name = {"int2e_cart": "int2e", "int3c2e_cart": "int3c2e"}.get(family, family)
+        name = {"int2e_cart": "int2e", "int3c2e_cart": "int3c2e", "int2e_spinor": "int2e",
+                "int3c2e_spinor": "int3c2e"}.get(family, family)
         name = name[:-4] if name.endswith("_sph") else name
         self.h = ctypes.c_void_p()
         fn = getattr(lib(), name + "_optimizer")
@@ -217,15 +248,18 @@ class Optimizer:
 
 
 def raw_call(family, shls, atm, bas, env, dims=None, opt=None):
-    """libcint raw signature: returns (out, not0).  opt: an Optimizer or None."""
+    """libcint raw signature: returns (out, not0).  opt: an Optimizer or None.
+    Spinor families return complex128 (the F-order libcint buffer)."""
     atm, bas, env = _arrays(atm, bas, env)
     d = natural_dims(family, shls, bas) if dims is None else list(dims)
-    out = np.zeros(int(np.prod(d)) * ncomp(family))
+    out = np.zeros(int(np.prod(d)) * ncomp(family) * (2 if is_complex(family) else 1))
     sh = np.ascontiguousarray(list(shls), dtype=np.int32)
     dd = None if dims is None else np.ascontiguousarray(dims, dtype=np.int32)
     fn = getattr(lib(), family)
     not0 = fn(_dp(out), _ip(dd), _ip(sh), _ip(atm), len(atm), _ip(bas), len(bas), _dp(env),
               None if opt is None else opt.h, None)
+    if is_complex(family):
+        out = out.view(np.complex128)
     return out, not0
 
 
@@ -417,9 +451,11 @@ class Plan:
         return self._tuples
 
     def block_size(self, shls):
-        n = 3 if "_ip" in self.family else 1
-        for s in shls[:arity(self.family)]:
-            n *= shell_functions(self.family, int(self._bas[s][1])) * int(self._bas[s][3])
+        """doubles of a tuple's block (two per complex element)"""
+        n = (3 if "_ip" in self.family else 1) * (2 if is_complex(self.family) else 1)
+        for a, s in enumerate(shls[:arity(self.family)]):
+            n *= shell_functions(self.family, int(self._bas[s][1]), int(self._bas[s][4]),
+                                 a) * int(self._bas[s][3])
         return n
 
     def work(self):

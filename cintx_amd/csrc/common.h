@@ -71,4 +71,43 @@ struct EriArgs {
   unsigned long long* counters;  // optional: [0] surviving primitive tuples
 };
 
+// ------------------------------------------------------------------
+// Cartesian -> spinor coupling table: per l = 0..kMaxL, the 4l+2 spinors
+// of the shell as rows (first the 2l of j = l-1/2, then the 2l+2 of
+// j = l+1/2, each m_j = -j..j), each row 2 ncart(l) complex coefficients:
+// the alpha then the beta component over libcint's Cartesian functions.
+// kappa < 0 selects the j = l+1/2 rows, kappa > 0 the j = l-1/2 rows,
+// kappa = 0 all rows in table order (libcint's KAPPA_OF convention,
+// cart2sph.c:3537-3547).
+// ------------------------------------------------------------------
+__host__ __device__ constexpr int spinor_tab_off(int l) {
+  return l <= 0 ? 0 : spinor_tab_off(l - 1) + (4 * (l - 1) + 2) * 2 * ncart(l - 1);
+}
+constexpr int kSpinorTab = spinor_tab_off(kMaxL + 1);  // complex entries
+__host__ __device__ constexpr int spinor_len(int l, int kappa) {
+  return kappa == 0 ? 4 * l + 2 : (kappa < 0 ? 2 * l + 2 : 2 * l);
+}
+__host__ __device__ constexpr int spinor_row0(int l, int kappa) { return kappa < 0 ? 2 * l : 0; }
+
+// One contracted block of one tuple for the spinor transform.
+struct SpinorJob {
+  int64_t src;    // first double of the Cartesian block (canonical slot order)
+  int64_t dst;    // first complex element of the output block
+  int so[3];      // output strides (complex elements) of argument slots 1..3
+  int cs[4];      // strides of argument slots 0..3 inside the Cartesian block
+  int8_t l[4];    // angular momentum per argument slot
+  int8_t kappa[4];  // sign of KAPPA_OF per argument slot (see above)
+};
+
+// complex elements of the per-workgroup workspace a job needs
+__host__ __device__ inline int64_t spinor_job_slice(const SpinorJob& J, bool four) {
+  const int fi = ncart(J.l[0]), fj = ncart(J.l[1]), fk = ncart(J.l[2]);
+  const int fl = four ? ncart(J.l[3]) : 1;
+  const int di = spinor_len(J.l[0], J.kappa[0]), dj = spinor_len(J.l[1], J.kappa[1]);
+  const int dk = four ? spinor_len(J.l[2], J.kappa[2]) : nsph(J.l[2]);
+  const int64_t n1 = 2LL * di * fj * fk * fl, n2 = (int64_t)di * dj * fk * fl;
+  const int64_t n3 = four ? 2LL * di * dj * dk * fl : 0;
+  return (n1 > n3 ? n1 : n3) + n2;
+}
+
 }  // namespace cx

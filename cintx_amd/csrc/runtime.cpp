@@ -50,6 +50,9 @@ size_t jk_scratch_bytes();
 hipError_t launch_jk_digest(const double* eri, int64_t out_base, int64_t block, const int2* q,
                             int64_t nq, const int2* pair_sh, const int* ao_off, const int* shl_l,
                             const double* D, double* J, double* K, int nao, hipStream_t s);
+hipError_t launch_spinor(const SpinorJob* jobs, int64_t njobs, const double* cart, double* out,
+                         const double* tab, const double* c2s, double* scratch, int64_t slice,
+                         int grid, bool four, hipStream_t s);
 static constexpr int kGenericGrid = 1024;
 
 // ------------------------------------------------------------------
@@ -115,8 +118,11 @@ struct FamilyInfo {
   bool two;     // the canonical bra is one Gaussian
   int dcenter;  // argument shell under nabla
   bool cart = false;  // Cartesian output (no cart -> sph)
+  // complex two-component output: the Cartesian blocks (cart) go through
+  // the spinor transform (slot k of a three-centre family stays spherical)
+  bool spinor = false;
 };
-static constexpr int kNumFamilies = 9;
+static constexpr int kNumFamilies = 11;
 static const FamilyInfo kFam[kNumFamilies] = {
     {false, 0, 1, 4, "int2e_sph", false, -1},
     {false, 1, 3, 4, "int2e_ip1_sph", false, 0},
@@ -128,10 +134,20 @@ static const FamilyInfo kFam[kNumFamilies] = {
     // (ij|nabla k): bra = k alone, ket = the (i j) pair
     {false, 1, 3, 3, "int3c2e_ip2_sph", true, 2},
     {false, 0, 1, 4, "int2e_cart", false, -1, true},
-    {true, 0, 1, 3, "int3c2e_cart", false, -1, true}};
+    {true, 0, 1, 3, "int3c2e_cart", false, -1, true},
+    {false, 0, 1, 4, "int2e_spinor", false, -1, true, true},
+    {true, 0, 1, 3, "int3c2e_spinor", false, -1, true, true}};
 
-// functions per shell in a family's output
+// functions per shell in a family's canonical (kernel) blocks
 static int nfun(const FamilyInfo& f, int l) { return f.cart ? ncart(l) : nsph(l); }
+// functions of argument slot a in a family's output (libcint's counts,
+// CINTcgto_spinor for the spinor slots)
+static int out_fun(const FamilyInfo& f, int a, int l, int kappa) {
+  if (!f.spinor) return nfun(f, l);
+  return (f.three && a == 2) ? nsph(l) : spinor_len(l, kappa);
+}
+// doubles per output element
+static int out_width(const FamilyInfo& f) { return f.spinor ? 2 : 1; }
 
 static constexpr double kExpCutoff = 60.0;     // cint_config.h.in EXPCUTOFF
 static constexpr double kMinExpCutoff = 40.0;  // cint_config.h.in MIN_EXPCUTOFF
@@ -197,6 +213,73 @@ static void build_c2s_table(std::vector<double>& tab) {
 }
 
 // ------------------------------------------------------------------
+// Cartesian -> spinor coupling table (layout in common.h): the two-component
+// spinors |l, j, m_j> built from the complex spherical harmonics (Condon-
+// Shortley phase) and the Clebsch-Gordan coefficients <l m 1/2 s|j m_j>:
+//   j = l+1/2: sqrt((2l+1+2m_j)/(2(2l+1))) Y_l^{m_j-1/2} alpha
+//            + sqrt((2l+1-2m_j)/(2(2l+1))) Y_l^{m_j+1/2} beta
+//   j = l-1/2: -sqrt((2l+1-2m_j)/(2(2l+1))) Y_l^{m_j-1/2} alpha
+//            + sqrt((2l+1+2m_j)/(2(2l+1))) Y_l^{m_j+1/2} beta
+// with Y_l^{+m} = (-1)^m (S_lm + i S_l,-m)/sqrt(2), Y_l^{-m} =
+// (S_lm - i S_l,-m)/sqrt(2), Y_l^0 = S_l0 over the real solid harmonics S
+// of the c2s table (p: x, y, z are S_11, S_1-1, S_10), so the coefficients
+// carry libcint's Cartesian normalisation.  (Same convention as libcint's
+// g_trans_cart2jR/I, cart2sph.c:3561, scripts/cart2sph.py:187-265.)
+// `tab` receives 2 kSpinorTab doubles (re, im interleaved).
+// ------------------------------------------------------------------
+static void build_spinor_table(const std::vector<double>& c2s, std::vector<double>& tab) {
+  tab.assign((size_t)2 * kSpinorTab, 0.0);
+  for (int l = 0; l <= kMaxL; ++l) {
+    const int nf = ncart(l);
+    // complex harmonics Y[m + l][cart] (re, im)
+    std::vector<long double> yr((size_t)(2 * l + 1) * nf, 0.0L), yi(yr.size(), 0.0L);
+    auto real_row = [&](int m) -> const double* {
+      int row = m + l;
+      if (l == 1) row = m == 1 ? 0 : (m == -1 ? 1 : 2);  // p rows are x, y, z
+      return c2s.data() + ((size_t)l * kC2sM + row) * kC2sC;
+    };
+    if (l == 0) {
+      yr[0] = 1.0L;
+    } else {
+      for (int c = 0; c < nf; ++c) yr[(size_t)l * nf + c] = real_row(0)[c];
+      const long double h = sqrtl(0.5L);
+      for (int m = 1; m <= l; ++m) {
+        const long double sg = (m & 1) ? -1.0L : 1.0L;
+        for (int c = 0; c < nf; ++c) {
+          const long double cr = real_row(m)[c], ci = real_row(-m)[c];
+          yr[(size_t)(l + m) * nf + c] = sg * h * cr;
+          yi[(size_t)(l + m) * nf + c] = sg * h * ci;
+          yr[(size_t)(l - m) * nf + c] = h * cr;
+          yi[(size_t)(l - m) * nf + c] = -h * ci;
+        }
+      }
+    }
+    double* out = tab.data() + 2 * (size_t)spinor_tab_off(l);
+    int row = 0;
+    auto put = [&](int twomj, long double ca, long double cb) {
+      const int ma = (twomj - 1) / 2, mb = (twomj + 1) / 2;
+      double* r = out + (size_t)row * 2 * nf * 2;
+      for (int c = 0; c < nf; ++c) {
+        if (ma >= -l && ma <= l) {
+          r[2 * c] = (double)(ca * yr[(size_t)(ma + l) * nf + c]);
+          r[2 * c + 1] = (double)(ca * yi[(size_t)(ma + l) * nf + c]);
+        }
+        if (mb >= -l && mb <= l) {
+          r[2 * (nf + c)] = (double)(cb * yr[(size_t)(mb + l) * nf + c]);
+          r[2 * (nf + c) + 1] = (double)(cb * yi[(size_t)(mb + l) * nf + c]);
+        }
+      }
+      ++row;
+    };
+    const long double n2 = 2.0L * (2 * l + 1);
+    for (int t = -(2 * l - 1); t <= 2 * l - 1; t += 2)  // j = l - 1/2
+      put(t, -sqrtl((2 * l + 1 - t) / n2), sqrtl((2 * l + 1 + t) / n2));
+    for (int t = -(2 * l + 1); t <= 2 * l + 1; t += 2)  // j = l + 1/2
+      put(t, sqrtl((2 * l + 1 + t) / n2), sqrtl((2 * l + 1 - t) / n2));
+  }
+}
+
+// ------------------------------------------------------------------
 // Device context (one per process; current device at first use).
 // ------------------------------------------------------------------
 constexpr int kChains = 8;  // parallel chains of a captured plan graph
@@ -209,6 +292,7 @@ struct DevCtx {
   double* d_rys = nullptr;
   double* d_large = nullptr;
   double* d_c2s = nullptr;
+  double* d_spinor = nullptr;  // Cartesian -> spinor table (complex)
   Registry* reg = nullptr;
   double* d_jk_scratch = nullptr;  // the static scratch of cintx_plan_jk_noalloc
   // autotuned class-kernel variant per (deriv, form, class); -1: untuned
@@ -245,6 +329,10 @@ static DevCtx& ctx() {
   build_c2s_table(c2s);
   HIPCHK(hipMalloc(&c->d_c2s, c2s.size() * sizeof(double)));
   HIPCHK(hipMemcpy(c->d_c2s, c2s.data(), c2s.size() * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> sp;
+  build_spinor_table(c2s, sp);
+  HIPCHK(hipMalloc(&c->d_spinor, sp.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(c->d_spinor, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice));
   // address of the allocation-free J/K's static scratch on this device
   HIPCHK(jk_scratch_address(&c->d_jk_scratch));
   c->reg = new Registry();
@@ -269,7 +357,7 @@ struct HostBasis {
   std::vector<VShell> v;
   std::vector<double> exps, coefs;
   std::vector<int> first_v;  // original shell -> first virtual shell
-  std::vector<int> l, nctr, nprim;
+  std::vector<int> l, nctr, nprim, kappa;
   int nbas = 0;
 };
 
@@ -286,6 +374,7 @@ static HostBasis ingest(const int* atm, int natm, const int* bas, int nbas, cons
   B.l.resize(nbas);
   B.nctr.resize(nbas);
   B.nprim.resize(nbas);
+  B.kappa.resize(nbas);
   auto env_ok = [&](long p, long n) {
     return p >= 0 && (env_len == 0 || (size_t)(p + n) <= env_len);
   };
@@ -307,6 +396,7 @@ static HostBasis ingest(const int* atm, int natm, const int* bas, int nbas, cons
     B.l[s] = l;
     B.nctr[s] = nc;
     B.nprim[s] = np;
+    B.kappa[s] = b[4];
     for (int c = 0; c < nc; ++c) {
       VShell vs;
       vs.l = l;
@@ -501,6 +591,15 @@ struct cintx_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   double* graph_out = nullptr;
+  // spinor families: the bins' Cartesian blocks and the transform jobs
+  // (one per contracted block) that turn them into d_out's layout
+  int64_t cart_total = 0;  // doubles of the Cartesian blocks
+  double* d_cart = nullptr;
+  cx::SpinorJob* d_jobs = nullptr;
+  int64_t njobs = 0;
+  double* d_spin_scratch = nullptr;
+  int64_t spin_slice = 0;  // complex elements per workgroup
+  int spin_grid = 0;
   // device pointers of the pair SoA
   double *d_geo = nullptr, *d_rec = nullptr;
   int* d_poff = nullptr;
@@ -540,10 +639,16 @@ static void canon_perm(int fam, const int* l, int p[4]) {
   }
 }
 
+// output extent of argument slot a of a tuple (contractions included)
+static int out_extent(const FamilyInfo& f, const HostBasis& B, int a, int sh) {
+  return out_fun(f, a, B.l[sh], B.kappa[sh]) * B.nctr[sh];
+}
+
+// doubles of a tuple's output block
 static int64_t nat_size(int fam, const HostBasis& B, const int* shl) {
   const FamilyInfo& f = kFam[fam];
-  int64_t n = f.ncomp;
-  for (int a = 0; a < f.arity; ++a) n *= (int64_t)nfun(f, B.l[shl[a]]) * B.nctr[shl[a]];
+  int64_t n = (int64_t)f.ncomp * out_width(f);
+  for (int a = 0; a < f.arity; ++a) n *= out_extent(f, B, a, shl[a]);
   return n;
 }
 
@@ -661,15 +766,80 @@ static void plan_sort_bins(cintx_plan* P) {
   for (VQ& v : P->vqs) v.idx = remap[v.bin][(size_t)v.idx];
 }
 
+// The spinor-transform job of one contracted block: argument-slot shells
+// shl, canonical permutation perm (canonical slot c holds argument
+// perm[c]) and classes cl, contraction columns cc (canonical order), the
+// block at `src` in the Cartesian buffer, the output block of the tuple at
+// complex element dst0 with argument-slot strides so (complex elements).
+static SpinorJob spinor_job(const FamilyInfo& f, const HostBasis& B, const int* shl,
+                            const int* perm, const int* cl, const int* cc, int64_t src,
+                            int64_t dst0, const int64_t* so) {
+  SpinorJob J;
+  std::memset(&J, 0, sizeof J);
+  J.src = src;
+  int cst = 1, cca[4] = {0, 0, 0, 0};
+  for (int c = 0; c < 4; ++c) {
+    const int a = perm[c];
+    if (a < f.arity) {
+      J.cs[a] = cst;
+      cca[a] = cc[c];
+    }
+    cst *= ncart(cl[c]);
+  }
+  int64_t dst = dst0;
+  for (int a = 0; a < f.arity; ++a) {
+    const int sh = shl[a], l = B.l[sh], kap = B.kappa[sh];
+    J.l[a] = (int8_t)l;
+    J.kappa[a] = (int8_t)(kap < 0 ? -1 : (kap > 0 ? 1 : 0));
+    dst += (int64_t)cca[a] * out_fun(f, a, l, kap) * so[a];
+    if (a > 0) {
+      if (so[a] > INT32_MAX) fail(CINTX_STATUS_UNSUPPORTED_API, "spinor block too large");
+      J.so[a - 1] = (int)so[a];
+    }
+  }
+  J.dst = dst;
+  return J;
+}
+
+// Workgroups of a spinor launch: enough to fill the device, with their
+// workspace held under `budget` bytes.
+static int spinor_grid(int64_t njobs, int64_t slice, size_t budget) {
+  const int64_t fit = std::max<int64_t>(1, (int64_t)(budget / ((size_t)slice * 16)));
+  return (int)std::max<int64_t>(1, std::min<int64_t>({njobs, fit, (int64_t)2048}));
+}
+
 static void plan_upload(cintx_plan* P) {
   DevCtx& C = ctx();
+  const FamilyInfo& fam = kFam[P->family];
   plan_sort_bins(P);
   int64_t base = 0;
   for (auto& b : P->bins) {
     b.out_base = base;
     base += b.block * (int64_t)b.q.size();
   }
-  if (base != P->total) fail(CINTX_STATUS_PANIC, "output size mismatch");
+  if (fam.spinor)
+    P->cart_total = base;
+  else if (base != P->total)
+    fail(CINTX_STATUS_PANIC, "output size mismatch");
+  std::vector<SpinorJob> jobs;
+  if (fam.spinor) {
+    const HostBasis& B = P->basis;
+    P->spin_slice = 1;
+    for (const TupleRec& tr : P->tuples) {
+      int64_t so[4] = {1, 1, 1, 1};
+      for (int a = 1; a < fam.arity; ++a) so[a] = so[a - 1] * out_extent(fam, B, a - 1, tr.shl[a - 1]);
+      for (int64_t k = 0; k < tr.nvq; ++k) {
+        const VQ& vq = P->vqs[tr.first_vq + k];
+        const Bin& b = P->bins[vq.bin];
+        const int cc[4] = {vq.cc[0], vq.cc[1], vq.cc[2], vq.cc[3]};
+        jobs.push_back(spinor_job(fam, B, tr.shl, tr.perm, b.l, cc, b.out_base + vq.idx * b.block,
+                                  tr.nat_off / 2, so));
+        P->spin_slice = std::max(P->spin_slice, spinor_job_slice(jobs.back(), fam.arity == 4));
+      }
+    }
+    P->njobs = (int64_t)jobs.size();
+    P->spin_grid = spinor_grid(P->njobs, P->spin_slice, (size_t)1 << 30);
+  }
   const PairSet& S = P->pairs;
   size_t npp = S.aij.size(), npair = S.off.size();
   std::vector<std::pair<size_t, size_t>> segs;  // (offset, bytes)
@@ -705,6 +875,9 @@ static void plan_upload(cintx_plan* P) {
     P->scratch_chains *= 2;
   P->scratch_stride = scratch;
   size_t o_scr = take((size_t)scratch * 8 * (size_t)P->scratch_chains);
+  size_t o_jobs = take(jobs.size() * sizeof(SpinorJob));
+  size_t o_cart = take((size_t)P->cart_total * 8);
+  size_t o_sscr = take(fam.spinor ? (size_t)P->spin_grid * (size_t)P->spin_slice * 16 : 0);
   size_t host_bytes = o_out;  // everything before the output is uploaded
   P->blob_bytes = cur;
   HIPCHK(hipMalloc(&P->blob, P->blob_bytes));
@@ -747,7 +920,15 @@ static void plan_upload(cintx_plan* P) {
     b.d_q = (int2*)(P->blob + o_q[i]);
   }
   HIPCHK(hipMemcpyAsync(P->blob, h.data(), host_bytes, hipMemcpyHostToDevice, C.stream));
+  if (!jobs.empty())
+    HIPCHK(hipMemcpyAsync(P->blob + o_jobs, jobs.data(), jobs.size() * sizeof(SpinorJob),
+                          hipMemcpyHostToDevice, C.stream));
   HIPCHK(hipStreamSynchronize(C.stream));
+  if (fam.spinor) {
+    P->d_jobs = (SpinorJob*)(P->blob + o_jobs);
+    P->d_cart = (double*)(P->blob + o_cart);
+    P->d_spin_scratch = (double*)(P->blob + o_sscr);
+  }
   P->d_rec = (double*)(P->blob + o_rec);
   P->d_poff = (int*)(P->blob + o_poff);
   P->d_pnp = (int*)(P->blob + o_pnp);
@@ -889,12 +1070,25 @@ static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count,
   DevCtx& C = ctx();
   if (!st) st = C.stream;
   Bin& b = P->bins[i];
+  if (kFam[P->family].spinor) out = P->d_cart;  // the transform reads them from there
   const KernelInfo& k = variant_info(P, b, b.variant);
   const PairDev pd{P->d_rec, P->d_poff, P->d_pnp, P->d_geo};
   double* scratch = b.generic ? P->d_scratch + (size_t)chain * (size_t)P->scratch_stride : nullptr;
   launch_bin_kernels(P->family, b.l, b.variant, b.generic, k, pd, P->expcutoff, b.d_q,
                      (int64_t)b.q.size(), out, b.out_base, b.block, scratch,
                      count ? P->d_counters + 2 * i : nullptr, st);
+}
+
+// Spinor families: the Cartesian blocks (all bins done) -> `out`.
+static void plan_launch_spinor(cintx_plan* P, double* out, hipStream_t st) {
+  if (!kFam[P->family].spinor || P->njobs == 0) return;
+  DevCtx& C = ctx();
+  (void)hipGetLastError();
+  hipError_t e = launch_spinor(P->d_jobs, P->njobs, P->d_cart, out, C.d_spinor, C.d_c2s,
+                               P->d_spin_scratch, P->spin_slice, P->spin_grid,
+                               kFam[P->family].arity == 4, st);
+  if (e != hipSuccess)
+    fail(CINTX_STATUS_EXECUTION_FAILED, std::string("spinor transform launch: ") + hipGetErrorString(e));
 }
 
 static void plan_launch_all(cintx_plan* P, double* out, bool count,
@@ -905,6 +1099,7 @@ static void plan_launch_all(cintx_plan* P, double* out, bool count,
     plan_launch_bin(P, i, out, count);
     if (evs) HIPCHK(hipEventRecord((*evs)[2 * i + 1], C.stream));
   }
+  plan_launch_spinor(P, out, C.stream);
 }
 
 static cintx_plan* plan_new(int family, const int* atm, int natm, const int* bas, int nbas,
@@ -1020,6 +1215,12 @@ static int cache_size(int fam, const int* shls, const int* bas) {
     size_t nc_all = nf * nctr;
     size_t a = leng + len0 + nc_all * ncomp * 3 + pdata;
     size_t b = nc_all * ncomp + nf * 4;
+    if (f.spinor) {  // CINT2e_spinor_drv, cint2e.c:886-895
+      const size_t ci = (size_t)spinor_len(l[0], B(shls[0], 4)) * nc[0];
+      const size_t cj = (size_t)spinor_len(l[1], B(shls[1], 4)) * nc[1];
+      const size_t n1 = ci * ncart(l[2]) * nc[2] * ncart(l[3]) * nc[3] * cj;
+      b = nc_all * ncomp + n1 * 2 + nf * 32 * 2;
+    }
     return (int)std::max(a, b);
   } else {
     int llc = l[2] + inc[2];  // g3c2e.c:68-69: k sits in the l slot
@@ -1035,6 +1236,7 @@ static int cache_size(int fam, const int* shls, const int* bas) {
     size_t nc_all = nf * nctr;
     size_t a = leng + len0 + nc_all * ncomp * 3 + pdata;
     size_t b = nc_all * ncomp + nf * 3;
+    if (f.spinor) b = nc_all * ncomp + nf * 14 * 2;  // CINT3c2e_spinor_drv, cint3c2e.c:640-645
     return (int)std::max(a, b);
   }
 }
@@ -1234,7 +1436,26 @@ static int raw_eval(int fam, double* out, const int* dims, const int* shls, cons
     const int v = C.tuned[f.deriv][f.cart ? 2 : 0][cl[0]][cl[1]][cl[2]][cl[3]];
     if (v > 0 && v < kVariants && ks[v].fn) variant = v;
   }
+  // natural extents (complex elements for the spinor families)
+  int n[4] = {1, 1, 1, 1};
+  for (int a = 0; a < f.arity; ++a) n[a] = out_extent(f, *B, a, shl[a]);
+  const int64_t nnat = (int64_t)n[0] * n[1] * n[2] * n[3];
+  // spinor families: one transform job per contracted block, into a
+  // compact natural block
+  std::vector<SpinorJob> jobs;
+  int64_t sslice = 1;
+  int sgrid = 0;
+  if (f.spinor) {
+    const int64_t so[4] = {1, n[0], (int64_t)n[0] * n[1], (int64_t)n[0] * n[1] * n[2]};
+    for (int64_t v = 0; v < nq; ++v) {
+      const int ccv[4] = {cc[v][0], cc[v][1], cc[v][2], cc[v][3]};
+      jobs.push_back(spinor_job(f, *B, shl, perm, cl, ccv, v * block, 0, so));
+      sslice = std::max(sslice, spinor_job_slice(jobs.back(), f.arity == 4));
+    }
+    sgrid = spinor_grid(nq, sslice, (size_t)256 << 20);
+  }
   // arena: [pairs (no optimizer)] [tuples] [output] [counters] [scratch]
+  //        [spinor jobs, output, workspace]
   const size_t pbytes = opt ? 0 : pairs_bytes(local_pairs);
   const size_t o_q = pbytes, o_out = o_q + align256(nq * sizeof(int2));
   const size_t o_cnt = o_out + align256(nq * block * 8);
@@ -1242,7 +1463,11 @@ static int raw_eval(int fam, double* out, const int* dims, const int* shls, cons
   const size_t scr = generic ? (size_t)std::min<int64_t>(nq, kGenericGrid) *
                                    generic_scratch_per_block(cl, f.deriv) * 8
                              : 0;
-  char* A = arena(T, o_scr + scr);
+  const size_t o_jobs = align256(o_scr + scr);
+  const size_t o_sout = o_jobs + align256(jobs.size() * sizeof(SpinorJob));
+  const size_t o_sscr = o_sout + align256(f.spinor ? (size_t)nnat * 16 : 0);
+  const size_t sscr = f.spinor ? (size_t)sgrid * sslice * 16 : 0;
+  char* A = arena(T, f.spinor ? o_sscr + sscr : o_scr + scr);
   const PairDev pd = opt ? opt->dev : pairs_upload(local_pairs, A, T.host, T.stream);
   HIPCHK(hipMemcpyAsync(A + o_q, q.data(), nq * sizeof(int2), hipMemcpyHostToDevice, T.stream));
   HIPCHK(hipMemsetAsync(A + o_cnt, 0, 16, T.stream));
@@ -1251,14 +1476,6 @@ static int raw_eval(int fam, double* out, const int* dims, const int* shls, cons
   launch_bin_kernels(fam, cl, variant, generic, ks[variant], pd, fam_expcutoff(fam, env),
                      (const int2*)(A + o_q), nq, dout, 0, block,
                      generic ? (double*)(A + o_scr) : nullptr, dcnt, T.stream);
-  std::vector<double> canon((size_t)(nq * block));
-  unsigned long long cnt[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(canon.data(), dout, canon.size() * 8, hipMemcpyDeviceToHost, T.stream));
-  HIPCHK(hipMemcpyAsync(cnt, dcnt, 16, hipMemcpyDeviceToHost, T.stream));
-  HIPCHK(hipStreamSynchronize(T.stream));
-  // place the canonical blocks into libcint's layout with the caller's dims
-  int n[4] = {1, 1, 1, 1};
-  for (int a = 0; a < f.arity; ++a) n[a] = nfun(f, B->l[shl[a]]) * B->nctr[shl[a]];
   int d[4] = {n[0], n[1], n[2], n[3]};
   if (dims)
     for (int a = 0; a < f.arity; ++a) d[a] = dims[a];
@@ -1266,6 +1483,38 @@ static int raw_eval(int fam, double* out, const int* dims, const int* shls, cons
     if (d[a] < n[a]) fail(CINTX_STATUS_INVALID_INPUT, "dims smaller than the shell extents");
   const int64_t so[4] = {1, d[0], (int64_t)d[0] * d[1], (int64_t)d[0] * d[1] * d[2]};
   const int64_t nout_d = (int64_t)d[0] * d[1] * d[2] * d[3];
+  unsigned long long cnt[2] = {0, 0};
+  if (f.spinor) {
+    // transform on the device, then place the compact block with the
+    // caller's dims (cart2sph.c:4676-4703 zcopy_iklj placement)
+    HIPCHK(hipMemcpyAsync(A + o_jobs, jobs.data(), jobs.size() * sizeof(SpinorJob),
+                          hipMemcpyHostToDevice, T.stream));
+    (void)hipGetLastError();
+    HIPCHK(launch_spinor((const SpinorJob*)(A + o_jobs), nq, dout, (double*)(A + o_sout),
+                         C.d_spinor, C.d_c2s, (double*)(A + o_sscr), sslice, sgrid,
+                         f.arity == 4, T.stream));
+    std::vector<double> nat((size_t)nnat * 2);
+    HIPCHK(hipMemcpyAsync(nat.data(), A + o_sout, nat.size() * 8, hipMemcpyDeviceToHost,
+                          T.stream));
+    HIPCHK(hipMemcpyAsync(cnt, dcnt, 16, hipMemcpyDeviceToHost, T.stream));
+    HIPCHK(hipStreamSynchronize(T.stream));
+    const bool nz = cnt[0] != 0;
+    int64_t e = 0;
+    for (int m3 = 0; m3 < n[3]; ++m3)
+      for (int m2 = 0; m2 < n[2]; ++m2)
+        for (int m1 = 0; m1 < n[1]; ++m1)
+          for (int m0 = 0; m0 < n[0]; ++m0, ++e) {
+            double* dp = out + 2 * (m0 + m1 * so[1] + m2 * so[2] + m3 * so[3]);
+            dp[0] = nz ? nat[(size_t)(2 * e)] : 0.0;
+            dp[1] = nz ? nat[(size_t)(2 * e + 1)] : 0.0;
+          }
+    return nz ? 1 : 0;
+  }
+  std::vector<double> canon((size_t)(nq * block));
+  HIPCHK(hipMemcpyAsync(canon.data(), dout, canon.size() * 8, hipMemcpyDeviceToHost, T.stream));
+  HIPCHK(hipMemcpyAsync(cnt, dcnt, 16, hipMemcpyDeviceToHost, T.stream));
+  HIPCHK(hipStreamSynchronize(T.stream));
+  // place the canonical blocks into libcint's layout with the caller's dims
   const bool not0 = cnt[0] != 0;
   int ns[4];
   int64_t st[4];
@@ -1307,6 +1556,10 @@ static int family_of_api(int api) {
       return 4;
     case CINTX_API_INT2E_CART:
       return 7;
+    case CINTX_API_INT2E_SPINOR:
+      return 9;
+    case CINTX_API_INT3C2E_SPINOR:
+      return 10;
     default:
       return -1;
   }
@@ -1348,7 +1601,7 @@ static int raw_entry(int fam, double* out, int* dims, int* shls, int* atm, int n
     if (strict && strict[0] == '1') abort();
     const FamilyInfo& f = kFam[fam];
     if (shls && bas && nbas > 0) {
-      int64_t n = f.ncomp;
+      int64_t n = (int64_t)f.ncomp * out_width(f);
       bool ok = true;
       for (int a = 0; a < f.arity; ++a) {
         if (shls[a] < 0 || shls[a] >= nbas) {
@@ -1356,7 +1609,7 @@ static int raw_entry(int fam, double* out, int* dims, int* shls, int* atm, int n
           break;
         }
         const int* b = bas + 8 * (size_t)shls[a];
-        n *= dims ? (int64_t)dims[a] : (int64_t)nfun(f, b[1]) * b[3];
+        n *= dims ? (int64_t)dims[a] : (int64_t)out_fun(f, a, b[1], b[4]) * b[3];
       }
       if (ok && n > 0) std::memset(out, 0, (size_t)n * sizeof(double));
     }
@@ -1382,6 +1635,9 @@ CINTX_RAW_ENTRY(int2e_ip2_sph, 5)
 CINTX_RAW_ENTRY(int3c2e_ip2_sph, 6)
 CINTX_RAW_ENTRY(int2e_cart, 7)
 CINTX_RAW_ENTRY(int3c2e_cart, 8)
+// out is double complex (re, im interleaved)
+CINTX_RAW_ENTRY(int2e_spinor, 9)
+CINTX_RAW_ENTRY(int3c2e_spinor, 10)
 #undef CINTX_RAW_ENTRY
 
 // optimizer.c:344-425 builds the pair data of a basis once; so does this,
@@ -1436,7 +1692,7 @@ static void shim_common(int fam, const int* dims, size_t dims_len, const int* sh
   int n[4] = {1, 1, 1, 1};
   for (int a = 0; a < f.arity; ++a) {
     if (shls[a] < 0 || shls[a] >= nbas) fail(CINTX_STATUS_INVALID_INPUT, "shell index out of range");
-    n[a] = nfun(f, B.l[shls[a]]) * B.nctr[shls[a]];
+    n[a] = out_extent(f, B, a, shls[a]);
   }
   if (dims) {
     // crates/cintx-compat/src/layout.rs:25-64: dims must equal the natural extents
@@ -1444,7 +1700,7 @@ static void shim_common(int fam, const int* dims, size_t dims_len, const int* sh
     for (int a = 0; a < f.arity; ++a)
       if (dims[a] != n[a]) fail(CINTX_STATUS_INVALID_INPUT, "dims differ from natural extents");
   }
-  *required = (int64_t)f.ncomp * n[0] * n[1] * n[2] * n[3];
+  *required = (int64_t)f.ncomp * out_width(f) * n[0] * n[1] * n[2] * n[3];
 }
 
 int cintrs_query_workspace(int api, const int* dims, size_t dims_len, const int* shls,
@@ -1736,6 +1992,7 @@ int cintx_plan_execute_graph(cintx_plan* plan, double* out_dev) {
           HIPCHK(hipEventRecord(ev[c + 1], C.branch[c]));
           HIPCHK(hipStreamWaitEvent(C.stream, ev[c + 1], 0));
         }
+        plan_launch_spinor(plan, out, C.stream);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(C.stream, &g);
@@ -1778,6 +2035,12 @@ int cintx_plan_copy_output(cintx_plan* plan, double* host_out) {
     if (!plan) fail(CINTX_STATUS_NULL_POINTER, "plan is null");
     if (!host_out) fail(CINTX_STATUS_NULL_POINTER, "host_out is null");
     DevCtx& C = ctx();
+    if (kFam[plan->family].spinor) {  // already in the callers' layout
+      HIPCHK(hipMemcpyAsync(host_out, plan->d_out, (size_t)plan->total * 8,
+                            hipMemcpyDeviceToHost, C.stream));
+      HIPCHK(hipStreamSynchronize(C.stream));
+      return;
+    }
     std::vector<double> canon((size_t)plan->total);
     HIPCHK(hipMemcpyAsync(canon.data(), plan->d_out, (size_t)plan->total * 8,
                           hipMemcpyDeviceToHost, C.stream));
@@ -1835,6 +2098,13 @@ int cintx_plan_copy_tuples(cintx_plan* plan, const int64_t* which, int64_t n, do
         fail(CINTX_STATUS_INVALID_INPUT, "tuple index out of range");
       const TupleRec& tr = plan->tuples[t];
       if (offsets) offsets[k] = pos;
+      if (f.spinor) {  // d_out holds the callers' layout
+        HIPCHK(hipMemcpyAsync(host_out + pos, plan->d_out + tr.nat_off, (size_t)tr.nat_size * 8,
+                              hipMemcpyDeviceToHost, C.stream));
+        HIPCHK(hipStreamSynchronize(C.stream));
+        pos += tr.nat_size;
+        continue;
+      }
       // gather this tuple's canonical blocks, then place them
       // size the staging buffer first: copies in flight must not see it move
       std::vector<int64_t> where((size_t)tr.nvq);
@@ -2463,6 +2733,23 @@ int cintx_plan_jk_noalloc(cintx_plan* plan, const double* D_dev, double* J_dev, 
     HIPCHK(launch_jk_sym_copy(D_dev, C.d_jk_scratch, plan->nao, C.stream));
     jk_enqueue(plan, C.d_jk_scratch, J_dev, K_dev, nullptr);
   });
+}
+
+int cintx_spinor_coeffs(int l, int kappa, double* re, double* im) {
+  if (l < 0 || l > kMaxL) return -1;
+  static const std::vector<double> tab = [] {
+    std::vector<double> c2s, t;
+    build_c2s_table(c2s);
+    build_spinor_table(c2s, t);
+    return t;
+  }();
+  const int nd = spinor_len(l, kappa), row0 = spinor_row0(l, kappa), n2 = 2 * ncart(l);
+  const double* src = tab.data() + 2 * ((size_t)spinor_tab_off(l) + (size_t)row0 * n2);
+  for (int k = 0; k < nd * n2; ++k) {
+    if (re) re[k] = src[2 * k];
+    if (im) im[k] = src[2 * k + 1];
+  }
+  return nd;
 }
 
 int cintx_plan_fill_output(cintx_plan* plan, double value) {

@@ -78,6 +78,19 @@ int int2e_ip2_sph(double *out, int *dims, int *shls, int *atm, int natm, int *ba
 int int3c2e_ip2_sph(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
                     double *env, CINTOpt *opt, double *cache);
 
+/* libcint-master/src/cint2e.c:1215-1224, cint3c2e.c:719-727: spinor
+ * (two-component, spin-free) output.  out is double complex (re, im
+ * interleaved) in libcint's layout: [dl][dk][dj][di] with
+ * CINTcgto_spinor(shell) per slot (4l+2 for KAPPA_OF 0, 2l+2 for kappa < 0,
+ * 2l for kappa > 0, times NCTR_OF); int3c2e_spinor's k stays spherical
+ * (2l+1).  Bra spinors (i, k) are conjugated and the spin is summed per
+ * electron, as c2s_sf_2e1/c2s_sf_2e2/c2s_sf_3c2e1 (cart2sph.c:5385-5528,
+ * 5997-6050).  They share int2e_optimizer / int3c2e_optimizer. */
+int int2e_spinor(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                 double *env, CINTOpt *opt, double *cache);
+int int3c2e_spinor(double *out, int *dims, int *shls, int *atm, int natm, int *bas, int nbas,
+                   double *env, CINTOpt *opt, double *cache);
+
 /* libcint-master/src/cint2e.c:1195, autocode/grad2.c:69,
  * cint3c2e.c:702, autocode/int3c2e.c:68 */
 void int2e_optimizer(CINTOpt **opt, int *atm, int natm, int *bas, int nbas, double *env);
@@ -97,8 +110,12 @@ void CINTdel_optimizer(CINTOpt **opt);
 /* ------------------------------------------------------------------ */
 
 /* api ids: crates/cintx-capi/src/shim.rs:8-49.  The reference enum has no
- * int3c2e_sph entry; CINTX_API_INT3C2E_SPH is this library's extension. */
+ * int3c2e_sph or int3c2e_spinor entry; CINTX_API_INT3C2E_SPH and
+ * CINTX_API_INT3C2E_SPINOR are this library's extensions.  For the spinor
+ * ids out_len counts doubles (two per complex element). */
 #define CINTX_API_INT2E_SPH 10
+#define CINTX_API_INT2E_SPINOR 11
+#define CINTX_API_INT3C2E_SPINOR 41
 #define CINTX_API_INT3C2E_IP1_SPH 19
 #define CINTX_API_INT2E_IP1_SPH 36
 #define CINTX_API_INT3C2E_SPH 40
@@ -159,6 +176,19 @@ size_t cintrs_copy_last_error_message(char *out, size_t out_len);
 #define CINTX_FAMILY_INT2E_IP1_SPH 1
 #define CINTX_FAMILY_INT3C2E_SPH 2
 #define CINTX_FAMILY_INT3C2E_IP1_SPH 3
+/* further families served by the same kernels: the DF metric, the nabla-k
+ * derivatives, Cartesian and spinor output (api_manifest.csv:11, 13, 15,
+ * 24, 228, 276 and int3c2e_spinor).
+ * A spinor plan's output (execute's out_dev, cintx_plan_copy_output) is
+ * the complex libcint layout of every tuple directly, re/im interleaved;
+ * cintx_plan_sizes counts doubles. */
+#define CINTX_FAMILY_INT2C2E_SPH 4
+#define CINTX_FAMILY_INT2E_IP2_SPH 5
+#define CINTX_FAMILY_INT3C2E_IP2_SPH 6
+#define CINTX_FAMILY_INT2E_CART 7
+#define CINTX_FAMILY_INT3C2E_CART 8
+#define CINTX_FAMILY_INT2E_SPINOR 9
+#define CINTX_FAMILY_INT3C2E_SPINOR 10
 
 /* precision modes: FP64 everywhere, or the mixed mode (FP32 recurrences and
  * primitive sums for tuples whose Schwarz bound is below the plan's
@@ -294,6 +324,17 @@ int cintx_rys_mlp_grad(const float *params, const float *z, const int *nroots,
 int cintx_rys_mlp_eval(const float *params, int n, const double *x, int64_t nx,
                        int newton_steps, int init_mode, double *u, double *w, int *status,
                        double *polish_err, double *ms);
+
+/* The Cartesian -> spinor coupling coefficients this library applies for
+ * shell angular momentum l (0..4) and KAPPA_OF kappa: CINTlen_spinor rows
+ * (m_j = -j..j; kappa = 0: the j = l-1/2 rows, then the j = l+1/2 rows),
+ * each 2 ncart(l) values, the alpha then the beta component; re/im may be
+ * NULL.  Returns the row count, -1 for l outside 0..4.  Host-only (no
+ * device use).  Same values as libcint's g_trans_cart2jR/g_trans_cart2jI
+ * (cart2sph.c:3561) and crates/cintx-cubecl/src/transform/
+ * c2spinor_coeffs.rs, generated here from the real solid harmonics and the
+ * Clebsch-Gordan coefficients. */
+int cintx_spinor_coeffs(int l, int kappa, double *re, double *im);
 
 /* Fill the plan's own output buffer with `value` on the plan's stream
  * (asynchronous; a kernel, so it may be captured in a graph).  No counterpart
