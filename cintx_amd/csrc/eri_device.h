@@ -532,32 +532,77 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
 #pragma unroll
     for (int kk = 0; kk < ACC; ++kk) acc[kk] = 0.0;
   }
-  for (int kp = 0; kp < mkl; ++kp) {
-    const bool kin = kp < np_kl;
-    const int kidx = koff + (kin ? kp : 0);
-    const double* pk = a.prim + 8 * (size_t)kidx;
-    const double akl = pk[0];
-    const double wk = pk[1], wl = pk[2];
-    double qk[3], qb[3];  // Q - R_k, Q - R(ket base)
+  // Each lane walks only its own surviving primitive tuples.  The primitive
+  // pairs of every pair are stored in ascending cce, so the bra primitives
+  // that pass the screen against ket primitive kp are a prefix of the bra
+  // list, and a ket primitive whose first bra primitive fails ends the
+  // tuple (later ket primitives have larger cce).  A wavefront then runs
+  // as many iterations as its lane with the most survivors (tuples are
+  // sorted by survivor count), not the product of the primitive counts.
+  // The bra record of the next candidate is loaded one iteration ahead (its
+  // latency then hides behind the current primitive tuple's work).
+  (void)mij;
+  (void)mkl;
+  int kp = 0, ip = 0;
+  double akl = 0, wk = 0, wl = 0, kkl = 0, hkl = 0, eijcut = 0;
+  double qk[3] = {0, 0, 0}, qb[3] = {0, 0, 0};  // Q - R_k, Q - R(ket base)
+  bool klok = false;
+  auto set_ket = [&](int k) {
+    const double* pk = a.prim + 8 * (size_t)(koff + k);
+    akl = pk[0];
+    wk = pk[1];
+    wl = pk[2];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       qk[d] = wl * rlk[d];
       qb[d] = KBASE ? qk[d] : -wk * rlk[d];
     }
-    const double kkl = pk[4];
-    const double hkl = pk[5];
+    kkl = pk[4];
+    hkl = pk[5];
     const double ccekl = a.three_center ? 0.0 : pk[6] - a.lkl_ceil * logd_kl;
-    const bool klok = active && kin && ccekl <= a.expcutoff;
-    const double eijcut = a.expcutoff - ccekl;
-    for (int ip = 0; ip < mij; ++ip) {
-      const bool iin = ip < np_ij;
-      const int iidx = boff + (iin ? ip : 0);
-      const double* pi = a.prim + 8 * (size_t)iidx;
-      const double cceij = pi[6] - a.lij_ceil * logd_ij;
-      // (libcint zeroes the prefactor of a bra pair with cce >= expcutoff,
-      // optimizer.c:317-341, so it contributes nothing)
-      const bool ok = klok && iin && cceij <= eijcut && cceij < a.expcutoff;
-      if (!ok) continue;
+    klok = ccekl <= a.expcutoff;
+    eijcut = a.expcutoff - ccekl;
+  };
+  double nb[8];  // record of bra candidate ip (prefetched)
+  auto load_bra = [&](int i) {
+    const double* q = a.prim + 8 * (size_t)(boff + i);
+#pragma unroll
+    for (int d = 0; d < 8; ++d) nb[d] = q[d];
+  };
+  if (active && np_kl > 0) {
+    set_ket(0);
+    load_bra(0);
+  }
+  for (;;) {
+    bool have = false;
+    while (active && kp < np_kl) {
+      if (klok && ip < np_ij) {
+        const double cceij = nb[6] - a.lij_ceil * logd_ij;
+        // (libcint zeroes the prefactor of a bra pair with cce >= expcutoff,
+        // optimizer.c:317-341, so it contributes nothing)
+        if (cceij <= eijcut && cceij < a.expcutoff) {
+          have = true;
+          break;
+        }
+      }
+      if (ip == 0) {  // nothing survives from this ket primitive on
+        kp = np_kl;
+        break;
+      }
+      ++kp;
+      ip = 0;
+      if (kp < np_kl) {
+        set_ket(kp);
+        load_bra(0);
+      }
+    }
+    if (!have) break;
+    double pi[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) pi[d] = nb[d];
+    ++ip;
+    if (ip < np_ij) load_bra(ip);
+    {
       PrimTuple p;
       p.aij = pi[0];
       p.akl = akl;

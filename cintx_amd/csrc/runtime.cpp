@@ -468,20 +468,38 @@ struct PairSet {
     for (int d = 0; d < 3; ++d) rr += (A.r[d] - Bs.r[d]) * (A.r[d] - Bs.r[d]);
     double aij_last = B.exps[A.off + A.nprim - 1] + B.exps[Bs.off + Bs.nprim - 1];
     double log_rr0 = 1.7 - 1.5 * std::log(aij_last);
+    // primitive pairs in ascending cce (libcint's jp-major order otherwise,
+    // which only fixes the summation order): the survivors of the screen
+    // cce_ij + cce_kl <= cutoff are then a prefix of each list, which the
+    // one-lane kernels walk without visiting screened primitive tuples
+    std::vector<std::pair<double, int>> ord;
     for (int jp = 0; jp < Bs.nprim; ++jp)
       for (int ip = 0; ip < A.nprim; ++ip) {
         double a_i = B.exps[A.off + ip], a_j = B.exps[Bs.off + jp];
         double ci = B.coefs[A.off + ip], cj = B.coefs[Bs.off + jp];
         double inv = 1.0 / (a_i + a_j);
         double eij = rr * a_i * a_j * inv;
-        aij.push_back(a_i + a_j);
-        hinv.push_back(0.5 * inv);
-        wi.push_back(a_i * inv);
-        wj.push_back(a_j * inv);
-        kfac.push_back(std::exp(-eij) * ci * cj);
-        ai.push_back(a_i);
-        cce.push_back(eij - log_rr0 - std::log(std::fabs(ci)) - std::log(std::fabs(cj)));
+        ord.emplace_back(eij - log_rr0 - std::log(std::fabs(ci)) - std::log(std::fabs(cj)),
+                         jp * A.nprim + ip);
       }
+    std::stable_sort(ord.begin(), ord.end(),
+                     [](const std::pair<double, int>& x, const std::pair<double, int>& y) {
+                       return x.first < y.first;
+                     });
+    for (const auto& o : ord) {
+      const int jp = o.second / A.nprim, ip = o.second % A.nprim;
+      double a_i = B.exps[A.off + ip], a_j = B.exps[Bs.off + jp];
+      double ci = B.coefs[A.off + ip], cj = B.coefs[Bs.off + jp];
+      double inv = 1.0 / (a_i + a_j);
+      double eij = rr * a_i * a_j * inv;
+      aij.push_back(a_i + a_j);
+      hinv.push_back(0.5 * inv);
+      wi.push_back(a_i * inv);
+      wj.push_back(a_j * inv);
+      kfac.push_back(std::exp(-eij) * ci * cj);
+      ai.push_back(a_i);
+      cce.push_back(o.first);
+    }
     for (int d = 0; d < 3; ++d) geo.push_back(A.r[d]);
     for (int d = 0; d < 3; ++d) geo.push_back(Bs.r[d]);
     geo.push_back(std::log(std::sqrt(rr) + 1.0));
@@ -738,29 +756,79 @@ static void plan_add_tuples(cintx_plan* P, const int* tuples, int64_t nt, int pr
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Primitive tuples of (bra, ket) that pass the screen of a class with
+// l-sums lij, lkl (cce_ij + cce_kl <= cutoff, cce_ij < cutoff, each with
+// its l log(|R|+1) term; no ket term for one-Gaussian kets).  The
+// primitive pairs are stored in ascending cce, so the survivors against
+// each ket primitive are a prefix of the bra list (two pointers).
+static int64_t surviving_prims(const PairSet& S, int bra, int ket, int lij, int lkl, bool three,
+                               double cutoff) {
+  const double* cb = S.cce.data() + S.off[bra];
+  const double* ck = S.cce.data() + S.off[ket];
+  const double tb = lij * S.geo[8 * (size_t)bra + 6], tk = lkl * S.geo[8 * (size_t)ket + 6];
+  int n = S.np[bra];
+  int64_t total = 0;
+  for (int kp = 0; kp < S.np[ket]; ++kp) {
+    const double ccekl = three ? 0.0 : ck[kp] - tk;
+    if (!(ccekl <= cutoff)) break;
+    const double cut = cutoff - ccekl;
+    while (n > 0 && !(cb[n - 1] - tb <= cut && cb[n - 1] - tb < cutoff)) --n;
+    if (n == 0) break;
+    total += n;
+  }
+  return total;
+}
+
+static const KernelInfo* bin_kernels(const cintx_plan* P, const Bin& b);
+
 // Lay out and upload pairs, bins and offsets in one device blob.
 // Order every bin's tuples by (bra, ket) primitive-pair counts (counting
-// sort) so that the tuples sharing a workgroup have equal loop trip counts.
+// sort), so that the tuples sharing a workgroup of the wide kernels have
+// equal loop trip counts; for a class with a one-lane kernel, then (stable)
+// by surviving primitive tuples, so that the lanes of a wavefront, which
+// each walk their own survivors, run equal numbers of iterations.
 static void plan_sort_bins(cintx_plan* P) {
   const PairSet& S = P->pairs;
+  const FamilyInfo& f = kFam[P->family];
   std::vector<std::vector<int64_t>> remap(P->bins.size());
   for (size_t bi = 0; bi < P->bins.size(); ++bi) {
     Bin& b = P->bins[bi];
     const int64_t n = (int64_t)b.q.size();
-    const int K = 65 * 65;
-    std::vector<int64_t> count(K + 1, 0);
-    auto key = [&](const int2& v) {
-      return std::min(S.np[v.x], 64) * 65 + std::min(S.np[v.y], 64);
+    auto counting_sort = [&](int K, const std::vector<int>& key, std::vector<int64_t>& pos) {
+      std::vector<int64_t> count((size_t)K + 1, 0);
+      for (int64_t i = 0; i < n; ++i) ++count[(size_t)key[(size_t)i] + 1];
+      for (int k = 0; k < K; ++k) count[k + 1] += count[k];
+      pos.resize((size_t)n);
+      for (int64_t i = 0; i < n; ++i) pos[(size_t)i] = count[(size_t)key[(size_t)i]]++;
     };
-    for (const int2& v : b.q) ++count[key(v) + 1];
-    for (int k = 0; k < K; ++k) count[k + 1] += count[k];
+    // first by primitive counts, then (stable) by survivors
+    std::vector<int> key((size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+      key[(size_t)i] = std::min(S.np[b.q[i].x], 64) * 65 + std::min(S.np[b.q[i].y], 64);
+    std::vector<int64_t> pos1, pos2;
+    counting_sort(65 * 65, key, pos1);
+    std::vector<int2> by_np((size_t)n);
+    for (int64_t i = 0; i < n; ++i) by_np[(size_t)pos1[(size_t)i]] = b.q[i];
+    bool one_lane = false;
+    if (const KernelInfo* ks = bin_kernels(P, b))
+      for (int v = 0; v < kVariants; ++v)
+        one_lane = one_lane || (ks[v].fn && ks[v].lanes_per_tuple == 1);
+    if (!one_lane) {
+      remap[bi] = pos1;
+      b.q.swap(by_np);
+      continue;
+    }
+    const int lij = b.l[0] + b.l[1] + f.deriv, lkl = b.l[2] + b.l[3];
+    constexpr int kSurvCap = 1 << 16;
+    for (int64_t i = 0; i < n; ++i)
+      key[(size_t)i] = (int)std::min<int64_t>(
+          surviving_prims(S, by_np[i].x, by_np[i].y, lij, lkl, f.three, P->expcutoff),
+          kSurvCap - 1);
+    counting_sort(kSurvCap, key, pos2);
     std::vector<int2> sorted((size_t)n);
     remap[bi].resize((size_t)n);
-    for (int64_t i = 0; i < n; ++i) {
-      int64_t d = count[key(b.q[i])]++;
-      sorted[(size_t)d] = b.q[i];
-      remap[bi][(size_t)i] = d;
-    }
+    for (int64_t i = 0; i < n; ++i) sorted[(size_t)pos2[(size_t)i]] = by_np[(size_t)i];
+    for (int64_t i = 0; i < n; ++i) remap[bi][(size_t)i] = pos2[(size_t)pos1[(size_t)i]];
     b.q.swap(sorted);
   }
   for (VQ& v : P->vqs) v.idx = remap[v.bin][(size_t)v.idx];
@@ -2600,6 +2668,7 @@ int cintx_rys_mlp_eval(const float* params, int n, const double* x, int64_t nx, 
                        double* ms) {
   return guarded([&] {
     if (n < 1 || n > 13) fail(CINTX_STATUS_INVALID_INPUT, "nroots must be in 1..13");
+    if (init_mode < 0 || init_mode > 3) fail(CINTX_STATUS_INVALID_INPUT, "init_mode must be 0..3");
     if (!params || !x || !u || !w || !status) fail(CINTX_STATUS_NULL_POINTER, "null argument");
     DevCtx& C = ctx();
     const int np = mlp_nparam();
