@@ -323,8 +323,11 @@ def profiled_traffic(kname, tuples):
 def learned_roots_report(nsamples=1 << 22, epochs=8):
     """Train the MLP initializer on the device (2^22 samples, x log-uniform in
     [1e-6, 1e3], n = 1..13) and compare, per n, two Newton steps started from
-    the network with the same steps started from the asymptotic rule; the
-    table rule judges both at 1e-14 relative (fallback otherwise)."""
+    the network with the same steps started from the asymptotic rule and
+    from the table rule itself (the floor of the Newton polish: FP64 Newton
+    on the moment equations cannot hold 1e-14 for n >= 4 even from exact
+    roots), and time the table rule alone (the method without the network);
+    the table rule judges every polished rule at 1e-14 relative."""
     from cintx_amd import rys_mlp
     from oracle import oracle as O
     t0 = time.perf_counter()
@@ -338,6 +341,8 @@ def learned_roots_report(nsamples=1 << 22, epochs=8):
                                                with_error=True)
         _, _, st0, _, e0 = rys_mlp.evaluate(params, n, x, newton_steps=2, init_mode=1,
                                             with_error=True)
+        _, _, st3, _, e3 = rys_mlp.evaluate(params, n, x, newton_steps=2, init_mode=3,
+                                            with_error=True)
         _, _, _, ms_tab = rys_mlp.evaluate(params, n, x, newton_steps=0, init_mode=2)
         err = 0.0
         for k in range(0, len(x), 2048):
@@ -347,14 +352,23 @@ def learned_roots_report(nsamples=1 << 22, epochs=8):
         per_n[n] = {"settled": float(st.mean()), "settled_without_network": float(st0.mean()),
                     "median_polish_err": float(np.median(e)),
                     "median_polish_err_without_network": float(np.median(e0)),
+                    "settled_from_exact_start": float(st3.mean()),
+                    "median_polish_err_from_exact_start": float(np.median(e3)),
                     "fallback_rate": float(1 - st.mean()), "max_rel_err_vs_oracle": err,
-                    "ms_mlp_newton_judge": ms_mlp, "ms_table": ms_tab}
-    summ = ", ".join(f"n={n}: {v['settled']:.2f}/{v['settled_without_network']:.2f} "
-                     f"({v['median_polish_err']:.1e}/{v['median_polish_err_without_network']:.1e})"
+                    "ms_mlp_newton_judge": ms_mlp, "ms_table": ms_tab,
+                    "table_speedup_over_mlp": ms_mlp / ms_tab if ms_tab > 0 else None}
+    summ = ", ".join(f"n={n}: {v['settled']:.2f}/{v['settled_without_network']:.2f}/"
+                     f"{v['settled_from_exact_start']:.2f} "
+                     f"({v['median_polish_err']:.1e}/{v['median_polish_err_without_network']:.1e}/"
+                     f"{v['median_polish_err_from_exact_start']:.1e})"
                      for n, v in per_n.items())
     return {"train_samples": nsamples, "epochs": epochs, "final_loss": loss,
             "train_s": train_s, "eval_points": len(x), "per_nroots": per_n,
-            "summary": "settled fraction (median polish error) with/without network: " + summ}
+            "summary": "settled fraction (median polish error) from network / asymptotic "
+                       "rule / exact table roots: " + summ,
+            "conclusion": "not wired into the ERI kernels: the table rule is exact and faster "
+                          "than network + Newton + judge, and the moment-equation Newton "
+                          "polish misses 1e-14 for n >= 4 even from the exact roots"}
 
 
 def dominant(plan, steps, real):

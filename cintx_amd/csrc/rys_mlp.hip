@@ -302,7 +302,14 @@ __global__ void mlp_eval_kernel(const float* __restrict__ prm, RysTable tab, int
   double err = -1.0;  // mode 2: not polished
   double s[kMaxRoots], ww[kMaxRoots];
   if (mode != 2) {
-    mlp_initial(sp, tab, n, xi, mode, s, ww);
+    if (mode == 3) {  // start from the table rule itself: the polish's own floor
+      for (int r = 0; r < n; ++r) {
+        s[r] = ut[r] / (1.0 + ut[r]);
+        ww[r] = wt[r];
+      }
+    } else {
+      mlp_initial(sp, tab, n, xi, mode, s, ww);
+    }
     err = INFINITY;
     if (newton_moments(n, xi, s, ww, steps)) {
       err = 0;

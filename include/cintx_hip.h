@@ -314,7 +314,8 @@ int cintx_rys_mlp_train(float *params, int64_t nsamples, int epochs, int batch, 
 int cintx_rys_mlp_grad(const float *params, const float *z, const int *nroots,
                        const float *targets, int batch, float *grad, float *loss);
 /* init_mode 0: network, 1: asymptotic rule (the initializer without the
- * network), 2: the table rule alone (no Newton; timing reference).  After
+ * network), 2: the table rule alone (no Newton; timing reference), 3: the
+ * table rule as the start (the floor of the Newton polish itself).  After
  * newton_steps FP64 Newton steps on the moment equations the table rule
  * (the plain solver) judges the polished rule: status[i] = 1 when every
  * root u_r and weight w_r is within 1e-14 relative of it (the polished rule

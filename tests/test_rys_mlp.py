@@ -49,11 +49,16 @@ def test_device_gradient_matches_cpu_backprop():
 
 
 @pytest.mark.gpu
-def test_trained_network_gains_over_newton_without_it(oracle):
-    """Ablation: the same two FP64 Newton steps on the moment equations,
-    started from the trained network or from the asymptotic (large-x) rule;
-    the table rule judges both at 1e-14 relative and is returned when the
-    polished rule misses."""
+def test_learned_initializer_ablation(oracle):
+    """Ablation of the learned initializer.  Two FP64 Newton steps on the
+    moment equations, started from (a) the trained network, (b) the
+    asymptotic (large-x) rule, (c) the table rule's own roots; the table
+    rule (the method without the network) judges each at 1e-14 relative and
+    is returned when the polished rule misses.  Findings the asserts hold:
+    the network starts Newton far closer than the asymptotic rule, but the
+    polish itself cannot keep 1e-14 for n >= 4 even from exact roots (the
+    moment system's conditioning), so the table rule, exact and faster, is
+    what the ERI kernels use."""
     p, loss = R.train(nsamples=1 << 20, epochs=8, batch=1024, lr=3e-3)
     print("training loss", loss)
     assert loss < 0.05
@@ -61,19 +66,28 @@ def test_trained_network_gains_over_newton_without_it(oracle):
     x = np.exp(rng.uniform(np.log(1e-3), np.log(60.0), 400))
     report = {}
     for n in (1, 2, 3, 4, 5):
-        u, w, st, _, err = R.evaluate(p, n, x, newton_steps=2, init_mode=0, with_error=True)
+        u, w, st, ms_net, err = R.evaluate(p, n, x, newton_steps=2, init_mode=0, with_error=True)
         _, _, st0, _, err0 = R.evaluate(p, n, x, newton_steps=2, init_mode=1, with_error=True)
-        med, med0 = np.median(err), np.median(err0)
-        report[n] = (float(st.mean()), float(st0.mean()), float(med), float(med0))
+        _, _, st3, _, err3 = R.evaluate(p, n, x, newton_steps=2, init_mode=3, with_error=True)
+        _, _, _, ms_tab = R.evaluate(p, n, x, newton_steps=0, init_mode=2)
+        med, med0, med3 = np.median(err), np.median(err0), np.median(err3)
+        report[n] = (float(st.mean()), float(st0.mean()), float(st3.mean()), float(med),
+                     float(med0), float(med3), ms_net, ms_tab)
         # whatever the path, the returned rule is the oracle's to 1e-12
         for k in range(0, len(x), 37):
             uo, wo = oracle.rys_roots(n, x[k])
             assert np.max(np.abs(u[k] - uo) / uo) < 1e-12
             assert np.max(np.abs(w[k] - wo) / wo) < 1e-12
-        # one root: Newton converges from either start; from two roots on,
-        # Newton from the asymptotic rule mostly leaves the domain below
-        # x ~ 35 while the network's start stays within reach of the rule
+        # the network's start is never worse than the asymptotic rule's, and
+        # from two roots on it is orders of magnitude closer after the polish
         assert st.mean() >= st0.mean() - 0.02, (n, st.mean(), st0.mean())
         if n >= 2:
             assert med < 1e-4 and med < 1e-2 * med0, (n, med, med0)
-    print("settled (network, asymptotic), median polish error (network, asymptotic):", report)
+        # no start settles more often than the exact roots do
+        assert st.mean() <= st3.mean() + 0.02, (n, st.mean(), st3.mean())
+    # the polish floor: exact roots stay exact for one root and drift past
+    # the 1e-14 gate from four roots on
+    assert report[1][5] <= 1e-15
+    assert report[4][5] > 1e-14 and report[5][5] > 1e-14, report
+    print("n: settled (network, asymptotic, exact start), median polish error (same), "
+          "ms (network path, table rule):", report)
