@@ -124,8 +124,8 @@ def main():
     log(f"sampled max abs err vs the judge: fp64 {err64:.3e}, mixed {errmx:.3e}")
 
     # J/K against a random symmetric density (N(0,1)/nbf): the FP64 plan's
-    # quartets digested with their 8-fold images (unfused: the plan's output
-    # is written, then read by the digest kernels)
+    # quartets digested with their 8-fold images inside the class kernels
+    # (fused: no ERI block is written)
     nao = mol.nao()
     rngd = np.random.default_rng(2)
     dens = rngd.normal(size=(nao, nao)) / nao
@@ -134,7 +134,24 @@ def main():
     t0 = time.perf_counter()
     J64, K64 = p64.jk(dens)
     jk_ms = (time.perf_counter() - t0) * 1e3
-    log(f"J/K build (plan + digest, host timed): {jk_ms:.1f} ms")
+    log(f"J/K build (host arrays, copies included): {jk_ms:.1f} ms")
+    # the same on device-resident D, J, K (the allocation-free entry point):
+    # the J/K step itself
+    from cintx_amd import capi as C
+    dD, dJ, dK = C.DeviceArray(nao * nao), C.DeviceArray(nao * nao), C.DeviceArray(nao * nao)
+    dD.upload(dens)
+    jk_dev = []
+    for _ in range(4):
+        t0 = time.perf_counter()
+        p64.jk_noalloc(dD.address, dJ.address, dK.address)
+        p64.sync()
+        jk_dev.append((time.perf_counter() - t0) * 1e3)
+    jk_dev_ms = float(np.median(jk_dev[1:]))
+    assert np.abs(dJ.download().reshape(nao, nao) - J64).max() <= 1e-12 * max(1.0, np.abs(J64).max())
+    for a in (dD, dJ, dK):
+        a.free()
+    log(f"J/K step on device-resident matrices: {jk_dev_ms:.1f} ms "
+        f"({p64.ntuples / (jk_dev_ms / 1e3):.3e} quartets/s with their J/K)")
     # the mixed plan's J/K against the FP64 plan's (whose ERIs match the
     # oracle to the sampled error above)
     Jm, Km = pmx.jk(dens)
@@ -197,9 +214,12 @@ def main():
                  "wall_ms_per_step": wall64, "gflops": achieved_step * 1e3,
                  "prim_quartets": prim64, "root_evals": roots64},
         "mixed_over_fp64": value / fp64_rate,
-        "jk": {"ms": jk_ms, "nbf": nao, **jk_err,
+        "jk": {"ms": jk_ms, "device_ms": jk_dev_ms,
+               "quartets_per_s_with_jk": p64.ntuples / (jk_dev_ms / 1e3), "nbf": nao, **jk_err,
                "note": "J and K from the FP64 plan's unique quartets and a random symmetric "
-                       "density; plan execution + digest kernels + 64 MB copies, host-timed"},
+                       "density, digested inside the class kernels (no ERI block written); "
+                       "ms: host arrays (64 MB copies each way), device_ms: device-resident "
+                       "D/J/K through cintx_plan_jk_noalloc, host-timed around a sync"},
         "fp64_max_abs_err_vs_judge": err64,
         "mixed_max_abs_err_vs_judge": errmx,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,

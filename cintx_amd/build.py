@@ -169,6 +169,11 @@ def generate(nparts=24, nchunkparts=12):
     for deriv, prec, ls in classes:
         for v, (tt, pp) in enumerate(variants(deriv, prec, ls)):
             (chunked if (tt == 1 and (pp & 255) > 1) else plain).append((deriv, prec, ls, v, tt, pp))
+            # the same launch shape with the fused J/K epilogue (PP bit 12):
+            # int2e_sph classes in FP64 and FP32
+            if deriv == 0 and prec in (0, 1):
+                (chunked if (tt == 1 and (pp & 255) > 1) else plain).append(
+                    (deriv, prec, ls, v, tt, pp | 4096))
     parts = [plain[i::nparts] for i in range(nparts)]
     parts += [chunked[i::nchunkparts] for i in range(nchunkparts)]
     nparts = len(parts)
@@ -178,7 +183,8 @@ def generate(nparts=24, nchunkparts=12):
                  f"void register_part_{p}(Registry& reg) {{"]
         for deriv, prec, (a, b, c, d), v, tt, pp in cls:
             real = "float" if prec == 1 else "double"
-            lines.append(f"  reg.k[{deriv}][{prec}][{a}][{b}][{c}][{d}][{v}] = "
+            slot = (f"jk[{prec}]" if pp & 4096 else f"k[{deriv}][{prec}]") + f"[{a}][{b}][{c}][{d}][{v}]"
+            lines.append(f"  reg.{slot} = "
                          f"class_info<{a}, {b}, {c}, {d}, {deriv}, {real}, {tt}, {pp}>();")
         lines += ["}", "}  // namespace cx", ""]
         stem = f"inst_{p:02d}" if p < len(parts) - nchunkparts else f"inst_reg_{p:02d}"
@@ -204,6 +210,7 @@ def generate(nparts=24, nchunkparts=12):
 
 def _compile(src, obj, extra):
     deps = [src, os.path.join(CSRC, "eri_device.h"), os.path.join(CSRC, "common.h"),
+            os.path.join(CSRC, "jk_device.h"),
             os.path.join(CSRC, "launch.h"), os.path.join(CSRC, "rys_table.h"),
             os.path.join(HERE, "..", "include", "cintx_hip.h")]
     if os.path.exists(obj):

@@ -6,8 +6,9 @@
 //                 contraction column, which is exact by linearity.
 //   pairs       : one record per shell pair (i, j) that a plan needs.
 //   prim pairs  : one 64-byte record per primitive pair of all pairs,
-//                 contiguous per pair, ordered jp-major/ip-minor like
-//                 CINTset_pairdata (libcint-master/src/optimizer.c:317-341).
+//                 contiguous per pair, in ascending screening key cce
+//                 (CINTset_pairdata's jp-major order, libcint-master/src/
+//                 optimizer.c:317-341, only breaks ties).
 //   quartets    : (bra pair, ket pair) int32 tuples, binned by class.
 #pragma once
 #include <stdint.h>
@@ -69,6 +70,15 @@ struct EriArgs {
   const double* c2s;       // [kC2sL+1][kC2sM][kC2sC]
   double* scratch;         // generic kernel only
   unsigned long long* counters;  // optional: [0] surviving primitive tuples
+  // fused J/K (int2e_sph blocks of unique quartets): when jk_D is set the
+  // class kernel digests each block into the accumulators jk_J / jk_K
+  // (halves of J and K, symmetrised afterwards) instead of writing it
+  const double* jk_D;      // symmetric density, nao x nao
+  double* jk_J;
+  double* jk_K;
+  const int2* pair_sh;     // original shells (a, b) of every pair
+  const int* ao_off;       // first spherical function of every shell
+  int nao;
 };
 
 // ------------------------------------------------------------------

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "common.h"
+#include "jk_device.h"
 #include "launch.h"
 
 namespace cx {
@@ -390,12 +391,29 @@ struct ClassShape {
   static constexpr int MINW = ((PP >> 9) & 3) + 1;
   // PP bit 11: Cartesian output (the _cart families), no cart -> sph
   static constexpr bool CART = (PP >> 11) & 1;
+  // PP bit 12: fused J/K epilogue (int2e_sph only): blocks are digested into
+  // J and K instead of written
+  static constexpr bool JK = (PP >> 12) & 1;
+  static_assert(!JK || (!DERIV && !CART), "fused J/K serves int2e_sph only");
   static constexpr int nout(int l) { return CART ? ncart(l) : nsph(l); }
   static constexpr int RPC = (NROW + NCH - 1) / NCH;  // rows per chunk
   static constexpr int G = T >= NROW ? T / NROW : 1;               // groups per row
   static constexpr int RPL = T >= NROW ? 1 : (NROW + T - 1) / T;   // rows per lane
   static constexpr int NFT = (NF + T - 1) / T;                     // T == 1 path
   static constexpr int NSC = 16;  // per-primitive scalars shared in LDS
+  // one-lane path, fused J/K: (k, l) output pairs staged per pass for the
+  // 256 lanes of a workgroup within 128 KB of dynamic LDS (the static LDS
+  // of these kernels, the Rys table block, stays below 30 KB)
+  // (never more than the (k, l) pairs a chunk reaches, so that small
+  // classes keep the occupancy of their plain kernels)
+  static constexpr int jk_rows_per_pass() {
+    const int rpp = 64 / (nout(LI) * nout(LJ));
+    const int reach = (LK < 2 && LL < 2) ? RPC : nout(LK) * nout(LL);
+    return rpp < 1 ? 1 : (rpp < reach ? rpp : reach);
+  }
+  static constexpr size_t jk_lds_bytes() {
+    return (size_t)jk_rows_per_pass() * nout(LI) * nout(LJ) * 256 * sizeof(double);
+  }
   // the Cartesian/spherical staging buffers of the epilogue alias the g slots
   template <typename R>
   static constexpr int union_doubles() {  // per tuple: g slots | staging
@@ -698,8 +716,9 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
   }
   // cart -> sph of the chunk, in registers: the row's own axes (i, j; i only
   // for nabla_i rows) per row, then the row axes as a weighted sum over the
-  // chunk's rows, added to what earlier chunks wrote
-  if (active) {
+  // chunk's rows, added to what earlier chunks wrote (or, for a fused J/K,
+  // staged in LDS and digested)
+  {
     constexpr bool CART = S::CART;
     constexpr int NSI = S::nout(LI), NSJ = S::nout(LJ), NSK = S::nout(LK), NSL = S::nout(LL);
     constexpr int NSB = DERIV ? NSI : NSI * NSJ;  // output size of a row
@@ -720,6 +739,108 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
           c2s_reg<LJ, NSI, 1>(t1, d + rl * NSB, a.c2s);
         }
       }
+      // The chunk's share of output (ml, mk, mj, mb).  s and p row axes map
+      // rows to outputs one to one: an output only takes rows whose
+      // components match there, so a chunk reaches just some outputs (all
+      // compile-time): `mine` when any of its rows reach the output,
+      // `earlier` when a row of an earlier chunk did.
+      auto out_value = [&](int ml, int mk, int mj, int mb, bool& mine, bool& earlier) -> double {
+        auto reaches = [&](int row) {
+          int fj = 0, fk, fl;
+          if constexpr (DERIV) {
+            fj = row % NFJ;
+            fk = (row / NFJ) % NFK;
+            fl = row / (NFJ * NFK);
+          } else {
+            fk = row % NFK;
+            fl = row / NFK;
+          }
+          return ((!CART && LK >= 2) || fk == mk) && ((!CART && LL >= 2) || fl == ml) &&
+                 (!DERIV || (!CART && LJ >= 2) || fj == mj);
+        };
+        mine = false;
+        earlier = false;
+#pragma unroll
+        for (int row = 0; row < NROW; ++row) {
+          if (!reaches(row)) continue;
+          if (row / RPC == ch) mine = true;
+          if (row / RPC < ch) earlier = true;
+        }
+        double v = 0;
+        if (!mine) return v;
+#pragma unroll
+        for (int rl = 0; rl < RPC; ++rl) {
+          const int row = ch * RPC + rl;
+          if (row >= NROW) continue;
+          int fj = 0, fk, fl;
+          if constexpr (DERIV) {
+            fj = row % NFJ;
+            fk = (row / NFJ) % NFK;
+            fl = row / (NFJ * NFK);
+          } else {
+            fk = row % NFK;
+            fl = row / NFK;
+          }
+          double cf;
+          if constexpr (CART) {
+            cf = (fk == mk && fl == ml && (!DERIV || fj == mj)) ? 1.0 : 0.0;
+          } else {
+            cf = c2s_coef(a.c2s, LK, mk, fk) * c2s_coef(a.c2s, LL, ml, fl);
+            if constexpr (DERIV) cf *= c2s_coef(a.c2s, LJ, mj, fj);
+          }
+          v += cf * d[rl * NSB + mb];
+        }
+        return v;
+      };
+      if constexpr (S::JK) {
+        {
+          // fused J/K: this chunk's outputs are staged in LDS (lane
+          // interleaved, a range of (k, l) output pairs per pass) and each
+          // lane digests its own rows (the reductions are linear, so the
+          // chunks' shares and the passes digest separately); the block
+          // never reaches global memory.  With s and p row axes a chunk
+          // reaches the (k, l) pairs of its own rows only.
+          constexpr int NKL = NSK * NSL;
+          constexpr bool ONE_TO_ONE = LK < 2 && LL < 2;
+          const int KL0 = ONE_TO_ONE ? ch * RPC : 0;
+          const int KL1 = ONE_TO_ONE ? cmin(ch * RPC + RPC, NKL) : NKL;
+          constexpr int RPP = S::jk_rows_per_pass();
+          // one digest site: one chunk, one pass (the loops are unrolled)
+          constexpr bool ONE_SITE = NCH == 1 && (ONE_TO_ONE ? RPC : NKL) <= RPP;
+          extern __shared__ double jks[];
+          const int sa = a.pair_sh[pq.x].x, sbb = a.pair_sh[pq.x].y;
+          const int sc = a.pair_sh[pq.y].x, sd = a.pair_sh[pq.y].y;
+          const int sh[4] = {sa, sbb, sc, sd};
+          const int off[4] = {a.ao_off[sa], a.ao_off[sbb], a.ao_off[sc], a.ao_off[sd]};
+          const int lane = threadIdx.x;
+#pragma unroll
+          for (int r0 = KL0; r0 < KL1; r0 += RPP) {
+            const int nr = cmin(RPP, KL1 - r0);
+#pragma unroll
+            for (int r = 0; r < nr; ++r) {
+              const int mk = (r0 + r) % NSK, ml = (r0 + r) / NSK;
+#pragma unroll
+              for (int mb = 0; mb < NSB; ++mb) {
+                bool mine, earlier;
+                const double v = out_value(ml, mk, 0, mb, mine, earlier);
+                jks[(r * NSB + mb) * 256 + lane] = mine ? v : 0.0;
+              }
+            }
+            __syncthreads();
+            if constexpr (ONE_SITE)
+              jk_digest_rows<NSI, NSJ, NSK, NSL>(lane, 256, r0, nr, active, sh[0], sh[1], sh[2],
+                                                 sh[3], off[0], off[1], off[2], off[3], a.jk_D,
+                                                 a.jk_J, a.jk_K, a.nao);
+            else
+              jk_digest_rows_call<NSI, NSJ, NSK, NSL>(lane, 256, r0, nr, active, sh[0], sh[1],
+                                                      sh[2], sh[3], off[0], off[1], off[2], off[3],
+                                                      a.jk_D, a.jk_J, a.jk_K, a.nao);
+            __syncthreads();
+          }
+          continue;
+        }
+      }
+      if (!active) continue;
       double* o = a.out + a.out_base + q * a.block + comp * a.st_comp;
 #pragma unroll
       for (int ml = 0; ml < NSL; ++ml)
@@ -729,54 +850,9 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
           for (int mj = 0; mj < (DERIV ? NSJ : 1); ++mj)
 #pragma unroll
             for (int mb = 0; mb < NSB; ++mb) {
-              // s and p row axes map rows to outputs one to one: an output
-              // only takes rows whose components match there, so a chunk
-              // writes just the outputs its rows reach (all compile-time),
-              // storing where no earlier chunk did and adding otherwise
-              auto reaches = [&](int row) {
-                int fj = 0, fk, fl;
-                if constexpr (DERIV) {
-                  fj = row % NFJ;
-                  fk = (row / NFJ) % NFK;
-                  fl = row / (NFJ * NFK);
-                } else {
-                  fk = row % NFK;
-                  fl = row / NFK;
-                }
-                return ((!CART && LK >= 2) || fk == mk) && ((!CART && LL >= 2) || fl == ml) &&
-                       (!DERIV || (!CART && LJ >= 2) || fj == mj);
-              };
-              bool mine = false, earlier = false;
-#pragma unroll
-              for (int row = 0; row < NROW; ++row) {
-                if (!reaches(row)) continue;
-                if (row / RPC == ch) mine = true;
-                if (row / RPC < ch) earlier = true;
-              }
+              bool mine, earlier;
+              const double v = out_value(ml, mk, mj, mb, mine, earlier);
               if (!mine) continue;
-              double v = 0;
-#pragma unroll
-              for (int rl = 0; rl < RPC; ++rl) {
-                const int row = ch * RPC + rl;
-                if (row >= NROW) continue;
-                int fj = 0, fk, fl;
-                if constexpr (DERIV) {
-                  fj = row % NFJ;
-                  fk = (row / NFJ) % NFK;
-                  fl = row / (NFJ * NFK);
-                } else {
-                  fk = row % NFK;
-                  fl = row / NFK;
-                }
-                double cf;
-                if constexpr (CART) {
-                  cf = (fk == mk && fl == ml && (!DERIV || fj == mj)) ? 1.0 : 0.0;
-                } else {
-                  cf = c2s_coef(a.c2s, LK, mk, fk) * c2s_coef(a.c2s, LL, ml, fl);
-                  if constexpr (DERIV) cf *= c2s_coef(a.c2s, LJ, mj, fj);
-                }
-                v += cf * d[rl * NSB + mb];
-              }
               const int mi = DERIV ? mb : mb % NSI;
               const int mjj = DERIV ? mj : mb / NSI;
               double* dst = o + mi * a.st[0] + mjj * a.st[1] + mk * a.st[2] + ml * a.st[3];
@@ -1081,7 +1157,18 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
       dst = tmp;
       tuple_sync<T>();
     }
-    if (active) {
+    bool fused = false;
+    if constexpr (S::JK) {
+      if (active) {  // fused J/K: digest the staged block
+        const int2 sb = a.pair_sh[pq.x], sq = a.pair_sh[pq.y];
+        const int sh[4] = {sb.x, sb.y, sq.x, sq.y};
+        const int ns[4] = {NSI, NSJ, NSK, NSL};
+        const int off[4] = {a.ao_off[sb.x], a.ao_off[sb.y], a.ao_off[sq.x], a.ao_off[sq.y]};
+        jk_digest_block(src, ns, sh, off, t, T, a.jk_D, a.jk_J, a.jk_K, a.nao);
+      }
+      fused = true;
+    }
+    if (active && !fused) {
       double* o = a.out + a.out_base + q * a.block + comp * a.st_comp;
       for (int e = t; e < NS; e += T) {
         int mi = e % NSI, e1 = e / NSI;
@@ -1104,8 +1191,14 @@ hipError_t launch_class(const EriArgs& a, hipStream_t s) {
   using S = ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>;
   constexpr int Q = S::template Q<R>();
   const int64_t nblk = (a.nq + Q - 1) / Q;
+  // the one-lane kernels stage their blocks for a fused J/K in dynamic LDS
+  size_t dyn = 0;
+  if constexpr (S::T == 1 && S::JK) {
+    static_assert(S::jk_lds_bytes() <= 128 * 1024, "fused J/K staging");
+    dyn = S::jk_lds_bytes();
+  }
   hipLaunchKernelGGL((eri_class_kernel<LI, LJ, LK, LL, DERIV, R, TT, PP>), dim3((unsigned)nblk),
-                     dim3(S::T * Q), 0, s, a);
+                     dim3(S::T * Q), dyn, s, a);
   return hipGetLastError();
 }
 

@@ -1,41 +1,14 @@
-// Coulomb/exchange digestion of unique shell quartets:
-//   J[p][q] = sum_rs (pq|rs) D[r][s],  K[p][r] = sum_qs (pq|rs) D[q][s]
-// for a symmetric D.  A stored quartet (IJ|KL) stands for its orbit under the
-// 8-fold permutational symmetry; summed over all 8 permutations with weight
-// 1/|stabiliser| (= distinct images / 8) every ordered quartet counts once.
-// With D symmetric the 8 permutations give only six distinct reductions and
-// their transposes:
-//   A[I,J] += 2w sum_kl (ij|kl) D_kl,   A[K,L] += 2w sum_ij (ij|kl) D_ij,
-//   B[I,K] += w sum_jl (ij|kl) D_jl,    B[J,K] += w sum_il (ij|kl) D_il,
-//   B[I,L] += w sum_jk (ij|kl) D_jk,    B[J,L] += w sum_ik (ij|kl) D_ik,
-// and J = A + A^T, K = B + B^T (jk_symmetrize).  Each reduction issues one
-// FP64 atomic per output element.  The block layout is the plan's canonical
-// one ([l][k][j][i], i fastest).
+// Coulomb/exchange digestion of unique shell quartets written by the class
+// kernels (the bins whose kernel has no fused J/K epilogue: the
+// runtime-class kernel), the density symmetrisation and the allocation-free
+// J/K's static scratch.  The reductions themselves are in jk_device.h.
 #include <hip/hip_runtime.h>
 
 #include "../../include/cintx_hip.h"
 #include "common.h"
+#include "jk_device.h"
 
 namespace cx {
-
-// out[(o[a]+ma), (o[b]+mb)] += wgt sum_{mc, md} blk[...] D[(o[c]+mc), (o[d]+md)]
-// over the output elements ln, ln + nl, ...
-__device__ __forceinline__ void jk_reduce(const double* __restrict__ blk, const int* ns,
-                                          const int* st, const int* off, int a, int b, int c,
-                                          int d, double wgt, const double* __restrict__ D,
-                                          double* __restrict__ out, int nao, int ln, int nl) {
-  const int na = ns[a], nb = ns[b], nc = ns[c], nd = ns[d];
-  for (int o = ln; o < na * nb; o += nl) {
-    const int ma = o % na, mb = o / na;
-    const double* e = blk + ma * st[a] + mb * st[b];
-    double sum = 0;
-    for (int md = 0; md < nd; ++md) {
-      const double* drow = D + (size_t)(off[d] + md) + (size_t)off[c] * nao;
-      for (int mc = 0; mc < nc; ++mc) sum += e[mc * st[c] + md * st[d]] * drow[(size_t)mc * nao];
-    }
-    if (sum != 0.0) atomicAdd(&out[(size_t)(off[a] + ma) * nao + off[b] + mb], wgt * sum);
-  }
-}
 
 // The digestion of quartet t by `nl` cooperating lanes (lane index `ln`).
 __device__ __forceinline__ void jk_digest_one(
@@ -48,20 +21,7 @@ __device__ __forceinline__ void jk_digest_one(
   const int ns[4] = {2 * shl_l[sh[0]] + 1, 2 * shl_l[sh[1]] + 1, 2 * shl_l[sh[2]] + 1,
                      2 * shl_l[sh[3]] + 1};
   const int off[4] = {ao_off[sh[0]], ao_off[sh[1]], ao_off[sh[2]], ao_off[sh[3]]};
-  // distinct shell-level images of (IJ|KL): 8 / |stabiliser|
-  const bool ij = sh[0] == sh[1], kl = sh[2] == sh[3];
-  const bool braket = (sh[0] == sh[2] && sh[1] == sh[3]) || (sh[0] == sh[3] && sh[1] == sh[2]);
-  int stab = (ij ? 2 : 1) * (kl ? 2 : 1);
-  if (braket) stab *= 2;
-  const double w = 1.0 / stab;
-  const double* blk = eri + out_base + t * block;
-  const int st[4] = {1, ns[0], ns[0] * ns[1], ns[0] * ns[1] * ns[2]};
-  if (!skip_bra_j) jk_reduce(blk, ns, st, off, 0, 1, 2, 3, 2 * w, D, J, nao, ln, nl);
-  jk_reduce(blk, ns, st, off, 2, 3, 0, 1, 2 * w, D, J, nao, ln, nl);
-  jk_reduce(blk, ns, st, off, 0, 2, 1, 3, w, D, K, nao, ln, nl);
-  jk_reduce(blk, ns, st, off, 1, 2, 0, 3, w, D, K, nao, ln, nl);
-  jk_reduce(blk, ns, st, off, 0, 3, 1, 2, w, D, K, nao, ln, nl);
-  jk_reduce(blk, ns, st, off, 1, 3, 0, 2, w, D, K, nao, ln, nl);
+  jk_digest_block(eri + out_base + t * block, ns, sh, off, ln, nl, D, J, K, nao, skip_bra_j);
 }
 
 // Large blocks: one 64-lane workgroup per quartet.
@@ -106,9 +66,7 @@ __global__ void __launch_bounds__(256) jk_digest_lane_kernel(
       nl = 2 * shl_l[sl] + 1;
       ok = ao_off[sk];
       ol = ao_off[sl];
-      const bool braket = (si == sk && sj == sl) || (si == sl && sj == sk);
-      const int stab = (si == sj ? 2 : 1) * (sk == sl ? 2 : 1) * (braket ? 2 : 1);
-      wgt = 2.0 / stab;
+      wgt = 2.0 * jk_weight(si, sj, sk, sl);
       blk = eri + out_base + t * block;
     }
     const int st2 = ni * nj, st3 = ni * nj * nk;

@@ -24,6 +24,8 @@ constexpr int kForms = 3;
 
 struct Registry {
   KernelInfo k[2][kForms][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1][kVariants];
+  // int2e_sph launch shapes with the fused J/K epilogue, FP64 and FP32 forms
+  KernelInfo jk[2][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1][kVariants];
 };
 
 // Filled by the generated instantiation units.

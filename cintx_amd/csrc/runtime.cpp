@@ -295,6 +295,7 @@ struct DevCtx {
   double* d_spinor = nullptr;  // Cartesian -> spinor table (complex)
   Registry* reg = nullptr;
   double* d_jk_scratch = nullptr;  // the static scratch of cintx_plan_jk_noalloc
+  hipEvent_t jk_ev[kChains] = {};  // fork/join events of cintx_plan_jk_noalloc's chains
   // autotuned class-kernel variant per (deriv, form, class); -1: untuned
   signed char tuned[2][kForms][kMaxL + 1][kMaxL + 1][kMaxL + 1][kMaxL + 1];
 };
@@ -333,8 +334,12 @@ static DevCtx& ctx() {
   build_spinor_table(c2s, sp);
   HIPCHK(hipMalloc(&c->d_spinor, sp.size() * sizeof(double)));
   HIPCHK(hipMemcpy(c->d_spinor, sp.data(), sp.size() * sizeof(double), hipMemcpyHostToDevice));
-  // address of the allocation-free J/K's static scratch on this device
+  // address of the allocation-free J/K's static scratch on this device, and
+  // the events its chains fork and join on (made here, so that the call
+  // itself creates nothing)
   HIPCHK(jk_scratch_address(&c->d_jk_scratch));
+  for (int k = 0; k < kChains; ++k)
+    HIPCHK(hipEventCreateWithFlags(&c->jk_ev[k], hipEventDisableTiming));
   c->reg = new Registry();
   std::memset(c->reg, 0, sizeof(Registry));
   register_class_kernels(*c->reg);
@@ -1035,6 +1040,16 @@ static const KernelInfo& variant_info(const cintx_plan* P, const Bin& b, int v) 
   return ctx().reg->k[f.deriv][0][b.l[0]][b.l[1]][b.l[2]][b.l[3]][v - kVariants];
 }
 
+// The fused J/K build of variant v (same launch shape; none for other
+// families, runtime-class bins or shapes without an instantiation).
+static const KernelInfo& variant_info_jk(const cintx_plan* P, const Bin& b, int v) {
+  static const KernelInfo none{};
+  if (P->family != 0 || b.generic || v < 0 || v >= 2 * kVariants) return none;
+  if (v < kVariants) return ctx().reg->jk[b.prec][b.l[0]][b.l[1]][b.l[2]][b.l[3]][v];
+  if (b.prec != 1) return none;
+  return ctx().reg->jk[0][b.l[0]][b.l[1]][b.l[2]][b.l[3]][v - kVariants];
+}
+
 static signed char& tuned_slot(const cintx_plan* P, const Bin& b) {
   const FamilyInfo& f = kFam[P->family];
   return ctx().tuned[f.deriv][bin_form(P, b)][b.l[0]][b.l[1]][b.l[2]][b.l[3]];
@@ -1068,11 +1083,21 @@ struct PairDev {
 // slots) and form prec, blocks of `block` elements written from out_base.
 // Variant `k` of the compiled class kernels, or the runtime-class kernel
 // (generic) with `scratch` of kGenericGrid slices.
+// Fused J/K of a launch: the class kernel digests its blocks into J/K.
+struct JkFuse {
+  const double* D;
+  double* J;
+  double* K;
+  const int2* pair_sh;
+  const int* ao_off;
+  int nao;
+};
+
 static void launch_bin_kernels(int fam, const int* l, int variant, bool generic,
                                const KernelInfo& k, const PairDev& pd, double expcutoff,
                                const int2* d_q, int64_t nq, double* out, int64_t out_base,
                                int64_t block, double* scratch, unsigned long long* counters,
-                               hipStream_t st) {
+                               hipStream_t st, const JkFuse* jk = nullptr) {
   DevCtx& C = ctx();
   const FamilyInfo& f = kFam[fam];
   const double common_base = (M_PI * M_PI * M_PI) * 2.0 / std::sqrt(M_PI);
@@ -1101,6 +1126,16 @@ static void launch_bin_kernels(int fam, const int* l, int variant, bool generic,
   a.c2s = C.d_c2s;
   a.counters = counters;
   a.scratch = scratch;
+  if (jk) {
+    if (generic || f.deriv || f.cart || f.three || f.two)
+      fail(CINTX_STATUS_PANIC, "fused J/K needs an int2e_sph class kernel");
+    a.jk_D = jk->D;
+    a.jk_J = jk->J;
+    a.jk_K = jk->K;
+    a.pair_sh = jk->pair_sh;
+    a.ao_off = jk->ao_off;
+    a.nao = jk->nao;
+  }
   // a dispatch holds at most 2^32 - 1 work-items per grid dimension: launch
   // in chunks of at most 2^31 lanes (2^28 tuples for the runtime-class
   // kernel, whose grid is capped anyway)
@@ -1134,17 +1169,18 @@ static void launch_bin_kernels(int fam, const int* l, int variant, bool generic,
 }
 
 static void plan_launch_bin(cintx_plan* P, size_t i, double* out, bool count,
-                            hipStream_t st = nullptr, int chain = 0) {
+                            hipStream_t st = nullptr, int chain = 0,
+                            const JkFuse* jk = nullptr) {
   DevCtx& C = ctx();
   if (!st) st = C.stream;
   Bin& b = P->bins[i];
   if (kFam[P->family].spinor) out = P->d_cart;  // the transform reads them from there
-  const KernelInfo& k = variant_info(P, b, b.variant);
+  const KernelInfo& k = jk ? variant_info_jk(P, b, b.variant) : variant_info(P, b, b.variant);
   const PairDev pd{P->d_rec, P->d_poff, P->d_pnp, P->d_geo};
   double* scratch = b.generic ? P->d_scratch + (size_t)chain * (size_t)P->scratch_stride : nullptr;
   launch_bin_kernels(P->family, b.l, b.variant, b.generic, k, pd, P->expcutoff, b.d_q,
                      (int64_t)b.q.size(), out, b.out_base, b.block, scratch,
-                     count ? P->d_counters + 2 * i : nullptr, st);
+                     count ? P->d_counters + 2 * i : nullptr, st, jk);
 }
 
 // Spinor families: the Cartesian blocks (all bins done) -> `out`.
@@ -2728,15 +2764,22 @@ static void jk_enqueue(cintx_plan* plan, const double* Dsym, double* dJ, double*
     HIPCHK(hipEventRecord(ev[0], C.stream));
     for (int c = 0; c < kChains - 1; ++c) HIPCHK(hipStreamWaitEvent(C.branch[c], ev[0], 0));
   }
-  // kernels of different classes and the atomic-bound digests overlap
-  // across chains (disjoint ERI output regions; J and K take atomics only);
-  // runtime-class bins use their chain's workspace slice
+  // Every compiled class kernel digests its own blocks (fused J/K: no ERI
+  // reaches memory); the runtime-class kernel writes its bins' blocks and a
+  // digest kernel reads them back.  Bins of different classes overlap across
+  // chains (J and K take atomics only); runtime-class bins use their
+  // chain's workspace slice.
+  const JkFuse fuse{Dsym, dJ, dK, plan->d_pair_sh, plan->d_ao_off, nao};
   size_t n_other = 0, n_gen = 0;
   for (size_t i = 0; i < plan->bins.size(); ++i) {
     const Bin& b = plan->bins[i];
     const int chain = b.generic ? (int)(n_gen++ % std::min(plan->scratch_chains, nchain))
                                 : (int)(n_other++ % nchain);
     hipStream_t st = chain ? C.branch[chain - 1] : C.stream;
+    if (variant_info_jk(plan, b, b.variant).fn) {
+      plan_launch_bin(plan, i, plan->d_out, false, st, chain, &fuse);
+      continue;
+    }
     plan_launch_bin(plan, i, plan->d_out, false, st, chain);
     HIPCHK(launch_jk_digest(plan->d_out, b.out_base, b.block, b.d_q, (int64_t)b.q.size(),
                             plan->d_pair_sh, plan->d_ao_off, plan->d_shl_l, Dsym, dJ, dK, nao,
@@ -2800,7 +2843,7 @@ int cintx_plan_jk_noalloc(cintx_plan* plan, const double* D_dev, double* J_dev, 
                " bytes) exceeds the " + std::to_string(jk_scratch_bytes()) +
                "-byte scratch of the allocation-free J/K");
     HIPCHK(launch_jk_sym_copy(D_dev, C.d_jk_scratch, plan->nao, C.stream));
-    jk_enqueue(plan, C.d_jk_scratch, J_dev, K_dev, nullptr);
+    jk_enqueue(plan, C.d_jk_scratch, J_dev, K_dev, C.jk_ev);
   });
 }
 

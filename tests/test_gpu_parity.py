@@ -503,6 +503,55 @@ def test_jk_build_cfg1(oracle):
     p.close()
 
 
+def _jk_reference(eri, nao, seed):
+    rng = np.random.default_rng(seed)
+    D = rng.normal(size=(nao, nao)) / nao
+    D = D + D.T
+    return D, np.einsum("pqrs,rs->pq", eri, D), np.einsum("pqrs,qs->pr", eri, D)
+
+
+def test_jk_fused_every_launch_variant(oracle):
+    """The fused J/K epilogues (one-lane kernels: blocks staged in LDS per
+    chunk and digested by their lane; wide kernels: the staged spherical
+    block digested by the tuple's lanes) in every compiled launch shape,
+    against an einsum over the oracle's full cfg-1 tensor."""
+    mol = M.config(1)
+    eri = oracle.full_tensor_int2e(mol.atm, mol.bas, mol.env, ext=True)
+    D, Jr, Kr = _jk_reference(eri, eri.shape[0], 12)
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, screened=(1e-300, 1))
+    for v in range(capi.NVARIANTS):
+        p.set_variant(v)
+        J, K = p.jk(D)
+        say(f"variant {v}: J err {np.abs(J - Jr).max():.3e}  K err {np.abs(K - Kr).max():.3e}")
+        assert np.abs(J - Jr).max() <= 1e-11 * max(1.0, np.abs(Jr).max())
+        assert np.abs(K - Kr).max() <= 1e-11 * max(1.0, np.abs(Kr).max())
+    p.close()
+
+
+def test_jk_with_runtime_class_bins(oracle):
+    """J/K over s..g shells on two centres: the f and g classes run the
+    runtime-class kernel (blocks written, then the digest kernel), the rest
+    the fused class kernels; against an einsum over the oracle tensor."""
+    mol = M.Molecule()
+    a0 = mol.add_atom(8, (0.0, 0.0, 0.0))
+    a1 = mol.add_atom(8, (0.3, -0.4, 1.9))
+    rng = np.random.default_rng(21)
+    for atom in (a0, a1):
+        for l in range(5):
+            e = sorted(rng.uniform(0.3, 6.0, 2), reverse=True)
+            mol.add_shell(atom, l, e, [[M.gto_norm(l, x) * c for x, c in zip(e, (0.6, 0.5))]])
+    mol.build()
+    eri = oracle.full_tensor_int2e(mol.atm, mol.bas, mol.env)
+    D, Jr, Kr = _jk_reference(eri, eri.shape[0], 13)
+    p = capi.Plan("int2e_sph", mol.atm, mol.bas, mol.env, screened=(1e-300, 1))
+    J, K = p.jk(D)
+    say(f"s..g: J err {np.abs(J - Jr).max():.3e}  K err {np.abs(K - Kr).max():.3e}  "
+        f"|J| {np.abs(Jr).max():.3e}")
+    assert np.abs(J - Jr).max() <= 1e-11 * max(1.0, np.abs(Jr).max())
+    assert np.abs(K - Kr).max() <= 1e-11 * max(1.0, np.abs(Kr).max())
+    p.close()
+
+
 @pytest.mark.parametrize("variant", range(capi.NVARIANTS))
 def test_every_launch_variant(oracle, variant):
     """Every compiled launch shape (lanes per tuple, primitive tuples per
