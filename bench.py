@@ -68,7 +68,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the integral work shards without exchange; see DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU run would give each rank its own bra pairs and its own output "
+              "region, with one all-reduce of J and K at the end)", file=sys.stderr)
         sys.exit(2)
 
     if args.secondary_only:
