@@ -60,7 +60,39 @@ def class_sets():
             seen.add(c)
             uniq.append(c)
     uniq += [(0, 2, ls) for deriv, prec, ls in uniq if deriv == 0 and prec == 0]
-    return uniq
+    return uniq + [(0, 0, ls) for ls in high_l_classes()]
+
+
+HIGH_L_MAX_NF = 1000
+
+
+def high_l_classes():
+    """int2e_sph classes with an f or g shell (canonical: in the bra) and at
+    most HIGH_L_MAX_NF Cartesian elements, FP64 only, without Cartesian or
+    fused-J/K copies.  Their blocks are small, so the runtime-class kernel
+    (one tuple per 256-lane workgroup, one primitive tuple per step) leaves
+    most of its lanes idle on them; the template kernels batch tuples and
+    primitive tuples instead.  An f or g shell must pair with an s or p
+    shell: the templates transfer from the shell of larger l (libcint's
+    rule), and a transfer between two shells of l >= 2 with one of them f or
+    g loses up to 1e-10 in FP64 (libcint's own route: 144x the 1e-12 bar on
+    (g g|s s), 12x on (g g|p s) of the s..g molecule); those classes and
+    the larger ones keep the runtime-class kernel, whose transfer base is
+    the centre of the larger exponent per primitive pair."""
+    nc = [(l + 1) * (l + 2) // 2 for l in range(5)]
+    pairs = [(a, b) for a in range(5) for b in range(5) if a >= b and (a < 3 or b < 2)]
+    out = []
+    for bra in pairs:
+        for ket in pairs:
+            ls = bra + ket
+            if bra >= ket and max(ls) >= 3 and \
+                    nc[ls[0]] * nc[ls[1]] * nc[ls[2]] * nc[ls[3]] <= HIGH_L_MAX_NF:
+                out.append(ls)
+    return out
+
+
+def is_high_l(deriv, ls):
+    return deriv == 0 and max(ls[0], ls[1]) >= 3
 
 
 def _p2(v):
@@ -140,7 +172,9 @@ def variants(deriv, prec, ls):
         cands.insert(0, (1, 1))
     elif ne <= 81:
         cands.insert(3, (1, 1))
-    else:
+    elif not is_high_l(deriv, ls):
+        # (f/g classes: no chunked one-lane variant; with the promoted-alloca
+        # limits below their compile time runs to tens of minutes)
         nch = register_chunks(deriv, prec, ls)
         if nch:
             cands.insert(3, (1, nch))
@@ -170,8 +204,8 @@ def generate(nparts=24, nchunkparts=12):
         for v, (tt, pp) in enumerate(variants(deriv, prec, ls)):
             (chunked if (tt == 1 and (pp & 255) > 1) else plain).append((deriv, prec, ls, v, tt, pp))
             # the same launch shape with the fused J/K epilogue (PP bit 12):
-            # int2e_sph classes in FP64 and FP32
-            if deriv == 0 and prec in (0, 1):
+            # int2e_sph classes in FP64 and FP32 (not the f/g classes)
+            if deriv == 0 and prec in (0, 1) and not is_high_l(deriv, ls):
                 (chunked if (tt == 1 and (pp & 255) > 1) else plain).append(
                     (deriv, prec, ls, v, tt, pp | 4096))
     parts = [plain[i::nparts] for i in range(nparts)]

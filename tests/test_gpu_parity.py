@@ -471,6 +471,32 @@ def test_high_l_ip1_cfg5(oracle):
     check_plan(oracle, "int2e_ip1_sph", tuples, mol)
 
 
+@pytest.mark.parametrize("variant", range(capi.NVARIANTS))
+def test_high_l_template_classes_every_variant(oracle, variant):
+    """The int2e_sph classes with an f or g shell that run as template
+    kernels (at most 1000 Cartesian elements; cintx_amd.build.high_l_classes):
+    on the s..g molecule, eight tuples of every such class, shells taken from
+    two neighbouring atoms (so the blocks are not screened to zero) and put
+    in a random one of the eight orders of the quartet's symmetry, in every
+    compiled launch shape, against the judge."""
+    from cintx_amd import build as B
+    mol = M.config(5)
+    ls, at = mol.bas[:, 1], mol.bas[:, 0]
+    natm = int(at.max()) + 1
+    rng = np.random.default_rng(40 + variant)
+    tuples = []
+    for cls in B.high_l_classes():
+        for _ in range(8):
+            a = int(rng.integers(0, natm))
+            near = (at == a) | (at == (a + 1) % natm)
+            sh = [int(rng.choice(np.where(near & (ls == l))[0])) for l in cls]
+            i, j, k, l = sh
+            orders = [(i, j, k, l), (j, i, k, l), (i, j, l, k), (j, i, l, k),
+                      (k, l, i, j), (l, k, i, j), (k, l, j, i), (l, k, j, i)]
+            tuples.append(orders[int(rng.integers(0, 8))])
+    check_plan(oracle, "int2e_sph", tuples, mol, variant)
+
+
 def test_general_contraction_all_classes(oracle):
     mol = M.general_contraction_h2()
     rng = np.random.default_rng(10)
