@@ -132,20 +132,24 @@ __device__ void generic_ket(double* __restrict__ g, const GenShape& S, int axis,
 // registers (two workgroups per CU; one for nabla_i, whose three components
 // per element need the larger register budget); a block with more rows than
 // one chunk (256 RPL rows) is done chunk by chunk, each chunk a full pass
-// over the primitive tuples with the roots and chains recomputed.
+// over the primitive tuples with the roots and chains recomputed.  WIDE:
+// such a block runs with about 48 accumulators per lane at one workgroup
+// per CU (the register budget of one wave per SIMD), so a (g f|g p) block
+// takes 3 chunks instead of 9.
 constexpr size_t kGenLdsBytes = 150 * 1024;  // dynamic LDS for g (gfx950: 160 KB per CU)
 
-// rows per lane: about 16 accumulators (nabla_i: 45, with twice the registers)
-__host__ __device__ constexpr int gen_rows_per_lane(int li, bool deriv) {
-  return deriv ? cmax(1, 15 / ncart(li)) : cmax(1, 16 / ncart(li));
+// rows per lane: about 16 accumulators (nabla_i, and WIDE: 45-48, with twice
+// the registers)
+__host__ __device__ constexpr int gen_rows_per_lane(int li, bool deriv, bool wide = false) {
+  return deriv ? cmax(1, 15 / ncart(li)) : cmax(1, (wide ? 48 : 16) / ncart(li));
 }
 
-template <bool GLDS, bool DERIV, int LI>
-__global__ void __launch_bounds__(256, (DERIV ? 1 : 2)) eri_generic_kernel(EriArgs a, GenShape S,
-                                                          int64_t per_block) {
+template <bool GLDS, bool DERIV, int LI, bool WIDE = false>
+__global__ void __launch_bounds__(256, (DERIV || WIDE ? 1 : 2))
+    eri_generic_kernel(EriArgs a, GenShape S, int64_t per_block) {
   constexpr int NC = DERIV ? 3 : 1;
-  constexpr int NCI = ncart(LI);                      // elements per row
-  constexpr int RPL = gen_rows_per_lane(LI, DERIV);   // rows per lane
+  constexpr int NCI = ncart(LI);                           // elements per row
+  constexpr int RPL = gen_rows_per_lane(LI, DERIV, WIDE);  // rows per lane
   constexpr int NLD = LI + 1 + (DERIV ? 1 : 0);       // i powers loaded per axis
   constexpr int ACC = RPL * NCI;                      // elements per lane
   constexpr int CHUNK = 256 * RPL;                    // rows per chunk
@@ -398,12 +402,18 @@ hipError_t launch_generic(const EriArgs& a, const int* l, int deriv, int grid, h
   const size_t gbytes = (size_t)3 * S.g4 * S.nr * sizeof(double);
   const bool glds = gbytes <= kGenLdsBytes;
   const size_t dyn = glds ? gbytes : 0;
+  // more rows than one chunk of the narrow rows: the wide kernel
+  const int nrow = ncart(l[1]) * ncart(l[2]) * ncart(l[3]);
+  const bool wide = !deriv && nrow > 256 * gen_rows_per_lane(l[0], false);
   switch (l[0]) {
 #define CX_GEN_LAUNCH(LI)                                                                     \
   case LI:                                                                                    \
     if (glds && deriv)                                                                        \
       hipLaunchKernelGGL((eri_generic_kernel<true, true, LI>), dim3(grid), dim3(256), dyn, s, a, \
                          S, per);                                                             \
+    else if (glds && wide)                                                                    \
+      hipLaunchKernelGGL((eri_generic_kernel<true, false, LI, true>), dim3(grid), dim3(256),   \
+                         dyn, s, a, S, per);                                                  \
     else if (glds)                                                                            \
       hipLaunchKernelGGL((eri_generic_kernel<true, false, LI>), dim3(grid), dim3(256), dyn, s,  \
                          a, S, per);                                                          \
