@@ -73,7 +73,8 @@ def main():
         sys.exit(2)
 
     if args.secondary_only:
-        print(json.dumps(secondary_configs()), flush=True)
+        print(json.dumps({"secondary": secondary_configs(), "raw_path": raw_path_rate()}),
+              flush=True)
         return
 
     from cintx_amd import capi, molecule
@@ -185,6 +186,12 @@ def main():
             log(f"{k}: {v['per_s']:.3e} {v['unit']}, {v['ms']:.1f} ms, "
                 f"{v['tflops']:.2f} TFLOP/s, max err {v['max_abs_err_vs_judge']:.2e}")
 
+    raw = None
+    if not args.no_secondary:
+        raw = raw_path_rate()
+        log(f"raw int2e_sph calls: {raw['calls_per_s_with_optimizer']:.0f}/s with an optimizer, "
+            f"{raw['calls_per_s_without_optimizer']:.0f}/s without")
+
     cpu = None
     if not args.no_cpu:
         t64, _ = p64.tuples_cached()
@@ -253,6 +260,7 @@ def main():
         "cpu_baseline": cpu,
         "launch_shapes": tuned,
         "secondary": secondary,
+        "raw_path": raw,
         "learned_roots": learned,
     }
     print(json.dumps(res), flush=True)
@@ -270,7 +278,7 @@ def secondary_configs():
     from oracle import oracle as O
     out = {}
 
-    def run(key, plan, family, mol, unit, reps=3):
+    def run(key, plan, family, mol, unit, reps=3, sample=300):
         log(f"secondary {key}: {plan.ntuples} tuples")
         plan.autotune(reps=1)
         plan.execute(graph=True)
@@ -283,11 +291,20 @@ def secondary_configs():
         _, _, flops = plan.work()
         t, _ = plan.tuples_cached()
         rng = np.random.default_rng(5)
-        pick = np.sort(rng.choice(len(t), size=min(300, len(t)), replace=False))
+        pick = np.sort(rng.choice(len(t), size=min(sample, len(t)), replace=False))
         got, _, _ = plan.output_tuples(pick)
-        arity = 3 if family.startswith("int3c2e") else 4
-        ref, _, _ = O.eval_list(family, [tuple(int(v) for v in t[q][:arity]) for q in pick],
-                                mol.atm, mol.bas, mol.env, ext=True)
+        arity = {"int2c2e": 2, "int3c2e": 3}.get(family.split("_")[0], 4)
+        sh = [tuple(int(v) for v in t[q][:arity]) for q in pick]
+        if family.endswith("_spinor"):
+            # the spinor oracle (extended precision on the judge's Cartesian
+            # blocks), libcint's F-order block per tuple
+            from oracle import spinor as S
+            fn = S.int2e_spinor if arity == 4 else S.int3c2e_spinor
+            got = got.view(np.complex128)
+            ref = np.concatenate([np.asarray(fn(q, mol.atm, mol.bas, mol.env)[0]).transpose(
+                tuple(range(arity))[::-1]).ravel() for q in sh])
+        else:
+            ref, _, _ = O.eval_list(family, sh, mol.atm, mol.bas, mol.env, ext=True)
         m = float(np.mean(ms))
         out[key] = {"family": family, "tuples": plan.ntuples, "ms": m,
                     "per_s": plan.ntuples / (m / 1e3), "unit": unit,
@@ -322,7 +339,66 @@ def secondary_configs():
     run("int2e_sph_sg_basis", p5, "int2e_sph", m5, "quartets/s")
     run("int2e_ip1_sph_sg_basis", capi.Plan("int2e_ip1_sph", m5.atm, m5.bas, m5.env, t5),
         "int2e_ip1_sph", m5, "quartets/s")
+    # the other forms, each on a share of the same inputs: the unique
+    # quartets of the 64-atom lattice (Schwarz 1e-12, bra stride 32) and the
+    # aux-lattice triples
+    m2 = molecule.config(2)
+    p2 = capi.Plan("int2e_sph", m2.atm, m2.bas, m2.env, screened=(1e-12, 32))
+    t2, _ = p2.tuples_cached()
+    t2 = np.array(t2, dtype=np.int32)
+    p2.close()
+    run("int2e_cart_lattice64", capi.Plan("int2e_cart", m2.atm, m2.bas, m2.env, t2),
+        "int2e_cart", m2, "quartets/s")
+    run("int2e_ip2_sph_lattice64", capi.Plan("int2e_ip2_sph", m2.atm, m2.bas, m2.env, t2[::8]),
+        "int2e_ip2_sph", m2, "quartets/s")
+    run("int2e_spinor_lattice64",
+        capi.Plan("int2e_spinor", m2.atm, m2.bas, m2.env, t2[::64]),
+        "int2e_spinor", m2, "quartets/s", sample=40)
+    run("int3c2e_cart_aux_lattice", capi.Plan("int3c2e_cart", m4.atm, m4.bas, m4.env, trip[::8]),
+        "int3c2e_cart", m4, "triples/s")
+    run("int3c2e_ip2_sph_aux_lattice",
+        capi.Plan("int3c2e_ip2_sph", m4.atm, m4.bas, m4.env, trip[::32]),
+        "int3c2e_ip2_sph", m4, "triples/s")
+    run("int3c2e_spinor_aux_lattice",
+        capi.Plan("int3c2e_spinor", m4.atm, m4.bas, m4.env, trip[::256]),
+        "int3c2e_spinor", m4, "triples/s", sample=40)
+    # the density-fitting metric: every (P|Q), P >= Q, of the aux basis
+    pp, qq = np.tril_indices(len(aux))
+    pq = np.zeros((len(pp), 4), dtype=np.int32)
+    pq[:, 0] = aux[pp]
+    pq[:, 1] = aux[qq]
+    run("int2c2e_sph_aux_metric", capi.Plan("int2c2e_sph", m4.atm, m4.bas, m4.env, pq),
+        "int2c2e_sph", m4, "pairs/s")
     return out
+
+
+def raw_path_rate(ncalls=4000):
+    """The libcint raw signature from a host caller: one int2e_sph call per
+    quartet of the 64-atom lattice (host buffers: each call uploads its
+    tuple, launches, waits and copies its block back), with a device
+    CINTOpt and without (the call then builds its pair data itself); calls
+    from Python through ctypes, one thread."""
+    from cintx_amd import capi, molecule
+    m2 = molecule.config(2)
+    rng = np.random.default_rng(11)
+    sh = rng.integers(0, m2.nbas, (ncalls, 4)).astype(np.int32)
+    opt = capi.Optimizer("int2e_sph", m2.atm, m2.bas, m2.env)
+    for q in sh[:50]:
+        capi.raw_call("int2e_sph", q, m2.atm, m2.bas, m2.env, opt=opt)
+    t0 = time.perf_counter()
+    for q in sh:
+        capi.raw_call("int2e_sph", q, m2.atm, m2.bas, m2.env, opt=opt)
+    with_opt = ncalls / (time.perf_counter() - t0)
+    n0 = ncalls // 8
+    t0 = time.perf_counter()
+    for q in sh[:n0]:
+        capi.raw_call("int2e_sph", q, m2.atm, m2.bas, m2.env)
+    without = n0 / (time.perf_counter() - t0)
+    opt.close()
+    return {"family": "int2e_sph", "calls": ncalls, "calls_per_s_with_optimizer": with_opt,
+            "calls_per_s_without_optimizer": without,
+            "note": "PCIe-inclusive host-buffer boundary, one quartet per call, Python ctypes "
+                    "caller, one thread; random shell quartets of the 64-atom lattice"}
 
 
 def profiled_traffic(kname, tuples):
