@@ -72,23 +72,28 @@ def high_l_classes():
     fused-J/K copies.  Their blocks are small, so the runtime-class kernel
     (one tuple per 256-lane workgroup, one primitive tuple per step) leaves
     most of its lanes idle on them; the template kernels batch tuples and
-    primitive tuples instead.  An f or g shell must pair with an s or p
-    shell: the templates transfer from the shell of larger l (libcint's
-    rule), and a transfer between two shells of l >= 2 with one of them f or
-    g loses up to 1e-10 in FP64 (libcint's own route: 144x the 1e-12 bar on
-    (g g|s s), 12x on (g g|p s) of the s..g molecule); those classes and
-    the larger ones keep the runtime-class kernel, whose transfer base is
-    the centre of the larger exponent per primitive pair."""
+    primitive tuples instead.  A bra whose f or g shell pairs with a shell
+    of l >= 2 runs the wide path with the transfer base chosen per
+    primitive tuple (needs_pb); libcint's fixed base loses up to 1e-10 in
+    FP64 there (144x the 1e-12 bar on (g g|s s) tuples of the s..g
+    molecule, 12x on (g g|p s)).  Kets stay s..d or an f/g shell with an s
+    or p partner (no per-primitive ket base); larger classes keep the
+    runtime-class kernel."""
     nc = [(l + 1) * (l + 2) // 2 for l in range(5)]
-    pairs = [(a, b) for a in range(5) for b in range(5) if a >= b and (a < 3 or b < 2)]
+    pairs = [(a, b) for a in range(5) for b in range(5) if a >= b]
     out = []
     for bra in pairs:
         for ket in pairs:
             ls = bra + ket
-            if bra >= ket and max(ls) >= 3 and \
+            if bra >= ket and max(ls) >= 3 and (ket[0] < 3 or ket[1] < 2) and \
                     nc[ls[0]] * nc[ls[1]] * nc[ls[2]] * nc[ls[3]] <= HIGH_L_MAX_NF:
                 out.append(ls)
     return out
+
+
+def needs_pb(ls):
+    """f or g bra shell paired with l >= 2: per-primitive transfer base"""
+    return max(ls[0], ls[1]) >= 3 and min(ls[0], ls[1]) >= 2
 
 
 def is_high_l(deriv, ls):
@@ -178,6 +183,9 @@ def variants(deriv, prec, ls):
         nch = register_chunks(deriv, prec, ls)
         if nch:
             cands.insert(3, (1, nch))
+    if is_high_l(deriv, ls) and needs_pb(ls):
+        # wide path only, with the per-primitive bra base (PP bit 13)
+        cands = [(tt, pp | 8192) for tt, pp in cands if tt > 1]
     out = []
     for c in cands:
         if c not in out:

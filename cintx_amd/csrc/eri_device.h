@@ -124,7 +124,9 @@ struct PrimTuple {
 // slots, and the ket transfer then runs block by block in place.  This keeps
 // about 4*(NMAX+1) + (MMAX+1) values live instead of the whole 2D and 3D
 // intermediates.
-template <int LIc, int LJ, int LK, int LL, typename R, int RS = -1>
+// FLIP: the bra transfer starts from the other centre than libcint's rule
+// (prb and rij_ax are then the caller's for that centre).
+template <int LIc, int LJ, int LK, int LL, typename R, int RS = -1, bool FLIP = false>
 __device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, double s,
                                             double q, double wz, double inv_sum, double hij, double hkl,
                                             double aij, double akl, double dx, double prb,
@@ -132,7 +134,7 @@ __device__ __forceinline__ void build_chain(R* __restrict__ g, int axis, int r, 
                                             double rkl_ax) {
   constexpr int NR = (LIc + LJ + LK + LL) / 2 + 1;
   constexpr int NMAX = LIc + LJ, MMAX = LK + LL;
-  constexpr bool IBASE = LIc > LJ, KBASE = LK > LL;
+  constexpr bool IBASE = (LIc > LJ) != FLIP, KBASE = LK > LL;
   constexpr int DI = LIc + 1, DJ = LJ + 1, DK = LK + 1, DL = LL + 1;
   constexpr int A = IBASE ? LIc : LJ, B = IBASE ? LJ : LIc;
   constexpr int C = KBASE ? LK : LL, D = KBASE ? LL : LK;
@@ -395,6 +397,12 @@ struct ClassShape {
   // J and K instead of written
   static constexpr bool JK = (PP >> 12) & 1;
   static_assert(!JK || (!DERIV && !CART), "fused J/K serves int2e_sph only");
+  // PP bit 13 (wide path only): the bra transfer starts, per primitive
+  // tuple, from the centre of the larger exponent, as the runtime-class
+  // kernel does (for f and g shells paired with a shell of l >= 2, where
+  // libcint's fixed base loses up to 1e-10 in FP64)
+  static constexpr bool PB = (PP >> 13) & 1;
+  static_assert(!PB || (TT > 1 && !JK), "per-primitive transfer base: wide path, no J/K");
   static constexpr int nout(int l) { return CART ? ncart(l) : nsph(l); }
   static constexpr int RPC = (NROW + NCH - 1) / NCH;  // rows per chunk
   static constexpr int G = T >= NROW ? T / NROW : 1;               // groups per row
@@ -934,15 +942,20 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
                         cceij <= a.expcutoff - ccekl && cceij < a.expcutoff;
         double* uwp = suw + pp * 3 * NR;
         double* sc = sps + pp * NSC;
+        bool flip = false;  // PB: the bra base of this primitive tuple is not libcint's
         if (ok) {
           const double aij = pi[0], akl = pk[0];
+          // bra transfer base: libcint's rule, or (PB) the centre of the
+          // larger exponent (a_i = pi[7], a_j = aij - a_i)
+          const bool ib = S::PB ? (pi[7] >= aij - pi[7]) : IBASE;
+          flip = ib != IBASE;
           double dx[3], pb[3], qb[3], rr = 0;
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
             const double pr = pi[2] * rji[d], qr = pk[2] * rlk[d];  // P - R_i, Q - R_k
             dx[d] = (rik[d] + pr) - qr;
             rr += dx[d] * dx[d];
-            pb[d] = IBASE ? pr : -pi[1] * rji[d];
+            pb[d] = ib ? pr : -pi[1] * rji[d];
             qb[d] = KBASE ? qr : -pk[1] * rlk[d];
           }
           const double inv_sum = 1.0 / (aij + akl);
@@ -971,7 +984,8 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
             ++nsurv;
           }
         }
-        if (r == 0) sc[15] = ok ? 1.0 : 0.0;
+        // 0: screened out, 1: survives, 2: survives with the other bra base
+        if (r == 0) sc[15] = ok ? (flip ? 2.0 : 1.0) : 0.0;
         any = any || ok;
       }
       if (!tuple_any<T>(any)) continue;
@@ -993,10 +1007,15 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
         R loc[G4];
         const double rij_a = axis == 0 ? rij[0] : (axis == 1 ? rij[1] : rij[2]);
         const double rkl_a = axis == 0 ? rkl[0] : (axis == 1 ? rkl[1] : rkl[2]);
-        build_chain<LIc, LJ, LK, LL, R, 1>(loc, 0, 0, uwp[r], uwp[2 * NR + r], uwp[NR + r], sc[0],
-                                           sc[1], sc[2], sc[3], sc[4], sc[5 + axis],
-                                           sc[8 + axis], sc[11 + axis], axis == 2 ? 0.0 : 1.0,
-                                           rij_a, rkl_a);
+        if (S::PB && sc[15] == 2.0)  // the other bra base: the transfer step changes sign
+          build_chain<LIc, LJ, LK, LL, R, 1, S::PB>(
+              loc, 0, 0, uwp[r], uwp[2 * NR + r], uwp[NR + r], sc[0], sc[1], sc[2], sc[3], sc[4],
+              sc[5 + axis], sc[8 + axis], sc[11 + axis], axis == 2 ? 0.0 : 1.0, -rij_a, rkl_a);
+        else
+          build_chain<LIc, LJ, LK, LL, R, 1>(loc, 0, 0, uwp[r], uwp[2 * NR + r], uwp[NR + r],
+                                             sc[0], sc[1], sc[2], sc[3], sc[4], sc[5 + axis],
+                                             sc[8 + axis], sc[11 + axis], axis == 2 ? 0.0 : 1.0,
+                                             rij_a, rkl_a);
         R* gp = sg + pp * GS + (size_t)axis * G4 * NR + r;
 #pragma unroll
         for (int k = 0; k < G4; ++k) gp[k * NR] = loc[k];
