@@ -63,7 +63,7 @@ def class_sets():
     return uniq + [(0, 0, ls) for ls in high_l_classes()]
 
 
-HIGH_L_MAX_NF = 2000
+HIGH_L_MAX_NF = 3500
 
 
 def high_l_classes():
