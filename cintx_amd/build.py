@@ -60,7 +60,8 @@ def class_sets():
             seen.add(c)
             uniq.append(c)
     uniq += [(0, 2, ls) for deriv, prec, ls in uniq if deriv == 0 and prec == 0]
-    return uniq + [(0, 0, ls) for ls in high_l_classes()]
+    return uniq + [(0, 0, ls) for ls in high_l_classes()] + \
+        [(1, 0, ls) for ls in high_l_nabla_classes()]
 
 
 HIGH_L_MAX_NF = 3500
@@ -91,13 +92,24 @@ def high_l_classes():
     return out
 
 
+def high_l_nabla_classes():
+    """nabla_i classes with an f or g shell i and an s shell j, kets s..d:
+    int3c2e_ip2_sph runs as nabla_i on (k | i j) with the aux shell k in the
+    i slot (and an empty j), and int2e_ip1_sph has them for (f s|..) and
+    (g s|..).  With j = s the bra needs no transfer; libcint's FP64 route
+    stays within a sixtieth (int3c2e_ip2) to a half (int2e_ip1 (f s|p s))
+    of the 1e-12 bar on tuples of neighbouring atoms."""
+    pairs = [(a, b) for a in range(3) for b in range(3) if a >= b]
+    return [(li, 0) + ket for li in (3, 4) for ket in pairs]
+
+
 def needs_pb(ls):
     """f or g bra shell paired with l >= 2: per-primitive transfer base"""
     return max(ls[0], ls[1]) >= 3 and min(ls[0], ls[1]) >= 2
 
 
 def is_high_l(deriv, ls):
-    return deriv == 0 and max(ls[0], ls[1]) >= 3
+    return max(ls[0], ls[1]) >= 3
 
 
 def _p2(v):

@@ -497,6 +497,45 @@ def test_high_l_template_classes_every_variant(oracle, variant):
     check_plan(oracle, "int2e_sph", tuples, mol, variant)
 
 
+@pytest.mark.parametrize("variant", range(capi.NVARIANTS))
+def test_high_l_nabla_templates_every_variant(oracle, variant):
+    """The nabla_i template classes with an f or g shell i and an s shell j
+    (cintx_amd.build.high_l_nabla_classes): int2e_ip1_sph (f s|..) and
+    (g s|..) tuples of neighbouring atoms on the s..g molecule, and
+    int3c2e_ip2_sph triples whose aux shell is f or g (nabla on it), in
+    every compiled launch shape, against the judge."""
+    from cintx_amd import build as B
+    rng = np.random.default_rng(60 + variant)
+    mol = M.config(5)
+    ls, at = mol.bas[:, 1], mol.bas[:, 0]
+    natm = int(at.max()) + 1
+    tuples = []
+    for cls in B.high_l_nabla_classes():
+        for _ in range(10):
+            a = int(rng.integers(0, natm))
+            near = (at == a) | (at == (a + 1) % natm)
+            i, j, k, l = [int(rng.choice(np.where(near & (ls == v))[0])) for v in cls]
+            tuples.append((i, j, k, l) if rng.integers(0, 2) else (i, j, l, k))
+    check_plan(oracle, "int2e_ip1_sph", tuples, mol, variant)
+    aux = M.config(4)
+    ls, at = aux.bas[:, 1], aux.bas[:, 0]
+    natm = int(at.max()) + 1
+    orb = np.arange(aux.nbas) < aux.aux_start
+    triples = []
+    for lk in (3, 4):
+        for li in range(3):
+            for lj in range(3):
+                for _ in range(6):
+                    a = int(rng.integers(0, natm))
+                    near = (at == a) | (at == (a + 1) % natm)
+                    pick = [np.where(near & orb & (ls == li))[0],
+                            np.where(near & orb & (ls == lj))[0],
+                            np.where(near & ~orb & (ls == lk))[0]]
+                    if all(len(c) for c in pick):
+                        triples.append(tuple(int(rng.choice(c)) for c in pick))
+    check_plan(oracle, "int3c2e_ip2_sph", triples, aux, variant)
+
+
 def test_general_contraction_all_classes(oracle):
     mol = M.general_contraction_h2()
     rng = np.random.default_rng(10)
