@@ -65,11 +65,15 @@ def class_sets():
 
 
 HIGH_L_MAX_NF = 3500
+# the wide path builds one chain (G4 values) in registers; up to 135 (with
+# the i-row shapes below) every f/g variant compiles without spills
+HIGH_L_MAX_G4 = 135
 
 
 def high_l_classes():
-    """int2e_sph classes with an f or g shell (canonical: in the bra) and at
-    most HIGH_L_MAX_NF Cartesian elements, FP64 only, without Cartesian or
+    """int2e_sph classes with an f or g shell (canonical: in the bra), at
+    most HIGH_L_MAX_NF Cartesian elements and HIGH_L_MAX_G4 values per
+    recurrence chain, FP64 only, without Cartesian or
     fused-J/K copies.  Their blocks are small, so the runtime-class kernel
     (one tuple per 256-lane workgroup, one primitive tuple per step) leaves
     most of its lanes idle on them; the template kernels batch tuples and
@@ -86,8 +90,10 @@ def high_l_classes():
     for bra in pairs:
         for ket in pairs:
             ls = bra + ket
+            g4 = (ls[0] + 1) * (ls[1] + 1) * (ls[2] + 1) * (ls[3] + 1)
             if bra >= ket and max(ls) >= 3 and (ket[0] < 3 or ket[1] < 2) and \
-                    nc[ls[0]] * nc[ls[1]] * nc[ls[2]] * nc[ls[3]] <= HIGH_L_MAX_NF:
+                    nc[ls[0]] * nc[ls[1]] * nc[ls[2]] * nc[ls[3]] <= HIGH_L_MAX_NF and \
+                    g4 <= HIGH_L_MAX_G4:
                 out.append(ls)
     return out
 
@@ -195,6 +201,16 @@ def variants(deriv, prec, ls):
         nch = register_chunks(deriv, prec, ls)
         if nch:
             cands.insert(3, (1, nch))
+    if is_high_l(deriv, ls) and not deriv and nb > 36:
+        # f/g bras with more (i, j) components than (d d|: rows of the i
+        # components alone (PP bit 8), so a lane's accumulators stay in
+        # registers (rows of 60-225 elements spilled 1.7-2.9 KB per lane and
+        # ran up to 50x slower)
+        nrow_i = nf // nc[0]
+        ti = min(256, max(4, _p2(max(nrow_i, 3 * nr))))
+        t2 = min(256, 2 * ti)
+        cands = [(ti, max(1, min(8, ti // (3 * nr), plds)) | 256), (ti, 1 | 256),
+                 (t2, max(1, min(8, t2 // (3 * nr), plds)) | 256), (max(4, ti // 2), 1 | 256)]
     if is_high_l(deriv, ls) and needs_pb(ls):
         # wide path only, with the per-primitive bra base (PP bit 13)
         cands = [(tt, pp | 8192) for tt, pp in cands if tt > 1]

@@ -474,7 +474,7 @@ def test_high_l_ip1_cfg5(oracle):
 @pytest.mark.parametrize("variant", range(capi.NVARIANTS))
 def test_high_l_template_classes_every_variant(oracle, variant):
     """The int2e_sph classes with an f or g shell that run as template
-    kernels (at most 3500 Cartesian elements; cintx_amd.build.high_l_classes):
+    kernels (at most 3500 Cartesian elements and 135 values per chain; cintx_amd.build.high_l_classes):
     on the s..g molecule, eight tuples of every such class, shells taken from
     two neighbouring atoms (so the blocks are not screened to zero) and put
     in a random one of the eight orders of the quartet's symmetry, in every
