@@ -59,13 +59,21 @@ def check_plan(O, name, tuples, mol, variant=None):
 
 
 def test_device_rys_rule_matches_oracle(oracle):
+    """The device rule against the oracle's, and against the host evaluation
+    of the same table (which test_rys_table_moment_equations pins to the
+    moment equations with mpmath alone, independently of the oracle)."""
     L = capi.lib()
     L.cintx_rys_eval_device.argtypes = [ctypes.c_int, capi.DP, ctypes.c_int64, capi.DP, capi.DP]
+    L.cintx_rys_eval_host.argtypes = [ctypes.c_int, capi.DP, ctypes.c_int64, capi.DP, capi.DP]
     for n in range(1, 15):
         xs = np.concatenate([np.linspace(0.0, 40.0 + 5 * n, 301), [1e-12, 3e-7, 1e4]])
         u = np.zeros(len(xs) * n)
         w = np.zeros(len(xs) * n)
         assert L.cintx_rys_eval_device(n, capi._dp(xs), len(xs), capi._dp(u), capi._dp(w)) == 0
+        uh = np.zeros(len(xs) * n)
+        wh = np.zeros(len(xs) * n)
+        assert L.cintx_rys_eval_host(n, capi._dp(xs), len(xs), capi._dp(uh), capi._dp(wh)) == 0
+        assert np.max(np.abs(u - uh) / uh) < 4e-14 and np.max(np.abs(w - wh) / wh) < 4e-14, n
         u, w = u.reshape(-1, n), w.reshape(-1, n)
         for k, x in enumerate(xs):
             uo, wo = oracle.rys_roots(n, x)
