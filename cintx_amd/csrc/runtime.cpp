@@ -789,9 +789,9 @@ static const KernelInfo* bin_kernels(const cintx_plan* P, const Bin& b);
 // Lay out and upload pairs, bins and offsets in one device blob.
 // Order every bin's tuples by (bra, ket) primitive-pair counts (counting
 // sort), so that the tuples sharing a workgroup of the wide kernels have
-// equal loop trip counts; for a class with a one-lane kernel, then (stable)
-// by surviving primitive tuples, so that the lanes of a wavefront, which
-// each walk their own survivors, run equal numbers of iterations.
+// equal loop trip counts; for a class with a one-lane kernel instead by
+// surviving primitive tuples (stable), so that the lanes of a wavefront,
+// which each walk their own survivors, run equal numbers of iterations.
 static void plan_sort_bins(cintx_plan* P) {
   const PairSet& S = P->pairs;
   const FamilyInfo& f = kFam[P->family];
@@ -823,17 +823,21 @@ static void plan_sort_bins(cintx_plan* P) {
       b.q.swap(by_np);
       continue;
     }
+    // one-lane classes: by survivors alone (stable), so that within equal
+    // survivor counts the enumeration's bra-major order stays and
+    // neighbouring lanes often share their bra pair (the fused J/K sums
+    // such lanes before its atomics); a lane walks its survivors only, so
+    // the primitive counts do not set its trip count
     const int lij = b.l[0] + b.l[1] + f.deriv, lkl = b.l[2] + b.l[3];
     constexpr int kSurvCap = 1 << 16;
     for (int64_t i = 0; i < n; ++i)
       key[(size_t)i] = (int)std::min<int64_t>(
-          surviving_prims(S, by_np[i].x, by_np[i].y, lij, lkl, f.three, P->expcutoff),
+          surviving_prims(S, b.q[i].x, b.q[i].y, lij, lkl, f.three, P->expcutoff),
           kSurvCap - 1);
     counting_sort(kSurvCap, key, pos2);
     std::vector<int2> sorted((size_t)n);
-    remap[bi].resize((size_t)n);
-    for (int64_t i = 0; i < n; ++i) sorted[(size_t)pos2[(size_t)i]] = by_np[(size_t)i];
-    for (int64_t i = 0; i < n; ++i) remap[bi][(size_t)i] = pos2[(size_t)pos1[(size_t)i]];
+    for (int64_t i = 0; i < n; ++i) sorted[(size_t)pos2[(size_t)i]] = b.q[(size_t)i];
+    remap[bi] = pos2;
     b.q.swap(sorted);
   }
   for (VQ& v : P->vqs) v.idx = remap[v.bin][(size_t)v.idx];
