@@ -813,8 +813,10 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
           const int KL0 = ONE_TO_ONE ? ch * RPC : 0;
           const int KL1 = ONE_TO_ONE ? cmin(ch * RPC + RPC, NKL) : NKL;
           constexpr int RPP = S::jk_rows_per_pass();
-          // one digest site: one chunk, one pass (the loops are unrolled)
+          // one digest site: one chunk, one pass (the loops are unrolled),
+          // over the rows [0, ONE_NR)
           constexpr bool ONE_SITE = NCH == 1 && (ONE_TO_ONE ? RPC : NKL) <= RPP;
+          constexpr int ONE_NR = ONE_TO_ONE ? (RPC < NKL ? RPC : NKL) : NKL;
           extern __shared__ double jks[];
           const int sa = a.pair_sh[pq.x].x, sbb = a.pair_sh[pq.x].y;
           const int sc = a.pair_sh[pq.y].x, sd = a.pair_sh[pq.y].y;
@@ -835,10 +837,10 @@ __global__ void __launch_bounds__((ClassShape<LI, LJ, LK, LL, DERIV, TT, PP>::T 
               }
             }
             __syncthreads();
-            if constexpr (ONE_SITE)
-              jk_digest_rows<NSI, NSJ, NSK, NSL>(lane, 256, r0, nr, active, sh[0], sh[1], sh[2],
-                                                 sh[3], off[0], off[1], off[2], off[3], a.jk_D,
-                                                 a.jk_J, a.jk_K, a.nao);
+            if constexpr (ONE_SITE)  // rows [0, ONE_NR), a compile-time range
+              jk_digest_rows_fixed<NSI, NSJ, NSK, NSL, ONE_NR>(
+                  lane, 256, active, sh[0], sh[1], sh[2], sh[3], off[0], off[1], off[2], off[3],
+                  a.jk_D, a.jk_J, a.jk_K, a.nao);
             else
               jk_digest_rows_call<NSI, NSJ, NSK, NSL>(lane, 256, r0, nr, active, sh[0], sh[1],
                                                       sh[2], sh[3], off[0], off[1], off[2], off[3],

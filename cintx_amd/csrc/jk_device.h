@@ -220,6 +220,104 @@ __device__ __forceinline__ void jk_digest_rows(int base, int ls, int r0, int nr,
       jk_put(live, J, (oI + mi) * n + oJ + mj, 2.0 * w * aij[mi + NI * mj], rIJ);
 }
 
+// jk_digest_rows for the compile-time rows [0, NR) (a kernel with one digest
+// site, inlined): the K contributions of all rows are summed in registers
+// per output element first, so each K element gets one segmented sum and
+// one atomic per tuple instead of one per row that reaches it.
+template <int NI, int NJ, int NK, int NL, int NR>
+__device__ __forceinline__ void jk_digest_rows_fixed(int base, int ls, bool live, int s0, int s1,
+                                                     int s2, int s3, int o0, int o1, int o2,
+                                                     int o3, const double* __restrict__ D,
+                                                     double* __restrict__ J,
+                                                     double* __restrict__ K, int nao) {
+  extern __shared__ double jk_stage[];
+  constexpr int NB = NI * NJ;
+  const double w = jk_weight(s0, s1, s2, s3);
+  const size_t n = (size_t)nao;
+  const size_t oI = o0, oJ = o1, oK = o2, oL = o3;
+  const unsigned long long dead = ~0ull - __lane_id();
+  const WaveRun rIJ = wave_run(live ? oI * n + oJ : dead);
+  const WaveRun rKL = wave_run(live ? oK * n + oL : dead);
+  const WaveRun rIK = wave_run(live ? oI * n + oK : dead);
+  const WaveRun rIL = wave_run(live ? oI * n + oL : dead);
+  const WaveRun rJK = wave_run(live ? oJ * n + oK : dead);
+  const WaveRun rJL = wave_run(live ? oJ * n + oL : dead);
+  double dij[NB], aij[NB];
+#pragma unroll
+  for (int mj = 0; mj < NJ; ++mj)
+#pragma unroll
+    for (int mi = 0; mi < NI; ++mi) {
+      dij[mi + NI * mj] = D[(oI + mi) * n + oJ + mj];
+      aij[mi + NI * mj] = 0.0;
+    }
+  double bik[NK][NI], bil[NL][NI], bjk[NK][NJ], bjl[NL][NJ];
+#pragma unroll
+  for (int mi = 0; mi < NI; ++mi) {
+#pragma unroll
+    for (int mk = 0; mk < NK; ++mk) bik[mk][mi] = 0.0;
+#pragma unroll
+    for (int ml = 0; ml < NL; ++ml) bil[ml][mi] = 0.0;
+  }
+#pragma unroll
+  for (int mj = 0; mj < NJ; ++mj) {
+#pragma unroll
+    for (int mk = 0; mk < NK; ++mk) bjk[mk][mj] = 0.0;
+#pragma unroll
+    for (int ml = 0; ml < NL; ++ml) bjl[ml][mj] = 0.0;
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int mk = r % NK, ml = r / NK;
+    const double dkl = D[(oK + mk) * n + oL + ml];
+    double djl[NJ], djk[NJ], dil[NI], dik[NI];
+#pragma unroll
+    for (int mj = 0; mj < NJ; ++mj) {
+      djl[mj] = D[(oJ + mj) * n + oL + ml];
+      djk[mj] = D[(oJ + mj) * n + oK + mk];
+    }
+#pragma unroll
+    for (int mi = 0; mi < NI; ++mi) {
+      dil[mi] = D[(oI + mi) * n + oL + ml];
+      dik[mi] = D[(oI + mi) * n + oK + mk];
+    }
+    double akl = 0.0;
+#pragma unroll
+    for (int mj = 0; mj < NJ; ++mj)
+#pragma unroll
+      for (int mi = 0; mi < NI; ++mi) {
+        const double v = live ? jk_stage[base + (r * NB + mi + NI * mj) * ls] : 0.0;
+        aij[mi + NI * mj] += v * dkl;
+        akl += v * dij[mi + NI * mj];
+        bik[mk][mi] += v * djl[mj];
+        bjk[mk][mj] += v * dil[mi];
+        bil[ml][mi] += v * djk[mj];
+        bjl[ml][mj] += v * dik[mi];
+      }
+    jk_put(live, J, (oK + mk) * n + oL + ml, 2.0 * w * akl, rKL);
+  }
+  // the k and l components the rows reached
+  constexpr int K1 = NR < NK ? NR : NK, L1 = (NR - 1) / NK + 1;
+#pragma unroll
+  for (int mk = 0; mk < K1; ++mk) {
+#pragma unroll
+    for (int mi = 0; mi < NI; ++mi) jk_put(live, K, (oI + mi) * n + oK + mk, w * bik[mk][mi], rIK);
+#pragma unroll
+    for (int mj = 0; mj < NJ; ++mj) jk_put(live, K, (oJ + mj) * n + oK + mk, w * bjk[mk][mj], rJK);
+  }
+#pragma unroll
+  for (int ml = 0; ml < L1; ++ml) {
+#pragma unroll
+    for (int mi = 0; mi < NI; ++mi) jk_put(live, K, (oI + mi) * n + oL + ml, w * bil[ml][mi], rIL);
+#pragma unroll
+    for (int mj = 0; mj < NJ; ++mj) jk_put(live, K, (oJ + mj) * n + oL + ml, w * bjl[ml][mj], rJL);
+  }
+#pragma unroll
+  for (int mj = 0; mj < NJ; ++mj)
+#pragma unroll
+    for (int mi = 0; mi < NI; ++mi)
+      jk_put(live, J, (oI + mi) * n + oJ + mj, 2.0 * w * aij[mi + NI * mj], rIJ);
+}
+
 template <int NI, int NJ, int NK, int NL>
 __device__ __noinline__ void jk_digest_rows_call(int base, int ls, int r0, int nr, bool live,
                                                  int s0, int s1, int s2, int s3, int o0, int o1,
