@@ -663,6 +663,21 @@ static int set_pairdata(PairPrim *pd, const Shl *si, const Shl *sj, int li_ceil,
     return empty;
 }
 
+/* The pair data of one primitive pair (exponents ai, aj at ri, rj, unit
+ * coefficients, s shells): out[0..2] = P, out[3] = exp(-ai aj/(ai+aj)|ri-rj|^2),
+ * out[4] = the screening key cce.  For the known-answer tests of
+ * crates/cintx-cubecl/tests/pdata_tests.rs. */
+void orc_pairdata_one(double ai, double aj, const double *ri, const double *rj, double *out)
+{
+    const double one = 1.0;
+    Shl si = {0, 1, 1, ri, &ai, &one}, sj = {0, 1, 1, rj, &aj, &one};
+    PairPrim pd[1];
+    set_pairdata(pd, &si, &sj, 0, 0, 1e300);
+    for (int d = 0; d < 3; d++) out[d] = (double)pd[0].P[d];
+    out[3] = (double)pd[0].e;
+    out[4] = pd[0].cce;
+}
+
 /* Apply per-axis transforms: src[l][k][j][i] (i fastest) with sizes
  * (ci,cj,ck,cl) -> dst with sizes (si,sj,sk,sl) using T_x[s][c]. */
 static void transform4(real *dst, const real *src, const int lv[4], int spherical, real *work)

@@ -47,6 +47,11 @@ void orc_rys_rule_ld(int nroots, double x, long double *s, long double *w);
  * (cart2sph.c:21 table; p is the identity, cart2sph.c:33-43). */
 double orc_c2s_coef(int l, int m, int c);
 
+/* One primitive pair's data (CINTset_pairdata, optimizer.c:288-342):
+ * out[0..2] = product centre P, out[3] = exp(-ai aj/(ai+aj)|ri-rj|^2),
+ * out[4] = the screening key. */
+void orc_pairdata_one(double ai, double aj, const double *ri, const double *rj, double *out);
+
 /* libcint-signature integrals (cint2e.c:1186, grad2.c:81, cint3c2e.c:693,
  * autocode/int3c2e.c:80).  opt and cache are accepted and ignored. */
 int orc_int2e_sph(double *out, int *dims, int *shls, int *atm, int natm,

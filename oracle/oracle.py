@@ -80,6 +80,15 @@ def boys(mmax, x):
     return f
 
 
+def pairdata_one(ai, aj, ri, rj):
+    """(P[3], exp(-ai aj/(ai+aj) r^2), cce) of one primitive pair."""
+    ri = np.ascontiguousarray(ri, dtype=np.float64)
+    rj = np.ascontiguousarray(rj, dtype=np.float64)
+    out = np.zeros(5)
+    lib().orc_pairdata_one(ctypes.c_double(ai), ctypes.c_double(aj), _dp(ri), _dp(rj), _dp(out))
+    return out[:3], out[3], out[4]
+
+
 def rys_roots(n, x, nodes=0):
     u = np.zeros(n)
     w = np.zeros(n)

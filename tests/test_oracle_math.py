@@ -30,6 +30,40 @@ def test_boys_against_incomplete_gamma(oracle, x):
         assert abs(f[m] - ref) <= 4e-16 * ref, (m, x)
 
 
+def test_boys_against_reference_golden(oracle):
+    """The golden F_0..F_4 at x = 1 of the reference's own Boys test
+    (fixture tests/golden/boys_golden.json, written by make_boys_fixture.py
+    from crates/cintx-cubecl/tests/boys_tests.rs): the oracle within the
+    reference's 1e-14, and the golden values themselves (FP64 results of
+    libcint's series, printed to 20 digits) within 2e-16 relative of
+    mpmath."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "boys_golden.json")
+    g = json.load(open(path))
+    f = oracle.boys(len(g["F"]) - 1, g["x"])
+    for k, v in enumerate(g["F"]):
+        assert abs(f[k] - float(v)) < g["tolerance"], (k, f[k], v)
+        ref = boys_mp(k, g["x"])
+        assert abs(mp.mpf(v) - ref) < 2e-16 * ref, k
+
+
+def test_pairdata_against_reference_known_answers(oracle):
+    """The oracle's primitive-pair data against the known answers of the
+    reference's pair-data tests (crates/cintx-cubecl/tests/pdata_tests.rs:
+    pdata_equal_exponents, pdata_asymmetric, pdata_coincident, at their
+    1e-12): product centre P and exp(-ai aj/(ai+aj) |Rij|^2)."""
+    cases = [  # (ai, aj, Ri, Rj, P, factor)
+        (1.0, 1.0, (0.0, 0.0, 0.0), (1.4, 0.0, 0.0), (0.7, 0.0, 0.0), math.exp(-0.98)),
+        (5.0, 2.0, (0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (2.0 / 7.0, 0.0, 0.0), math.exp(-10.0 / 7.0)),
+        (3.0, 4.0, (1.0, 2.0, 3.0), (1.0, 2.0, 3.0), (1.0, 2.0, 3.0), 1.0),
+    ]
+    for ai, aj, ri, rj, pw, fw in cases:
+        P, fac, _ = oracle.pairdata_one(ai, aj, ri, rj)
+        assert np.max(np.abs(P - np.array(pw))) < 1e-12, (ai, aj, P)
+        assert abs(fac - fw) < 1e-12, (ai, aj, fac)
+
+
 def test_boys_zero_argument(oracle):
     f = oracle.boys(10, 0.0)
     assert np.allclose(f, [1.0 / (2 * m + 1) for m in range(11)], rtol=0, atol=0)
