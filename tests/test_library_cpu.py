@@ -147,6 +147,12 @@ def test_rys_table_moment_equations():
                 assert abs(val - ref) <= 2e-15 * (m + 1) * ref, (n, x, m)
 
 
+def test_gto_norm_reference_known_answer():
+    """The generator's CINTgto_norm (misc.c:86) against the known answer of
+    crates/cintx-oracle/tests/cintgto_norm_parity.rs:31-38: (l=0, a=0.5)."""
+    assert abs(M.gto_norm(0, 0.5) - 1.502251088929885) < 1e-12
+
+
 def test_generated_molecules_have_the_quoted_sizes():
     sizes = {1: (3, 24), 2: (64, 509), 3: (256, 2819), 5: (32, 1088)}
     for cfg, (natm, nao) in sizes.items():
