@@ -133,15 +133,15 @@ __device__ void generic_ket(double* __restrict__ g, const GenShape& S, int axis,
 // per element need the larger register budget); a block with more rows than
 // one chunk (256 RPL rows) is done chunk by chunk, each chunk a full pass
 // over the primitive tuples with the roots and chains recomputed.  WIDE:
-// such a block runs with about 48 accumulators per lane at one workgroup
-// per CU (the register budget of one wave per SIMD), so a (g f|g p) block
-// takes 3 chunks instead of 9.
+// such a block runs with about 48 accumulators per lane (nabla_i: 90) at
+// one workgroup per CU (the register budget of one wave per SIMD), so a
+// (g f|g p) block takes 3 chunks instead of 9.
 constexpr size_t kGenLdsBytes = 150 * 1024;  // dynamic LDS for g (gfx950: 160 KB per CU)
 
-// rows per lane: about 16 accumulators (nabla_i, and WIDE: 45-48, with twice
-// the registers)
+// rows per lane: about 16 accumulators (nabla_i: 45, WIDE: 48, WIDE nabla_i:
+// 90, each with one workgroup per CU)
 __host__ __device__ constexpr int gen_rows_per_lane(int li, bool deriv, bool wide = false) {
-  return deriv ? cmax(1, 15 / ncart(li)) : cmax(1, (wide ? 48 : 16) / ncart(li));
+  return deriv ? cmax(1, (wide ? 30 : 15) / ncart(li)) : cmax(1, (wide ? 48 : 16) / ncart(li));
 }
 
 template <bool GLDS, bool DERIV, int LI, bool WIDE = false>
@@ -402,13 +402,17 @@ hipError_t launch_generic(const EriArgs& a, const int* l, int deriv, int grid, h
   const size_t gbytes = (size_t)3 * S.g4 * S.nr * sizeof(double);
   const bool glds = gbytes <= kGenLdsBytes;
   const size_t dyn = glds ? gbytes : 0;
-  // more rows than one chunk of the narrow rows: the wide kernel
+  // more rows than one chunk of the narrow rows: the wide kernel (nabla_i
+  // with an s shell i keeps the narrow one: 30 rows per lane spill)
   const int nrow = ncart(l[1]) * ncart(l[2]) * ncart(l[3]);
-  const bool wide = !deriv && nrow > 256 * gen_rows_per_lane(l[0], false);
+  const bool wide = nrow > 256 * gen_rows_per_lane(l[0], deriv != 0) && !(deriv && l[0] == 0);
   switch (l[0]) {
 #define CX_GEN_LAUNCH(LI)                                                                     \
   case LI:                                                                                    \
-    if (glds && deriv)                                                                        \
+    if (glds && deriv && wide)                                                                \
+      hipLaunchKernelGGL((eri_generic_kernel<true, true, LI, true>), dim3(grid), dim3(256), dyn, \
+                         s, a, S, per);                                                       \
+    else if (glds && deriv)                                                                   \
       hipLaunchKernelGGL((eri_generic_kernel<true, true, LI>), dim3(grid), dim3(256), dyn, s, a, \
                          S, per);                                                             \
     else if (glds && wide)                                                                    \
