@@ -133,15 +133,17 @@ __device__ void generic_ket(double* __restrict__ g, const GenShape& S, int axis,
 // per element need the larger register budget); a block with more rows than
 // one chunk (256 RPL rows) is done chunk by chunk, each chunk a full pass
 // over the primitive tuples with the roots and chains recomputed.  WIDE:
-// such a block runs with about 48 accumulators per lane (nabla_i: 90) at
+// such a block runs with about 60 accumulators per lane (nabla_i: 90) at
 // one workgroup per CU (the register budget of one wave per SIMD), so a
 // (g f|g p) block takes 3 chunks instead of 9.
 constexpr size_t kGenLdsBytes = 150 * 1024;  // dynamic LDS for g (gfx950: 160 KB per CU)
 
-// rows per lane: about 16 accumulators (nabla_i: 45, WIDE: 48, WIDE nabla_i:
-// 90, each with one workgroup per CU)
+// rows per lane: about 16 accumulators (nabla_i: 45, WIDE: 60 (48 for an s
+// shell i, whose 60 rows spill), WIDE nabla_i: 90, each with one workgroup
+// per CU)
 __host__ __device__ constexpr int gen_rows_per_lane(int li, bool deriv, bool wide = false) {
-  return deriv ? cmax(1, (wide ? 30 : 15) / ncart(li)) : cmax(1, (wide ? 48 : 16) / ncart(li));
+  return deriv ? cmax(1, (wide ? 30 : 15) / ncart(li))
+               : cmax(1, (wide ? (li > 0 ? 60 : 48) : 16) / ncart(li));
 }
 
 template <bool GLDS, bool DERIV, int LI, bool WIDE = false>
