@@ -284,8 +284,11 @@ int cintx_rys_eval_device(int n, const double *x, int64_t nx, double *u, double 
  * unique 8-fold shell quartets (each unordered quartet once) of segmented
  * shells: J_pq = sum_rs (pq|rs) D_rs, K_pr = sum_qs (pq|rs) D_qs for a
  * symmetric density D (nao x nao, row-major, host memory; a non-symmetric D
- * is replaced by (D + D^T) / 2).  No counterpart in the reference: the
- * Fock-like consumer of the unique-quartet workload. */
+ * is replaced by (D + D^T) / 2).  The class kernels digest their blocks
+ * into J and K in their epilogue (no ERI block of a compiled class reaches
+ * memory; runtime-class bins are written and read back).  No counterpart in
+ * the reference's API: the consumer of the unique-quartet workload whose
+ * loop libcint-master/examples/time_c60.c:196-227 shows. */
 int cintx_plan_jk(cintx_plan *plan, const double *D, double *J, double *K);
 
 /* The same J/K build for callers that cannot allocate device memory (code
