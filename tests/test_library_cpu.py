@@ -153,6 +153,39 @@ def test_gto_norm_reference_known_answer():
     assert abs(M.gto_norm(0, 0.5) - 1.502251088929885) < 1e-12
 
 
+def test_build_variant_rules():
+    """The launch-shape rules of the f/g templates (cintx_amd/build.py):
+    classes within the LDS-staging and chain-register limits, kets pairing an
+    f/g shell only with s or p, a per-primitive bra base (PP bit 13) exactly
+    where the bra pairs an f/g shell with l >= 2, i-row shapes (PP bit 8) for
+    bras wider than (d d|, and no one-lane chunked variant (compile time)."""
+    from cintx_amd import build as B
+    nc = [(l + 1) * (l + 2) // 2 for l in range(5)]
+    hl = B.high_l_classes()
+    assert len(hl) == 71 and len(B.high_l_nabla_classes()) == 12
+    for ls in hl:
+        li, lj, lk, ll = ls
+        assert max(ls) >= 3 and (li, lj) >= (lk, ll) and li >= lj and lk >= ll
+        assert nc[li] * nc[lj] * nc[lk] * nc[ll] <= B.HIGH_L_MAX_NF
+        assert (li + 1) * (lj + 1) * (lk + 1) * (ll + 1) <= B.HIGH_L_MAX_G4
+        assert lk < 3 or ll < 2
+        v = B.variants(0, 0, ls)
+        assert 1 <= len(v) <= 6
+        pb = max(li, lj) >= 3 and min(li, lj) >= 2
+        for tt, pp in v:
+            assert bool(pp & 8192) == pb, (ls, tt, pp)
+            assert not (tt == 1 and (pp & 255) > 1), (ls, tt, pp)
+            if pb:
+                assert tt > 1
+            if nc[li] * nc[lj] > 36:
+                assert tt > 1 and pp & 256, (ls, tt, pp)
+    # the classes below the f/g set keep their shapes (no bit 13, no change
+    # of the J/K and Cartesian copies)
+    for deriv, prec, ls in B.class_sets():
+        if not B.is_high_l(deriv, ls):
+            assert all(not (pp & 8192) for _, pp in B.variants(deriv, prec, ls))
+
+
 def test_generated_molecules_have_the_quoted_sizes():
     sizes = {1: (3, 24), 2: (64, 509), 3: (256, 2819), 5: (32, 1088)}
     for cfg, (natm, nao) in sizes.items():
